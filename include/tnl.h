@@ -1,0 +1,144 @@
+/*
+ * tnl.h -- C ABI of the MI355X gridding library (libtnl.so).
+ *
+ * Plain pointers and sizes only; every device pointer is an HBM address
+ * owned by the caller; every call is asynchronous on the HIP stream passed
+ * as `stream` (a hipStream_t, NULL = the null stream).  Calls are reentrant:
+ * the library keeps no mutable global state besides a thread-local error
+ * string.  Every entry point returns TNL_OK (0) or a TNL_E* status; the
+ * message for the last failure on the calling thread is tnl_last_error().
+ *
+ * Complex data is passed as interleaved (re, im) float32, exactly as the
+ * reference passes torch.view_as_real(...) to its Triton kernels.
+ */
+#ifndef TNL_H
+#define TNL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+#define TNL_OK 0
+#define TNL_EINVAL 1      /* bad argument (maps to ValueError)             */
+#define TNL_ESHAPE 2      /* shape mismatch (maps to RuntimeError)         */
+#define TNL_EHIP 3        /* HIP runtime error (launch / memset)           */
+#define TNL_EWORKSPACE 4  /* workspace too small                           */
+
+/* ---- enums mirrored from the reference's string arguments ------------ */
+#define TNL_KERNEL_KAISER_BESSEL 0 /* kernel="kaiser_bessel"             */
+#define TNL_KERNEL_SPLINE 1        /* kernel="spline"                    */
+#define TNL_PAD_CIRCULAR 0         /* pad_mode="circular"                */
+#define TNL_PAD_ZERO 1             /* pad_mode="zero"                    */
+#define TNL_F32 0
+#define TNL_F64 1
+
+/* Largest per-dimension tap count ceil(w)+1 the kernels accept.  The
+ * reference rejects next_pow2(ceil(w+1)) > 64 (doubled for complex data)
+ * in get_block_width (src/torchlinops/functional/_interp/grid.py:145-158). */
+#define TNL_MAX_TAPS 64
+
+const char* tnl_last_error(void);
+int tnl_abi_version(void);
+
+/* ---- point-binning plan --------------------------------------------------
+ * A plan sorts the nonuniform points of one trajectory into spatial tiles
+ * so that grid/ungrid read and write HBM tile by tile.  It depends only on
+ * (locs, ndim, grid_size, width, pad_mode), never on values, so one plan
+ * serves every coil, every CG iteration and both directions.
+ * No reference counterpart: the reference's Triton kernels walk points in
+ * input order (src/torchlinops/functional/_interp/ungrid.py:154-503).
+ */
+typedef struct tnl_plan_layout {
+  int32_t ndim;
+  int32_t ntaps[3];      /* ceil(w_d)+1 per dim (dims left-padded to 3)      */
+  /* interpolation bins: each point goes to the bin of its first tap       */
+  int32_t u_tile[3];     /* bin edge per dim (cells)                         */
+  int32_t u_ntiles[3];
+  int64_t u_nbins;       /* prod(u_ntiles); bin u_nbins holds dead points    */
+  /* gridding tiles: each tile lists every point whose taps touch it      */
+  int32_t g_tile[3];
+  int32_t g_ntiles[3];
+  int64_t g_ntiles_total;
+  int64_t npts;
+  int64_t max_sub;       /* points per interpolation subproblem              */
+  int64_t sub_capacity;  /* upper bound on interpolation subproblems         */
+  int64_t g_cap;         /* touches per gridding work item                   */
+  int64_t touch_capacity;/* upper bound on (tile, point) touches             */
+  int64_t item_capacity; /* upper bound on gridding work items               */
+  /* byte offsets inside the workspace                                      */
+  int64_t off_counts, off_starts, off_cursor, off_perm, off_slocs;
+  int64_t off_subk, off_suboff, off_sub;
+  int64_t off_gcounts, off_gstarts, off_gcursor, off_tlist;
+  int64_t off_itemk, off_itemoff, off_items;
+  int64_t bytes;         /* total workspace bytes                             */
+} tnl_plan_layout;
+
+int tnl_plan_layout_query(int64_t npts, int ndim, const int64_t* grid_size,
+                          const float* width, int pad_mode,
+                          tnl_plan_layout* out);
+
+/* Build the plan into a caller-owned device workspace of layout.bytes. */
+int tnl_plan_build(const float* locs, int64_t npts, int ndim,
+                   const int64_t* grid_size, const float* width, int pad_mode,
+                   void* workspace, int64_t workspace_bytes, void* stream);
+
+/* ---- interpolation (ungrid) -------------------------------------------
+ * out[b, p] = sum_{g in taps(p)} w(t_p - g) * vals[b, g]
+ * Replaces UNGRID[ndim][grid](vals, locs, output, nbatch, npts, kernel,
+ * norm, pad_mode, is_complex, *grid_size, *width, *BLOCK_WIDTH, beta)
+ * at src/torchlinops/functional/_interp/ungrid.py:103-117.
+ * vals: [nbatch, *grid_size] (x2 floats if complex); locs: [npts, ndim];
+ * out: [nbatch, npts] (x2 if complex), fully written by the call.
+ * plan: NULL, or a workspace built by tnl_plan_build for the same locs.
+ */
+int tnl_ungrid(const float* vals, const float* locs, float* out,
+               int64_t nbatch, int64_t npts, int ndim,
+               const int64_t* grid_size, const float* width, int kernel,
+               int norm, int pad_mode, int is_complex, float beta,
+               const void* plan, int64_t plan_bytes, void* stream);
+
+/* ---- gridding (interpolate adjoint) ------------------------------------
+ * out[b, g] = sum_{p : g in taps(p)} w(t_p - g) * vals[b, p]
+ * Replaces GRID[ndim][grid](...) at
+ * src/torchlinops/functional/_interp/grid.py:107-122.  The reference's
+ * caller zero-fills `out` first (grid.py:96-101); here the call writes
+ * every element of out itself, so the caller may pass uninitialised memory.
+ */
+int tnl_grid(const float* vals, const float* locs, float* out, int64_t nbatch,
+             int64_t npts, int ndim, const int64_t* grid_size,
+             const float* width, int kernel, int norm, int pad_mode,
+             int is_complex, float beta, const void* plan, int64_t plan_bytes,
+             void* stream);
+
+/* ---- array_to_blocks (unfold) -------------------------------------------
+ * y[n, b..., j...] = x[n, b*s + j]  (pure copy, bit-exact)
+ * Replaces UNFOLD[ndim][grid](x, y, nbatch, *nblocks, *block_size,
+ * *im_size, *stride, *BLOCK_SIZE) at
+ * src/torchlinops/functional/_unfold/unfold.py:117-126.  Sizes are those
+ * after the reference's complex interleave (last dim, block and stride x2,
+ * unfold.py:581-587).  elem_bytes in {1, 2, 4, 8}; y is fully written.
+ */
+int tnl_unfold(const void* x, void* y, int elem_bytes, int64_t nbatch,
+               int ndim, const int64_t* nblocks, const int64_t* block_size,
+               const int64_t* im_size, const int64_t* stride, void* stream);
+
+/* ---- blocks_to_array (fold) ---------------------------------------------
+ * y[n, i] = sum over blocks covering i, in lexicographic block order, of
+ * x[n, b, i - b*s]; uncovered cells are 0.  Bit-exact against the
+ * reference's sequential `out[slc] += x[blk]` (fold.py:489-517).
+ * Replaces FOLD[ndim][grid](x, y, nbatch, *nblocks, *block_size, *im_size,
+ * *stride) at src/torchlinops/functional/_unfold/fold.py:105-113.
+ * dtype: TNL_F32 or TNL_F64; y is fully written.
+ */
+int tnl_fold(const void* x, void* y, int dtype, int64_t nbatch, int ndim,
+             const int64_t* nblocks, const int64_t* block_size,
+             const int64_t* im_size, const int64_t* stride, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TNL_H */

@@ -1,0 +1,159 @@
+"""GPU parity of interpolate / interpolate_adjoint against the oracle.
+
+Bar: relative L2 error <= 1e-5 against the oracle's reference-order float32
+restatement, and the error against the float64 oracle is asserted (and
+printed) under the same bound.  Both the binned (plan) kernels and the
+direct kernels are exercised.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests._util import crandn, radial_3d, rel_l2, to_dev
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+BETA = float(O.nufft_beta(4.0, 2.0))
+
+
+def _run(dev, grid_size, locs, nbatch, width, kernel, pad, norm, beta, plan, cplx, seed):
+    import torchlinops_amd.functional as F
+    from torchlinops_amd import config
+
+    rng = np.random.default_rng(seed)
+    shape = (nbatch, *grid_size)
+    x = crandn(rng, shape) if cplx else rng.standard_normal(shape).astype(np.float32)
+    npts = locs.shape[0]
+    y = crandn(rng, (nbatch, npts)) if cplx else rng.standard_normal((nbatch, npts)).astype(np.float32)
+    kp = {"beta": beta}
+    with config.using(interp_plan=plan):
+        got_u = F.interpolate(to_dev(x, dev), to_dev(locs, dev), width, kernel, norm, pad, kp)
+        got_g = F.interpolate_adjoint(to_dev(y, dev), to_dev(locs, dev), grid_size, width,
+                                      kernel, norm, pad, kp)
+    torch.cuda.synchronize()
+    got_u, got_g = got_u.cpu().numpy(), got_g.cpu().numpy()
+    ref_u = O.ungrid(x, locs, width, kernel, norm, pad, beta, "f32")
+    ref_g = O.grid(y, locs, grid_size, width, kernel, norm, pad, beta, "f32")
+    ref_u64 = O.ungrid(x, locs, width, kernel, norm, pad, beta, "f64")
+    ref_g64 = O.grid(y, locs, grid_size, width, kernel, norm, pad, beta, "f64")
+    e = dict(u32=rel_l2(got_u, ref_u), g32=rel_l2(got_g, ref_g),
+             u64=rel_l2(got_u, ref_u64), g64=rel_l2(got_g, ref_g64))
+    print(f"grid={grid_size} npts={npts} nb={nbatch} w={width} {kernel} {pad} norm={norm} "
+          f"plan={plan} cplx={cplx}: {e}")
+    for k, v in e.items():
+        assert v <= TOL, (k, v)
+    return got_u, got_g
+
+
+CASES = [
+    # grid_size, npts, nbatch, width, kernel, pad, norm
+    ((64,), 300, 2, 4.0, "kaiser_bessel", "circular", 1),
+    ((50,), 200, 1, 5.5, "spline", "zero", 1),
+    ((32, 40), 2000, 3, 4.0, "kaiser_bessel", "circular", 1),
+    ((33, 31), 1500, 2, (3.0, 6.0), "kaiser_bessel", "zero", 1),
+    ((24, 24), 800, 1, 2.0, "spline", "circular", 1),
+    ((20, 24), 500, 2, 4.0, "spline", "circular", 2),
+    ((16, 18, 20), 1500, 2, 4.0, "kaiser_bessel", "circular", 1),
+    ((16, 16, 16), 1000, 5, (4.0, 3.0, 7.0), "kaiser_bessel", "zero", 1),
+    ((12, 10, 14), 400, 1, 4.0, "spline", "zero", 2),
+    ((9, 10, 11), 300, 2, 9.5, "kaiser_bessel", "circular", 1),
+    ((6, 5, 7), 300, 3, 4.0, "kaiser_bessel", "circular", 1),  # grid smaller than taps*2
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}-{c[4]}-{c[5]}-n{c[6]}")
+@pytest.mark.parametrize("plan", ["never", "always"])
+@pytest.mark.parametrize("cplx", [True, False])
+def test_interp_parity(dev, case, plan, cplx):
+    grid_size, npts, nbatch, width, kernel, pad, norm = case
+    rng = np.random.default_rng(11)
+    ndim = len(grid_size)
+    locs = rng.uniform(-3, np.array(grid_size) + 3, (npts, ndim)).astype(np.float32)
+    locs[:7] = np.round(locs[:7])  # integer coordinates: w+1 taps
+    _run(dev, grid_size, locs, nbatch, width, kernel, pad, norm, 1.7, plan, cplx, seed=12)
+
+
+def test_c1_shape_full_size(dev):
+    """2-D 128^2 complex64, 16384 uniform points, KB w=4, beta=8.996."""
+    rng = np.random.default_rng(1)
+    locs = rng.uniform(0, 128, (16384, 2)).astype(np.float32)
+    for plan in ("never", "always"):
+        _run(dev, (128, 128), locs, 1, 4.0, "kaiser_bessel", "circular", 1, BETA, plan, True, 2)
+
+
+def test_radial_3d_reduced(dev):
+    """C2's trajectory family at reduced size: 3-D radial, 2 coils, 48^3."""
+    rng = np.random.default_rng(2)
+    locs = radial_3d(300, 48, 48, rng).reshape(-1, 3)
+    for plan in ("never", "always"):
+        _run(dev, (48, 48, 48), locs, 2, 4.0, "kaiser_bessel", "circular", 1, BETA, plan, True, 3)
+
+
+def test_dense_tiles_split(dev):
+    """Many points in one gridding tile: the tile is split across workgroups
+    (shared tiles combined with global atomics)."""
+    rng = np.random.default_rng(4)
+    locs = (rng.standard_normal((60000, 2)) * 1.5 + 30).astype(np.float32)
+    _run(dev, (64, 64), locs, 2, 4.0, "kaiser_bessel", "circular", 1, BETA, "always", True, 5)
+    locs3 = (rng.standard_normal((40000, 3)) * 1.0 + 10).astype(np.float32)
+    _run(dev, (24, 24, 32), locs3, 3, 4.0, "kaiser_bessel", "zero", 1, BETA, "always", True, 6)
+
+
+def test_empty_and_edge_inputs(dev):
+    import torchlinops_amd.functional as F
+
+    x = torch.randn(2, 16, 16, dtype=torch.complex64, device=dev)
+    locs = torch.zeros(0, 2, device=dev)
+    assert F.interpolate(x, locs, 4.0).shape == (2, 0)
+    g = F.interpolate_adjoint(torch.zeros(2, 0, dtype=torch.complex64, device=dev), locs, (16, 16), 4.0)
+    assert g.shape == (2, 16, 16) and bool((g == 0).all())
+    # non-finite coordinates contribute nothing
+    bad = torch.tensor([[float("nan"), 3.0], [2.0, float("inf")]], device=dev)
+    assert bool((F.interpolate(x, bad, 4.0) == 0).all())
+    assert bool((F.interpolate_adjoint(torch.ones(2, 2, dtype=torch.complex64, device=dev), bad,
+                                       (16, 16), 4.0) == 0).all())
+    # batch shapes: [..B, *grid] with locs [..L, D]
+    x5 = torch.randn(2, 3, 16, 16, dtype=torch.complex64, device=dev)
+    l5 = torch.rand(4, 5, 2, device=dev) * 16
+    out = F.interpolate(x5, l5, 4.0)
+    assert out.shape == (2, 3, 4, 5)
+    back = F.interpolate_adjoint(out, l5, (16, 16), 4.0)
+    assert back.shape == (2, 3, 16, 16)
+
+
+def test_autograd_pair(dev):
+    """vals.grad of interpolate is interpolate_adjoint(ones) and vice versa
+    (the reference's test_interp_diff.py)."""
+    import torchlinops_amd.functional as F
+
+    locs = torch.rand(700, 2, device=dev) * 32
+    x = torch.randn(2, 32, 32, dtype=torch.complex64, device=dev, requires_grad=True)
+    y = F.interpolate(x, locs, 4.0)
+    y.real.sum().backward()
+    want = F.interpolate_adjoint(torch.ones_like(y).real.to(y.dtype), locs, (32, 32), 4.0)
+    assert torch.allclose(x.grad, want, rtol=1e-5, atol=1e-5)
+    v = torch.randn(2, 700, dtype=torch.complex64, device=dev, requires_grad=True)
+    g = F.interpolate_adjoint(v, locs, (32, 32), 4.0)
+    g.real.sum().backward()
+    want2 = F.interpolate(torch.ones(2, 32, 32, dtype=torch.complex64, device=dev), locs, 4.0)
+    assert torch.allclose(v.grad, want2, rtol=1e-5, atol=1e-5)
+
+
+def test_plan_cache_reuse_and_invalidation(dev):
+    import torchlinops_amd.functional as F
+    from torchlinops_amd import config
+
+    rng = np.random.default_rng(8)
+    locs_np = rng.uniform(0, 40, (3000, 2)).astype(np.float32)
+    locs = to_dev(locs_np, dev)
+    x = to_dev(crandn(rng, (1, 40, 40)), dev)
+    with config.using(interp_plan="always"):
+        a = F.interpolate(x, locs, 4.0).cpu().numpy()
+        locs.add_(1.25)  # in-place edit bumps the version counter: new plan
+        b = F.interpolate(x, locs, 4.0).cpu().numpy()
+    ref_a = O.ungrid(x.cpu().numpy(), locs_np, 4.0, beta=1.0)
+    ref_b = O.ungrid(x.cpu().numpy(), locs_np + 1.25, 4.0, beta=1.0)
+    assert rel_l2(a, ref_a) < TOL and rel_l2(b, ref_b) < TOL

@@ -1,0 +1,107 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports
+every symbol include/tnl.h declares, argument validation in the library
+maps to the reference's exception types, and the Python front end raises
+the reference's errors before any device work."""
+
+import re
+from pathlib import Path
+
+import pytest
+import torch
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def header_symbols():
+    text = (ROOT / "include" / "tnl.h").read_text()
+    return sorted(set(re.findall(r"\b(tnl_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_header_symbol():
+    from torchlinops_amd import _lib
+
+    h = _lib.lib()
+    syms = header_symbols()
+    assert len(syms) >= 8
+    for s in syms:
+        assert hasattr(h, s), s
+    assert set(syms) == set(_lib.SIGNATURES)
+    assert h.tnl_abi_version() == 1
+
+
+def test_plan_layout_is_host_only_and_consistent():
+    from torchlinops_amd.functional._plan import plan_layout
+
+    L = plan_layout(10_240_000, 3, (256, 256, 256), (4.0, 4.0, 4.0), 0)
+    assert L.u_nbins == 32 ** 3
+    assert tuple(L.ntaps) == (5, 5, 5)
+    assert tuple(L.g_tile) == (8, 8, 16)
+    assert L.touch_capacity == 8 * 10_240_000
+    offs = [L.off_counts, L.off_starts, L.off_cursor, L.off_perm, L.off_slocs,
+            L.off_suboff, L.off_sub, L.off_gcounts, L.off_gstarts, L.off_gcursor,
+            L.off_tlist, L.off_itemoff, L.off_items]
+    assert all(o % 256 == 0 for o in offs)
+    assert offs == sorted(offs) and L.bytes > offs[-1]
+    # small grid: a single tile along each axis, one touch per point
+    L2 = plan_layout(100, 2, (8, 8), (4.0, 4.0), 0)
+    assert tuple(L2.g_ntiles)[1:] == (1, 1) and L2.touch_capacity == 100
+
+
+def test_library_validation_maps_to_exceptions():
+    from torchlinops_amd import _lib
+
+    h = _lib.lib()
+    g = _lib.i64_array([8, 8])
+    w = _lib.f32_array([4.0, 4.0])
+    with pytest.raises(ValueError, match="kernel"):
+        _lib.check(h.tnl_ungrid(None, None, None, 1, 1, 2, g, w, 7, 1, 0, 0, 1.0, None, 0, None))
+    with pytest.raises(ValueError, match="norm"):
+        _lib.check(h.tnl_grid(None, None, None, 1, 1, 2, g, w, 0, 3, 0, 0, 1.0, None, 0, None))
+    with pytest.raises(ValueError, match="ndim"):
+        _lib.check(h.tnl_ungrid(None, None, None, 1, 1, 4, g, w, 0, 1, 0, 0, 1.0, None, 0, None))
+    big = _lib.f32_array([4.0, 40.0])
+    with pytest.raises(ValueError, match="block width"):
+        _lib.check(h.tnl_ungrid(None, None, None, 1, 1, 2, g, big, 0, 1, 0, 1, 1.0, None, 0, None))
+    nb = _lib.i64_array([0])
+    with pytest.raises(ValueError, match="nblocks"):
+        _lib.check(h.tnl_unfold(None, None, 4, 1, 1, nb, _lib.i64_array([3]),
+                                _lib.i64_array([2]), _lib.i64_array([1]), None))
+    with pytest.raises(RuntimeError, match="inconsistent"):
+        _lib.check(h.tnl_fold(None, None, 0, 1, 1, _lib.i64_array([2]), _lib.i64_array([3]),
+                              _lib.i64_array([9]), _lib.i64_array([1]), None))
+
+
+def test_functional_errors_before_device_work():
+    from torchlinops_amd import functional as F
+
+    vals = torch.zeros(8, 8, dtype=torch.complex64)
+    locs = torch.zeros(5, 2)
+    with pytest.raises(ValueError, match="Unrecognized kernel"):
+        F.interpolate(vals, locs, 4.0, kernel="gauss")
+    with pytest.raises(ValueError, match="same length"):
+        F.interpolate(vals, locs, (4.0, 4.0, 4.0))
+    with pytest.raises(ValueError, match="last dim of locs"):
+        F.interpolate_adjoint(torch.zeros(5), locs, (8, 8, 8), 4.0)
+    with pytest.raises(ValueError, match="block width"):
+        F.interpolate(vals, locs, (4.0, 40.0))
+    with pytest.raises(ValueError, match="Found 0 in nblocks"):
+        F.array_to_blocks(torch.zeros(4, 4), (5, 5), (1, 1))
+    with pytest.raises(ValueError, match="dtypes must match"):
+        F.array_to_blocks(torch.zeros(9, 9), (3, 3), (1, 1), out=torch.zeros(1, dtype=torch.float64))
+    with pytest.raises(RuntimeError, match="Fold expected input"):
+        F.blocks_to_array(torch.zeros(2, 2, 3, 3), (9, 9), (3, 3), (2, 2))
+    # no CPU fallback: the hot ops refuse host tensors
+    with pytest.raises(RuntimeError, match="HIP devices only"):
+        F.interpolate(vals, locs, 4.0)
+    with pytest.raises(RuntimeError, match="HIP devices only"):
+        F.array_to_blocks(torch.zeros(9, 9), (3, 3), (1, 1))
+
+
+def test_get_nblocks_doc_examples():
+    from torchlinops_amd.functional import get_nblocks
+
+    assert get_nblocks((5, 5), (3, 3), (1, 1)) == (3, 3)
+    assert get_nblocks((5, 5), (3, 3), (2, 2)) == (2, 2)
+    assert get_nblocks((6, 6), (3, 3), (2, 2)) == (2, 2)
+    assert get_nblocks((7, 7), (3, 3), (2, 2)) == (3, 3)
+    assert get_nblocks((10, 10), (8, 8), (4, 4)) == (1, 1)

@@ -1,0 +1,283 @@
+"""Pins the oracle (oracle/) before any kernel is compared against it.
+
+Every oracle function is checked against something that does not share its
+code: a closed form, an independent library routine (scipy.special.i0,
+numpy.interp, numpy sliding_window_view, numpy.fft), brute force over every
+grid cell, or an algebraic invariant.  A few tests plant a mistake in a
+copy of the oracle's inputs and check the comparison notices.
+"""
+
+from itertools import product
+from math import prod
+
+import numpy as np
+import pytest
+from scipy.special import i0
+
+from oracle import oracle as O
+
+BETA_NUFFT = O.nufft_beta(4.0, 2.0)  # pi * sqrt(8.2) = 8.996...
+
+
+def kb_exact(x, beta):
+    x = np.asarray(x, dtype=np.float64)
+    return np.where(np.abs(x) <= 1, i0(beta * np.sqrt(np.clip(1 - x * x, 0, None))), 0.0)
+
+
+# ------------------------------------------------------------------ KB
+@pytest.mark.parametrize("beta", [1.0, 2.5, BETA_NUFFT, 14.0])
+def test_kb_matches_scipy_i0(beta):
+    x = np.linspace(-1, 1, 4001)
+    ref = kb_exact(x, beta)
+    got64 = O.kb(x, beta, "f64")
+    x32 = x.astype(np.float32)
+    got32 = O.kb(x32, beta, "f32")
+    ref32 = kb_exact(x32.astype(np.float64), beta)
+    # Abramowitz & Stegun 9.8.1/9.8.2: |rel err| < 1.9e-7
+    assert np.max(np.abs(got64 - ref) / ref) < 2.5e-7
+    assert np.max(np.abs(got32 - ref32) / ref32) < 2e-6
+
+
+def test_kb_edges_and_centre():
+    for beta in (1.0, BETA_NUFFT):
+        assert O.kb([1.0, -1.0], beta, "f64").tolist() == [1.0, 1.0]
+        assert O.kb([1.0, -1.0], beta, "f32").tolist() == [1.0, 1.0]
+        assert abs(O.kb([0.0], beta, "f64")[0] / i0(beta) - 1) < 2.5e-7
+
+
+def test_nufft_beta_closed_form():
+    assert abs(O.nufft_beta(4.0, 2.0) - np.pi * np.sqrt(8.2)) < 1e-12
+    assert abs(O.nufft_beta(4.0, 1.25) - np.pi * np.sqrt((4 / 1.25 * 0.75) ** 2 - 0.8)) < 1e-12
+
+
+# ------------------------------------------------------- interpolation
+def brute_weights(t, grid_size, width, beta, kernel="kaiser_bessel", pad="circular"):
+    """Weight of every grid cell for a point at t: sum over unwrapped integer
+    taps g with |t - g| <= w/2 that land on the cell (brute force, exact I0)."""
+    W = np.zeros(grid_size)
+    axes = []
+    for d, n in enumerate(grid_size):
+        r = width[d] / 2
+        gs = [g for g in range(int(np.floor(t[d] - r)) - 1, int(np.ceil(t[d] + r)) + 2)
+              if abs(t[d] - g) <= r]
+        axes.append(gs)
+    for gvec in product(*axes):
+        w = 1.0
+        cell = []
+        ok = True
+        for d, g in enumerate(gvec):
+            x = (t[d] - g) / (width[d] / 2)
+            w *= kb_exact(x, beta) if kernel == "kaiser_bessel" else max(1 - abs(x), 0)
+            if pad == "zero" and not (0 <= g < grid_size[d]):
+                ok = False
+            cell.append(g % grid_size[d])
+        if ok:
+            W[tuple(cell)] += w
+    return W
+
+
+@pytest.mark.parametrize("grid_size", [(7,), (6, 5), (5, 4, 6), (3, 3)])
+@pytest.mark.parametrize("pad", ["circular", "zero"])
+@pytest.mark.parametrize("kernel", ["kaiser_bessel", "spline"])
+def test_ungrid_grid_vs_brute_force(grid_size, pad, kernel):
+    rng = np.random.default_rng(1)
+    ndim = len(grid_size)
+    width = tuple(rng.choice([2.0, 3.0, 4.0, 3.5]) for _ in range(ndim))
+    npts = 9
+    locs = rng.uniform(-2, np.array(grid_size) + 2, size=(npts, ndim))
+    locs[0] = np.round(locs[0])  # exact integer: w+1 taps with edge weight 1
+    vals = rng.standard_normal((2, *grid_size)) + 1j * rng.standard_normal((2, *grid_size))
+    beta = 2.3
+    got = O.ungrid(vals, locs, width, kernel=kernel, pad_mode=pad, beta=beta, precision="f64")
+    ys = rng.standard_normal((2, npts)) + 1j * rng.standard_normal((2, npts))
+    got_g = O.grid(ys, locs, grid_size, width, kernel=kernel, pad_mode=pad, beta=beta,
+                   precision="f64")
+    want = np.zeros((2, npts), complex)
+    want_g = np.zeros((2, *grid_size), complex)
+    for p in range(npts):
+        W = brute_weights(locs[p], grid_size, width, beta, kernel, pad)
+        want[:, p] = (vals * W).reshape(2, -1).sum(-1)
+        want_g += ys[:, p].reshape(2, *([1] * ndim)) * W
+    assert np.allclose(got, want, rtol=1e-6, atol=1e-9)
+    assert np.allclose(got_g, want_g, rtol=1e-6, atol=1e-9)
+
+
+def test_brute_force_notices_swapped_axes():
+    """Planted mistake: feeding the oracle transposed coordinates must fail."""
+    rng = np.random.default_rng(2)
+    grid_size = (6, 5)
+    locs = rng.uniform(0, 5, size=(5, 2))
+    vals = rng.standard_normal(grid_size)
+    wrong = O.ungrid(vals, locs[:, ::-1].copy(), 4.0, beta=2.0, precision="f64")
+    want = np.array([(vals * brute_weights(l, grid_size, (4.0, 4.0), 2.0)).sum() for l in locs])
+    assert not np.allclose(wrong, want, rtol=1e-3)
+
+
+def test_integer_location_known_answer():
+    """t = 4.0, w = 4: five taps with weights 1, I0(b sqrt(3)/2), I0(b),
+    I0(b sqrt(3)/2), 1 (the case of the reference's test_edgecases.py)."""
+    beta = 1.0
+    vals = np.arange(1, 9, dtype=np.float64)
+    locs = np.array([[4.0]])
+    w = [1.0, i0(beta * np.sqrt(0.75)), i0(beta), i0(beta * np.sqrt(0.75)), 1.0]
+    want = sum(wk * vals[2 + k] for k, wk in enumerate(w))
+    got64 = O.ungrid(vals, locs, 4.0, beta=beta, precision="f64")
+    got32 = O.ungrid(vals.astype(np.float32), locs, 4.0, beta=beta, precision="f32")
+    assert abs(got64[0] / want - 1) < 5e-7
+    assert abs(got32[0] / want - 1) < 2e-6
+    g = O.grid(np.array([1.0]), locs, (8,), 4.0, beta=beta, precision="f64")
+    assert np.allclose(g[2:7], w, rtol=5e-7) and np.all(g[[0, 1, 7]] == 0)
+
+
+def test_spline_width2_is_linear_interpolation():
+    rng = np.random.default_rng(3)
+    n = 16
+    vals = rng.standard_normal(n)
+    t = rng.uniform(0, n, 50)
+    got = O.ungrid(vals, t[:, None], 2.0, kernel="spline", precision="f64")
+    want = np.interp(t, np.arange(n + 1), np.append(vals, vals[0]))
+    assert np.allclose(got, want, atol=1e-12)
+    # 2-D: bilinear
+    v2 = rng.standard_normal((8, 9))
+    pts = rng.uniform(0, 7, (20, 2))
+    got2 = O.ungrid(v2, pts, 2.0, kernel="spline", precision="f64")
+    x0, y0 = np.floor(pts[:, 0]).astype(int), np.floor(pts[:, 1]).astype(int)
+    fx, fy = pts[:, 0] - x0, pts[:, 1] - y0
+    want2 = ((1 - fx) * (1 - fy) * v2[x0 % 8, y0 % 9] + fx * (1 - fy) * v2[(x0 + 1) % 8, y0 % 9]
+             + (1 - fx) * fy * v2[x0 % 8, (y0 + 1) % 9] + fx * fy * v2[(x0 + 1) % 8, (y0 + 1) % 9])
+    assert np.allclose(got2, want2, atol=1e-12)
+
+
+def test_delta_grid_gives_separable_weights():
+    grid_size = (10, 9, 8)
+    vals = np.zeros(grid_size)
+    vals[4, 4, 4] = 1.0
+    t = np.array([[4.3, 3.1, 5.7]])
+    beta = BETA_NUFFT
+    got = O.ungrid(vals, t, 4.0, beta=beta, precision="f64")
+    want = np.prod([kb_exact((t[0, d] - 4) / 2, beta) for d in range(3)])
+    assert abs(got[0] / want - 1) < 1e-6
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_adjointness_f64(ndim):
+    rng = np.random.default_rng(4 + ndim)
+    grid_size = (9, 8, 7)[:ndim]
+    locs = rng.uniform(-3, 12, (40, ndim))
+    x = rng.standard_normal((3, *grid_size)) + 1j * rng.standard_normal((3, *grid_size))
+    y = rng.standard_normal((3, 40)) + 1j * rng.standard_normal((3, 40))
+    Ux = O.ungrid(x, locs, 3.7, beta=5.0, precision="f64")
+    Gy = O.grid(y, locs, grid_size, 3.7, beta=5.0, precision="f64")
+    lhs = np.vdot(y, Ux)
+    rhs = np.vdot(Gy, x)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_f32_oracle_close_to_f64():
+    rng = np.random.default_rng(5)
+    grid_size = (32, 32)
+    locs = rng.uniform(0, 32, (500, 2)).astype(np.float32)
+    x = (rng.standard_normal((2, *grid_size)) + 1j * rng.standard_normal((2, *grid_size))).astype(np.complex64)
+    a = O.ungrid(x, locs, 4.0, beta=BETA_NUFFT, precision="f32")
+    b = O.ungrid(x, locs, 4.0, beta=BETA_NUFFT, precision="f64")
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6
+
+
+# ------------------------------------------------------- unfold / fold
+def sliding_blocks(x, block, stride):
+    ndim = len(block)
+    v = np.lib.stride_tricks.sliding_window_view(x, block, axis=tuple(range(x.ndim - ndim, x.ndim)))
+    slc = (Ellipsis,) + tuple(slice(None, None, s) for s in stride) + (slice(None),) * ndim
+    return v[slc]
+
+
+@pytest.mark.parametrize(
+    "shape,block,stride",
+    [((2, 17), (5,), (3,)), ((1, 12, 11), (4, 3), (2, 2)), ((2, 9, 10, 8), (3, 4, 2), (2, 3, 1)),
+     ((1, 128, 128), (8, 8), (4, 4))],
+)
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.complex64])
+def test_unfold_vs_sliding_window(shape, block, stride, dtype):
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal(shape).astype(dtype)
+    if np.iscomplexobj(x):
+        x = x + 1j * rng.standard_normal(shape).astype(np.float32)
+        x = x.astype(dtype)
+    got = O.unfold(x, block, stride)
+    want = sliding_blocks(x, block, stride)
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("shape,block,stride", [((2, 13), (4,), (3,)), ((1, 9, 8), (3, 3), (2, 1)),
+                                                ((1, 7, 6, 8), (3, 2, 4), (2, 2, 2))])
+def test_fold_brute_force_and_adjoint(shape, block, stride):
+    rng = np.random.default_rng(7)
+    ndim = len(block)
+    im = shape[-ndim:]
+    nbl = O.get_nblocks(im, block, stride)
+    xb = rng.standard_normal((shape[0], *nbl, *block))
+    got = O.fold(xb, im, block, stride)
+    want = np.zeros(shape)
+    for n in range(shape[0]):
+        for b in product(*(range(k) for k in nbl)):
+            slc = tuple(slice(bi * s, bi * s + bl) for bi, s, bl in zip(b, stride, block))
+            want[(n, *slc)] += xb[(n, *b)]
+    assert np.array_equal(got, want)  # same order of additions -> identical
+    x = rng.standard_normal(shape)
+    assert abs(np.vdot(xb, O.unfold(x, block, stride)) - np.vdot(got, x)) < 1e-9
+
+
+def test_fold_is_lexicographic_order_in_f32():
+    """Planted ordering change: summing blocks in reverse order generally
+    changes float32 bits, so bit-exactness pins the order."""
+    rng = np.random.default_rng(8)
+    block, stride, im = (8, 8), (2, 2), (20, 20)
+    nbl = O.get_nblocks(im, block, stride)
+    xb = (rng.standard_normal((1, *nbl, *block)) * 1e3).astype(np.float32)
+    fwd = O.fold(xb, im, block, stride)
+    rev = O.fold(xb[:, ::-1, ::-1].copy(), im, block, stride)  # not the same op
+    flipped = np.zeros_like(fwd)
+    for b in reversed(list(product(*(range(k) for k in nbl)))):
+        slc = tuple(slice(bi * s, bi * s + bl) for bi, s, bl in zip(b, stride, block))
+        flipped[(0, *slc)] += xb[(0, *b)]
+    assert not np.array_equal(fwd, flipped)  # order matters in float32
+    assert rev.shape == fwd.shape
+
+
+# ------------------------------------------------------------- NUFFT
+@pytest.mark.parametrize("grid_size,oversamp", [((16, 16), 2.0), ((12, 10), 1.25), ((8, 8, 8), 2.0)])
+def test_nufft_oracle_approximates_dft(grid_size, oversamp):
+    rng = np.random.default_rng(9)
+    ndim = len(grid_size)
+    locs = rng.uniform(-0.5, 0.5, (60, ndim)) * np.array(grid_size)
+    x = rng.standard_normal(grid_size) + 1j * rng.standard_normal(grid_size)
+    want = O.dft(x, locs)
+    got = O.nufft(x, locs, oversamp=oversamp, width=4.0)
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    assert err < (2e-2 if oversamp < 2 else 5e-3), err
+    y = rng.standard_normal(60) + 1j * rng.standard_normal(60)
+    # adjoint of the same operator
+    lhs = np.vdot(y, got)
+    rhs = np.vdot(O.nufft_adjoint(y, locs, grid_size, oversamp=oversamp), x)
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+def test_pad_rules():
+    assert O.pad_amounts(4, 9) == (2, 3)  # even image, odd pad
+    assert O.pad_amounts(5, 10) == (3, 2)  # odd image, odd pad
+    assert O.pad_amounts(4, 8) == (2, 2)
+    x = np.arange(20.0).reshape(4, 5)
+    assert np.array_equal(O.center_crop(O.center_pad(x, (4, 5), (9, 10)), (9, 10), (4, 5)), x)
+
+
+# ----------------------------------------------------------------- CG
+def test_cg_known_answers():
+    rng = np.random.default_rng(10)
+    y = rng.standard_normal(7) + 1j * rng.standard_normal(7)
+    x = O.conjugate_gradients(lambda v: 2 * v, y, max_num_iters=3)
+    assert np.allclose(x, y / 2)
+    M = rng.standard_normal((6, 6)) + 1j * rng.standard_normal((6, 6))
+    A = M.conj().T @ M + 0.5 * np.eye(6)
+    x = O.conjugate_gradients(lambda v: A @ v, y[:6], max_num_iters=6)
+    assert np.allclose(x, np.linalg.solve(A, y[:6]), rtol=1e-8, atol=1e-10)

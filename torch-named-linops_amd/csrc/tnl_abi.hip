@@ -1,0 +1,143 @@
+// extern "C" entry points for interpolation / gridding and error plumbing.
+#include <cstring>
+
+#include "tnl_common.hpp"
+#include "tnl_internal.hpp"
+
+namespace tnl {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+int check_hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return TNL_OK;
+  return fail(TNL_EHIP,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Validates the arguments shared by tnl_ungrid / tnl_grid and fills the
+// launch geometry.  Mirrors the host-side checks of the reference:
+// unknown kernel (kernels.py:97-105), norm not in {1, 2} (kernels.py:137),
+// block width above 64 (grid.py:145-158).
+static int make_geom(int ndim, const int64_t* grid_size, const float* width,
+                     int kernel, int norm, int pad_mode, int is_complex,
+                     float beta, InterpGeom& G) {
+  if (ndim < 1 || ndim > 3) return fail(TNL_EINVAL, "ndim must be 1, 2 or 3");
+  if (kernel != TNL_KERNEL_KAISER_BESSEL && kernel != TNL_KERNEL_SPLINE)
+    return fail(TNL_EINVAL, "Unrecognized kernel");
+  if (norm != 1 && norm != 2) return fail(TNL_EINVAL, "Unexpected norm");
+  if (pad_mode != TNL_PAD_CIRCULAR && pad_mode != TNL_PAD_ZERO)
+    return fail(TNL_EINVAL, "Unrecognized pad_mode");
+  if (!grid_size || !width) return fail(TNL_EINVAL, "null size pointer");
+  std::memset(&G, 0, sizeof(G));
+  G.ndim = ndim;
+  G.kernel = kernel;
+  G.norm = norm;
+  G.pad = pad_mode;
+  G.beta = beta;
+  for (int d = 0; d < 3; ++d) {
+    G.n[d] = 1;
+    G.half[d] = 0.5f;
+    G.inv_half[d] = 2.0f;
+    G.ntap[d] = 1;
+  }
+  for (int d = 0; d < ndim; ++d) {
+    const int gd = 3 - ndim + d;
+    if (grid_size[d] < 1) return fail(TNL_EINVAL, "grid_size must be >= 1");
+    const float w = width[d];
+    if (!(w > 0.0f) || !std::isfinite(w))
+      return fail(TNL_EINVAL, "kernel width must be positive and finite");
+    // reference: next_pow2(ceil(w + 1)) <= 64, last dim doubled if complex
+    int64_t bw = 1;
+    while (bw < (int64_t)std::ceil((double)w + 1.0)) bw *= 2;
+    if (is_complex && d == ndim - 1) bw *= 2;
+    if (bw > TNL_MAX_TAPS)
+      return fail(TNL_EINVAL,
+                  "Necessary block width exceeds maximum width 64 (kernel "
+                  "width is doubled in last dim if input is complex)");
+    G.n[gd] = grid_size[d];
+    G.half[gd] = w * 0.5f;
+    G.inv_half[gd] = 1.0f / (w * 0.5f);
+    G.ntap[gd] = (int)std::ceil((double)w) + 1;
+  }
+  G.cstride[2] = 1;
+  G.cstride[1] = G.n[2];
+  G.cstride[0] = G.n[1] * G.n[2];
+  G.ncells = G.n[0] * G.n[1] * G.n[2];
+  return TNL_OK;
+}
+
+static int interp_entry(bool adjoint, const float* vals, const float* locs,
+                        float* out, int64_t nbatch, int64_t npts, int ndim,
+                        const int64_t* grid_size, const float* width,
+                        int kernel, int norm, int pad_mode, int is_complex,
+                        float beta, const void* plan, int64_t plan_bytes,
+                        void* stream) {
+  InterpGeom G;
+  if (int rc = make_geom(ndim, grid_size, width, kernel, norm, pad_mode,
+                         is_complex, beta, G))
+    return rc;
+  if (nbatch < 0 || npts < 0)
+    return fail(TNL_EINVAL, "nbatch and npts must be >= 0");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t ncomp = is_complex ? 2 : 1;
+  if (nbatch == 0) return TNL_OK;
+  if (!out) return fail(TNL_EINVAL, "null output pointer");
+  if (adjoint) {
+    if (npts == 0) return fill_zero(out, nbatch * G.ncells * ncomp, s);
+  } else if (npts == 0) {
+    return TNL_OK;
+  }
+  if (!vals || !locs) return fail(TNL_EINVAL, "null input pointer");
+  if (plan) {
+    tnl_plan_layout L;
+    if (int rc = plan_layout(npts, ndim, grid_size, width, pad_mode, &L))
+      return rc;
+    if (plan_bytes < L.bytes)
+      return fail(TNL_EWORKSPACE, "plan workspace smaller than its layout");
+    if (binned_supported(G, L))
+      return interp_binned(adjoint, vals, out, nbatch, G, is_complex, L, plan,
+                           s);
+  }
+  if (adjoint) {
+    if (int rc = fill_zero(out, nbatch * G.ncells * ncomp, s)) return rc;
+  }
+  return interp_direct(adjoint, vals, locs, out, nbatch, npts, G, is_complex,
+                       s);
+}
+
+}  // namespace tnl
+
+using namespace tnl;
+
+extern "C" const char* tnl_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int tnl_abi_version(void) { return 1; }
+
+extern "C" int tnl_ungrid(const float* vals, const float* locs, float* out,
+                          int64_t nbatch, int64_t npts, int ndim,
+                          const int64_t* grid_size, const float* width,
+                          int kernel, int norm, int pad_mode, int is_complex,
+                          float beta, const void* plan, int64_t plan_bytes,
+                          void* stream) {
+  return interp_entry(false, vals, locs, out, nbatch, npts, ndim, grid_size,
+                      width, kernel, norm, pad_mode, is_complex, beta, plan,
+                      plan_bytes, stream);
+}
+
+extern "C" int tnl_grid(const float* vals, const float* locs, float* out,
+                        int64_t nbatch, int64_t npts, int ndim,
+                        const int64_t* grid_size, const float* width,
+                        int kernel, int norm, int pad_mode, int is_complex,
+                        float beta, const void* plan, int64_t plan_bytes,
+                        void* stream) {
+  return interp_entry(true, vals, locs, out, nbatch, npts, ndim, grid_size,
+                      width, kernel, norm, pad_mode, is_complex, beta, plan,
+                      plan_bytes, stream);
+}

@@ -1,0 +1,122 @@
+// Shared device helpers and host error plumbing for libtnl.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <cstring>
+#include <cstdio>
+#include <string>
+
+#include "tnl.h"
+
+namespace tnl {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int check_hip(hipError_t e, const char* what);
+
+#define TNL_LAUNCH_CHECK(what)                                     \
+  do {                                                             \
+    hipError_t _e = hipGetLastError();                             \
+    if (_e != hipSuccess) return ::tnl::check_hip(_e, what);       \
+  } while (0)
+
+// ------------------------------------------------------- kernel weights
+// Kaiser-Bessel window I0(beta * sqrt(1 - x^2)), unnormalised, |x| <= 1.
+// I0 is the Abramowitz & Stegun 9.8.1 / 9.8.2 polynomial approximation
+// (Handbook of Mathematical Functions, p. 378), switching at 3.75, as in
+// src/torchlinops/functional/_interp/kernels.py:50-89.  Callers guarantee
+// |x| <= 1 (clamped), so sqrt never sees a negative argument.
+__device__ __forceinline__ float kb_window(float x, float beta) {
+  const float z = beta * sqrtf(fmaxf(1.0f - x * x, 0.0f));
+  if (z < 3.75f) {
+    const float t = z / 3.75f;
+    const float q = t * t;
+    return 1.0f +
+           q * (3.5156229f +
+                q * (3.0899424f +
+                     q * (1.2067492f +
+                          q * (0.2659732f + q * (0.0360768f + q * 0.0045813f)))));
+  }
+  const float r = 3.75f / z;  // 1/t
+  const float poly =
+      0.39894228f +
+      r * (0.01328592f +
+           r * (0.00225319f +
+                r * (-0.00157565f +
+                     r * (0.00916281f +
+                          r * (-0.02057706f +
+                               r * (0.02635537f +
+                                    r * (-0.01647633f + r * 0.00392377f)))))));
+  return expf(z) / sqrtf(z) * poly;
+}
+
+// Linear-interpolation hat max(1 - |x|, 0) (kernels.py:92-94).
+__device__ __forceinline__ float spline_window(float x) {
+  return fmaxf(1.0f - fabsf(x), 0.0f);
+}
+
+__device__ __forceinline__ float window(int kernel, float x, float beta) {
+  return kernel == TNL_KERNEL_KAISER_BESSEL ? kb_window(x, beta)
+                                            : spline_window(x);
+}
+
+__device__ __forceinline__ int64_t mod_pos(int64_t g, int64_t n) {
+  int64_t r = g % n;
+  return r < 0 ? r + n : r;
+}
+
+// Per-launch geometry of one interpolation problem.
+struct InterpGeom {
+  int64_t n[3];        // grid size per dim (dims padded to 3 on the left)
+  int64_t cstride[3];  // cell stride per dim, in cells
+  int64_t ncells;      // prod(n)
+  float half[3];       // w_d / 2
+  float inv_half[3];   // 2 / w_d
+  int ntap[3];         // ceil(w_d) + 1
+  int ndim;
+  int kernel;
+  int norm;
+  int pad;
+  float beta;
+};
+
+// One dimension of the tap set of a point at coordinate t:
+// g = ceil(t - w/2) + k for k in [0, ntap); a tap is kept iff
+// |g - t| <= w/2 (kernels.py:264-279, 383-388); under zero padding taps
+// outside [0, n) are dropped, under circular padding g wraps modulo n.
+// Returns the index of the first tap, the number of consecutive kept taps
+// and, for k < count, the distance (normalised by w/2) of each kept tap.
+struct TapRange {
+  int64_t g0;  // first kept tap, unwrapped
+  int count;
+};
+
+__device__ __forceinline__ TapRange tap_range(float t, float half, int ntap,
+                                              int64_t n, int pad) {
+  TapRange r;
+  r.g0 = 0;
+  r.count = 0;
+  // Non-finite coordinates have no taps (every |g - t| <= w/2 test fails).
+  if (!(fabsf(t) < 4.0e18f)) return r;
+  const int64_t lower = (int64_t)ceilf(t - half);
+  // kept taps form one interval; find it by testing the ends
+  int k0 = 0, k1 = ntap - 1;
+  while (k0 <= k1 && !(fabsf((float)(lower + k0) - t) <= half)) ++k0;
+  while (k1 >= k0 && !(fabsf((float)(lower + k1) - t) <= half)) --k1;
+  if (k1 < k0) return r;
+  int64_t a = lower + k0, b = lower + k1;
+  if (pad == TNL_PAD_ZERO) {
+    a = a < 0 ? 0 : a;
+    b = b > n - 1 ? n - 1 : b;
+    if (b < a) return r;
+  }
+  r.g0 = a;
+  r.count = (int)(b - a + 1);
+  return r;
+}
+
+}  // namespace tnl

@@ -1,0 +1,460 @@
+// Point-binning plan: a stable counting sort of the nonuniform points into
+// spatial tiles, built entirely on the device (no host synchronisation, so
+// it can be captured in a HIP graph).
+//
+// Two tilings share one workspace:
+//   * interpolation bins (u_*): each point belongs to the bin holding its
+//     first kept tap; ungrid stages bin + halo of the grid in LDS.
+//   * gridding tiles (g_*): each tile lists every point whose taps touch it;
+//     grid accumulates a tile in LDS and writes it once ("owner computes"),
+//     so the output needs neither zero-fill nor global atomics except for
+//     the few tiles dense enough to be split across workgroups.
+#include <algorithm>
+
+#include "tnl_common.hpp"
+#include "tnl_internal.hpp"
+
+namespace tnl {
+
+constexpr int PLAN_THREADS = 256;
+
+struct PlanDev {
+  int ndim;
+  int64_t n[3];
+  float half[3];
+  int ntap[3];
+  int pad;
+  int utile[3], untiles[3];
+  int gtile[3], gntiles[3];
+  int64_t u_nbins, g_ntiles_total;
+  int64_t npts;
+  int max_sub;
+  int g_cap;
+  int gmaxper[3];  // max distinct gridding tiles touched per dim
+};
+
+static int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+static int default_u_tile(int ndim) {
+  return ndim == 3 ? 8 : ndim == 2 ? 32 : 1024;
+}
+static void default_g_tile(int ndim, int t[3]) {
+  // innermost (contiguous) axis longest: coalesced tile writes
+  if (ndim == 3) { t[0] = 8; t[1] = 8; t[2] = 16; }
+  else if (ndim == 2) { t[0] = 1; t[1] = 16; t[2] = 64; }
+  else { t[0] = 1; t[1] = 1; t[2] = 4096; }
+}
+
+int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
+                const float* width, int pad_mode, tnl_plan_layout* L) {
+  if (ndim < 1 || ndim > 3) return fail(TNL_EINVAL, "ndim must be 1, 2 or 3");
+  if (npts < 0) return fail(TNL_EINVAL, "npts must be >= 0");
+  if (npts > (int64_t(1) << 28))
+    return fail(TNL_EINVAL, "plan supports at most 2^28 points");
+  if (pad_mode != TNL_PAD_CIRCULAR && pad_mode != TNL_PAD_ZERO)
+    return fail(TNL_EINVAL, "Unrecognized pad_mode");
+  std::memset(L, 0, sizeof(*L));
+  L->ndim = ndim;
+  L->npts = npts;
+  int gt[3];
+  default_g_tile(ndim, gt);
+  int64_t n[3] = {1, 1, 1};
+  int64_t touch_per_pt = 1;
+  for (int d = 0; d < 3; ++d) {
+    L->ntaps[d] = 1;
+    L->u_tile[d] = 1;
+    L->u_ntiles[d] = 1;
+    L->g_tile[d] = 1;
+    L->g_ntiles[d] = 1;
+  }
+  for (int d = 0; d < ndim; ++d) {
+    const int gd = 3 - ndim + d;
+    if (grid_size[d] < 1) return fail(TNL_EINVAL, "grid_size must be >= 1");
+    if (!(width[d] > 0.0f) || !std::isfinite(width[d]))
+      return fail(TNL_EINVAL, "kernel width must be positive and finite");
+    n[gd] = grid_size[d];
+    const int ntap = (int)std::ceil((double)width[d]) + 1;
+    if (ntap > TNL_MAX_TAPS) return fail(TNL_EINVAL, "kernel width too large");
+    L->ntaps[gd] = ntap;
+    int64_t ut = default_u_tile(ndim);
+    if (ut > n[gd]) ut = n[gd];
+    L->u_tile[gd] = (int32_t)ut;
+    L->u_ntiles[gd] = (int32_t)((n[gd] + ut - 1) / ut);
+    int64_t g = std::max<int64_t>(gt[gd], ntap);
+    if (2 * g >= n[gd]) g = n[gd];  // one tile along this axis
+    L->g_tile[gd] = (int32_t)g;
+    L->g_ntiles[gd] = (int32_t)((n[gd] + g - 1) / g);
+    const int per = L->g_ntiles[gd] == 1 ? 1 : (n[gd] % g == 0 ? 2 : 3);
+    touch_per_pt *= per;
+  }
+  L->u_nbins = (int64_t)L->u_ntiles[0] * L->u_ntiles[1] * L->u_ntiles[2];
+  L->g_ntiles_total = (int64_t)L->g_ntiles[0] * L->g_ntiles[1] * L->g_ntiles[2];
+  L->max_sub = 2048;
+  L->sub_capacity = L->u_nbins + (npts + L->max_sub - 1) / L->max_sub + 1;
+  L->g_cap = 8192;
+  L->touch_capacity = npts * touch_per_pt;
+  L->item_capacity =
+      L->g_ntiles_total + (L->touch_capacity + L->g_cap - 1) / L->g_cap + 1;
+  int64_t off = 0;
+  auto take = [&](int64_t bytes) {
+    const int64_t o = off;
+    off = align_up(off + bytes, 256);
+    return o;
+  };
+  const int64_t nb1 = L->u_nbins + 1;  // + dead bin
+  L->off_counts = take(4 * nb1);
+  L->off_starts = take(4 * (nb1 + 1));
+  L->off_cursor = take(4 * nb1);
+  L->off_perm = take(4 * std::max<int64_t>(npts, 1));
+  L->off_slocs = take(4 * std::max<int64_t>(npts, 1) * ndim);
+  L->off_subk = take(4 * L->u_nbins);
+  L->off_suboff = take(4 * (L->u_nbins + 1));
+  L->off_sub = take(16 * L->sub_capacity);
+  L->off_gcounts = take(4 * L->g_ntiles_total);
+  L->off_gstarts = take(4 * (L->g_ntiles_total + 1));
+  L->off_gcursor = take(4 * L->g_ntiles_total);
+  L->off_tlist = take(4 * std::max<int64_t>(L->touch_capacity, 1));
+  L->off_itemk = take(4 * L->g_ntiles_total);
+  L->off_itemoff = take(4 * (L->g_ntiles_total + 1));
+  L->off_items = take(16 * L->item_capacity);
+  L->bytes = off;
+  return TNL_OK;
+}
+
+// ------------------------------------------------------------ device side
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// atomicAdd(&ctr[key], 1) for every active lane, merged per distinct key so
+// a wave issues one atomic per key; slots follow lane order, which keeps the
+// sort stable inside a wave.
+__device__ __forceinline__ int agg_inc(int* ctr, int key, bool active) {
+  const int lane = lane_id();
+  uint64_t pending = __ballot(active);
+  int out = 0;
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int lkey = __shfl(key, leader);
+    const bool mine = ((pending >> lane) & 1ull) && key == lkey;
+    const uint64_t same = __ballot(mine);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(ctr + lkey, (int)__popcll(same));
+    base = __shfl(base, leader);
+    if (mine) out = base + (int)__popcll(same & ((1ull << lane) - 1ull));
+    pending &= ~same;
+  }
+  return out;
+}
+
+// Bin of a point for interpolation, or P.u_nbins when it has no taps.
+__device__ __forceinline__ int u_bin_of(const float* t, const PlanDev& P) {
+  int64_t bin = 0;
+  for (int d = 0; d < 3; ++d) {
+    int64_t c = 0;
+    if (d >= 3 - P.ndim) {
+      const TapRange r = tap_range(t[d - (3 - P.ndim)], P.half[d], P.ntap[d],
+                                   P.n[d], P.pad);
+      if (r.count == 0) return (int)P.u_nbins;
+      c = P.pad == TNL_PAD_CIRCULAR ? mod_pos(r.g0, P.n[d]) : r.g0;
+    }
+    bin = bin * P.untiles[d] + c / P.utile[d];
+  }
+  return (int)bin;
+}
+
+// Distinct gridding tiles touched along each axis (up to 3 per axis).
+struct Touch {
+  int cnt[3];
+  int tile[3][3];
+};
+
+__device__ __forceinline__ bool touches_of(const float* t, const PlanDev& P,
+                                           Touch& T) {
+  for (int d = 0; d < 3; ++d) {
+    if (d < 3 - P.ndim) {
+      T.cnt[d] = 1;
+      T.tile[d][0] = 0;
+      continue;
+    }
+    const TapRange r =
+        tap_range(t[d - (3 - P.ndim)], P.half[d], P.ntap[d], P.n[d], P.pad);
+    if (r.count == 0) return false;
+    int c = 0;
+    for (int k = 0; k < r.count; ++k) {
+      const int64_t g = P.pad == TNL_PAD_CIRCULAR ? mod_pos(r.g0 + k, P.n[d])
+                                                  : r.g0 + k;
+      const int tt = (int)(g / P.gtile[d]);
+      bool seen = false;
+      for (int j = 0; j < c; ++j) seen |= T.tile[d][j] == tt;
+      if (!seen && c < 3) T.tile[d][c++] = tt;
+    }
+    T.cnt[d] = c;
+  }
+  return true;
+}
+
+__global__ void zero_ints_kernel(int* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+__global__ __launch_bounds__(PLAN_THREADS) void count_kernel(
+    const float* __restrict__ locs, PlanDev P, int* __restrict__ counts) {
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < P.npts;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = base + threadIdx.x;
+    const bool active = p < P.npts;
+    float t[3] = {0.f, 0.f, 0.f};
+    if (active)
+      for (int d = 0; d < P.ndim; ++d) t[d] = locs[p * P.ndim + d];
+    const int bin = active ? u_bin_of(t, P) : 0;
+    agg_inc(counts, bin, active);
+  }
+}
+
+__global__ __launch_bounds__(PLAN_THREADS) void scatter_kernel(
+    const float* __restrict__ locs, PlanDev P, const int* __restrict__ starts,
+    int* __restrict__ cursor, int* __restrict__ perm,
+    float* __restrict__ slocs) {
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < P.npts;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = base + threadIdx.x;
+    const bool active = p < P.npts;
+    float t[3] = {0.f, 0.f, 0.f};
+    if (active)
+      for (int d = 0; d < P.ndim; ++d) t[d] = locs[p * P.ndim + d];
+    const int bin = active ? u_bin_of(t, P) : 0;
+    const int slot = agg_inc(cursor, bin, active);
+    if (active) {
+      const int pos = starts[bin] + slot;
+      perm[pos] = (int)p;
+      for (int d = 0; d < P.ndim; ++d) slocs[(int64_t)pos * P.ndim + d] = t[d];
+    }
+  }
+}
+
+// Touch counting / scattering walk the points in sorted order, so each
+// tile's list comes out grouped by interpolation bin.
+template <bool SCATTER>
+__global__ __launch_bounds__(PLAN_THREADS) void touch_kernel(
+    const float* __restrict__ slocs, PlanDev P, int* __restrict__ gctr,
+    const int* __restrict__ gstarts, int* __restrict__ tlist) {
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < P.npts;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = base + threadIdx.x;
+    const bool active = q < P.npts;
+    float t[3] = {0.f, 0.f, 0.f};
+    if (active)
+      for (int d = 0; d < P.ndim; ++d) t[d] = slocs[q * P.ndim + d];
+    Touch T;
+    const bool live = active && touches_of(t, P, T);
+    const int c0 = live ? T.cnt[0] : 0, c1 = live ? T.cnt[1] : 0,
+              c2 = live ? T.cnt[2] : 0;
+    const int ntouch = c0 * c1 * c2;
+    // every lane walks the same 27 slots so ballots stay wave-uniform
+    for (int s = 0; s < 27; ++s) {
+      const bool act = s < ntouch;
+      int tile = 0;
+      if (act) {
+        const int a = s / (c1 * c2), rem = s - a * (c1 * c2);
+        const int b = rem / c2, c = rem - b * c2;
+        tile = (T.tile[0][a] * P.gntiles[1] + T.tile[1][b]) * P.gntiles[2] +
+               T.tile[2][c];
+      }
+      if (__ballot(act) == 0) break;
+      const int slot = agg_inc(gctr, tile, act);
+      if (SCATTER && act) tlist[gstarts[tile] + slot] = (int)q;
+    }
+  }
+}
+
+// Exclusive scan of n ints (one workgroup): out[0..n], out[n] = total.
+// in may be given as a transform: k = ceil(in / div) (min 1 if min1).
+__global__ __launch_bounds__(1024) void scan_kernel(const int* __restrict__ in,
+                                                    int* __restrict__ out,
+                                                    int64_t n, int div,
+                                                    int min1) {
+  __shared__ int part[1024];
+  const int tid = threadIdx.x;
+  const int64_t chunk = (n + 1023) / 1024;
+  const int64_t lo = tid * chunk;
+  const int64_t hi = lo + chunk < n ? lo + chunk : n;
+  auto val = [&](int64_t i) {
+    int v = in[i];
+    if (div > 1) v = (v + div - 1) / div;
+    if (min1 && v < 1) v = 1;
+    return v;
+  };
+  int s = 0;
+  for (int64_t i = lo; i < hi; ++i) s += val(i);
+  part[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off *= 2) {
+    const int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = tid > 0 ? part[tid - 1] : 0;
+  for (int64_t i = lo; i < hi; ++i) {
+    out[i] = run;
+    run += val(i);
+  }
+  if (tid == 1023) out[n] = part[1023];
+}
+
+// Interpolation subproblems: <= max_sub points of one bin each.
+__global__ void sub_fill_kernel(const int* __restrict__ counts,
+                                const int* __restrict__ starts,
+                                const int* __restrict__ suboff, PlanDev P,
+                                int4* __restrict__ sub) {
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       b < P.u_nbins; b += (int64_t)gridDim.x * blockDim.x) {
+    const int c = counts[b];
+    const int o = suboff[b];
+    for (int j = 0; j * P.max_sub < c; ++j) {
+      const int cnt = min(P.max_sub, c - j * P.max_sub);
+      sub[o + j] = make_int4((int)b, starts[b] + j * P.max_sub, cnt, 0);
+    }
+  }
+}
+
+// Gridding work items: every tile gets >= 1 item (empty tiles write zeros);
+// tiles with more than g_cap touches are split and flagged shared (w = 1).
+__global__ void item_fill_kernel(const int* __restrict__ gcounts,
+                                 const int* __restrict__ gstarts,
+                                 const int* __restrict__ itemoff, PlanDev P,
+                                 int4* __restrict__ items) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       t < P.g_ntiles_total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int c = gcounts[t];
+    const int o = itemoff[t];
+    const int k = c <= P.g_cap ? 1 : (c + P.g_cap - 1) / P.g_cap;
+    for (int j = 0; j < k; ++j) {
+      const int cnt = max(0, min(P.g_cap, c - j * P.g_cap));
+      items[o + j] = make_int4((int)t, gstarts[t] + j * P.g_cap, cnt,
+                               k > 1 ? 1 : 0);
+    }
+  }
+}
+
+PlanDev make_plan_dev(const tnl_plan_layout& L, const int64_t* grid_size,
+                      const float* width, int pad_mode) {
+  PlanDev P;
+  std::memset(&P, 0, sizeof(P));
+  P.ndim = L.ndim;
+  for (int d = 0; d < 3; ++d) {
+    P.n[d] = 1;
+    P.half[d] = 0.5f;
+    P.ntap[d] = 1;
+    P.utile[d] = L.u_tile[d];
+    P.untiles[d] = L.u_ntiles[d];
+    P.gtile[d] = L.g_tile[d];
+    P.gntiles[d] = L.g_ntiles[d];
+  }
+  for (int d = 0; d < L.ndim; ++d) {
+    const int gd = 3 - L.ndim + d;
+    P.n[gd] = grid_size[d];
+    P.half[gd] = width[d] * 0.5f;
+    P.ntap[gd] = L.ntaps[gd];
+  }
+  P.pad = pad_mode;
+  P.u_nbins = L.u_nbins;
+  P.g_ntiles_total = L.g_ntiles_total;
+  P.npts = L.npts;
+  P.max_sub = (int)L.max_sub;
+  P.g_cap = (int)L.g_cap;
+  return P;
+}
+
+static unsigned grid_for(int64_t n, int64_t cap = 4096) {
+  int64_t b = (n + PLAN_THREADS - 1) / PLAN_THREADS;
+  if (b > cap) b = cap;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+}  // namespace tnl
+
+using namespace tnl;
+
+extern "C" int tnl_plan_layout_query(int64_t npts, int ndim,
+                                     const int64_t* grid_size,
+                                     const float* width, int pad_mode,
+                                     tnl_plan_layout* out) {
+  if (!out || !grid_size || !width) return fail(TNL_EINVAL, "null pointer");
+  return plan_layout(npts, ndim, grid_size, width, pad_mode, out);
+}
+
+extern "C" int tnl_plan_build(const float* locs, int64_t npts, int ndim,
+                              const int64_t* grid_size, const float* width,
+                              int pad_mode, void* workspace,
+                              int64_t workspace_bytes, void* stream) {
+  if (!grid_size || !width) return fail(TNL_EINVAL, "null pointer");
+  tnl_plan_layout L;
+  if (int rc = plan_layout(npts, ndim, grid_size, width, pad_mode, &L))
+    return rc;
+  if (!workspace) return fail(TNL_EINVAL, "null workspace");
+  if (workspace_bytes < L.bytes)
+    return fail(TNL_EWORKSPACE, "plan workspace too small");
+  if (npts > 0 && !locs) return fail(TNL_EINVAL, "null locs");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  int* counts = (int*)(ws + L.off_counts);
+  int* starts = (int*)(ws + L.off_starts);
+  int* cursor = (int*)(ws + L.off_cursor);
+  int* perm = (int*)(ws + L.off_perm);
+  float* slocs = (float*)(ws + L.off_slocs);
+  int* subk = (int*)(ws + L.off_subk);
+  int* suboff = (int*)(ws + L.off_suboff);
+  int4* sub = (int4*)(ws + L.off_sub);
+  int* gcounts = (int*)(ws + L.off_gcounts);
+  int* gstarts = (int*)(ws + L.off_gstarts);
+  int* gcursor = (int*)(ws + L.off_gcursor);
+  int* tlist = (int*)(ws + L.off_tlist);
+  int* itemoff = (int*)(ws + L.off_itemoff);
+  int4* items = (int4*)(ws + L.off_items);
+  (void)subk;
+  const PlanDev P = make_plan_dev(L, grid_size, width, pad_mode);
+  const int64_t nb1 = L.u_nbins + 1;
+
+  zero_ints_kernel<<<grid_for(nb1), PLAN_THREADS, 0, s>>>(counts, nb1);
+  zero_ints_kernel<<<grid_for(nb1), PLAN_THREADS, 0, s>>>(cursor, nb1);
+  zero_ints_kernel<<<grid_for(L.g_ntiles_total), PLAN_THREADS, 0, s>>>(
+      gcounts, L.g_ntiles_total);
+  zero_ints_kernel<<<grid_for(L.g_ntiles_total), PLAN_THREADS, 0, s>>>(
+      gcursor, L.g_ntiles_total);
+  TNL_LAUNCH_CHECK("plan zero");
+  if (npts > 0) {
+    count_kernel<<<grid_for(npts), PLAN_THREADS, 0, s>>>(locs, P, counts);
+    TNL_LAUNCH_CHECK("plan count");
+  }
+  scan_kernel<<<1, 1024, 0, s>>>(counts, starts, nb1, 1, 0);
+  TNL_LAUNCH_CHECK("plan scan");
+  if (npts > 0) {
+    scatter_kernel<<<grid_for(npts), PLAN_THREADS, 0, s>>>(
+        locs, P, starts, cursor, perm, slocs);
+    TNL_LAUNCH_CHECK("plan scatter");
+    touch_kernel<false><<<grid_for(npts), PLAN_THREADS, 0, s>>>(
+        slocs, P, gcounts, nullptr, nullptr);
+    TNL_LAUNCH_CHECK("plan touch count");
+  }
+  // subproblems: k_b = ceil(count_b / max_sub) (dead bin excluded)
+  scan_kernel<<<1, 1024, 0, s>>>(counts, suboff, L.u_nbins, (int)L.max_sub,
+                                 0);
+  sub_fill_kernel<<<grid_for(L.u_nbins), PLAN_THREADS, 0, s>>>(
+      counts, starts, suboff, P, sub);
+  TNL_LAUNCH_CHECK("plan subproblems");
+  scan_kernel<<<1, 1024, 0, s>>>(gcounts, gstarts, L.g_ntiles_total, 1, 0);
+  if (npts > 0) {
+    touch_kernel<true><<<grid_for(npts), PLAN_THREADS, 0, s>>>(
+        slocs, P, gcursor, gstarts, tlist);
+    TNL_LAUNCH_CHECK("plan touch scatter");
+  }
+  scan_kernel<<<1, 1024, 0, s>>>(gcounts, itemoff, L.g_ntiles_total,
+                                 (int)L.g_cap, 1);
+  item_fill_kernel<<<grid_for(L.g_ntiles_total), PLAN_THREADS, 0, s>>>(
+      gcounts, gstarts, itemoff, P, items);
+  TNL_LAUNCH_CHECK("plan items");
+  return TNL_OK;
+}

@@ -1,0 +1,26 @@
+"""Functional API of the gridding path (mirrors torchlinops.functional)."""
+
+from ._interp import grid, interpolate, interpolate_adjoint, ungrid
+from ._plan import clear_plan_cache
+from ._unfold import (
+    array_to_blocks,
+    blocks_to_array,
+    fold,
+    get_nblocks,
+    get_norm_weights,
+    unfold,
+)
+
+__all__ = [
+    "grid",
+    "ungrid",
+    "interpolate",
+    "interpolate_adjoint",
+    "array_to_blocks",
+    "blocks_to_array",
+    "fold",
+    "unfold",
+    "get_nblocks",
+    "get_norm_weights",
+    "clear_plan_cache",
+]

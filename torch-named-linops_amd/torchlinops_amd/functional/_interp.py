@@ -1,0 +1,307 @@
+"""Interpolation (ungrid) and gridding, the differentiable pair
+``interpolate`` / ``interpolate_adjoint``.
+
+Same signatures, defaults, shape rules and errors as the reference's
+functional API:
+  * ``ungrid`` / ``grid``            src/torchlinops/functional/_interp/ungrid.py:37-67,
+                                     grid.py:37-72
+  * shape preparation              ungrid.py:577-621, grid.py:522-568
+  * kernel defaults                kernels.py:97-105, 245-252
+  * autograd pair                  interp.py:13-236
+The arithmetic runs in the HIP kernels of libtnl (tnl_ungrid / tnl_grid);
+tensors must live on a HIP device.
+"""
+
+from __future__ import annotations
+
+from math import prod
+from typing import Optional
+
+import torch
+from torch.autograd import Function
+
+from .._lib import (
+    KERNELS,
+    PAD_MODES,
+    check,
+    f32_array,
+    i64_array,
+    lib,
+    require_device,
+    stream_of,
+)
+from ._plan import get_plan
+
+__all__ = [
+    "ungrid",
+    "grid",
+    "interpolate",
+    "interpolate_adjoint",
+    "get_kernel_params",
+]
+
+# Reference limit: next_pow2(ceil(w + 1)) <= 64, doubled in the last dim for
+# complex data (grid.py:145-158).
+MAX_BLOCK_WIDTH = 64
+
+
+def get_kernel_params(kernel: str, kernel_params: Optional[dict]) -> dict:
+    """Fill in defaults: ``beta = 1.0`` for Kaiser-Bessel (kernels.py:245-252)."""
+    kernel_params = {} if kernel_params is None else dict(kernel_params)
+    if kernel == "kaiser_bessel":
+        kernel_params["beta"] = kernel_params.get("beta", 1.0)
+    elif kernel == "spline":
+        pass
+    else:
+        raise ValueError(f"Unrecognized kernel type: {kernel}")
+    return kernel_params
+
+
+def _width_tuple(width, ndim: int) -> tuple:
+    if isinstance(width, (int, float)) and not isinstance(width, bool):
+        return (float(width),) * ndim
+    width = tuple(float(w) for w in width)
+    if len(width) != ndim:
+        raise ValueError(
+            "If width specified as tuple it must be same length as grid size but "
+            f"got len(width) = {len(width)} and len(grid_size) = {ndim}"
+        )
+    return width
+
+
+def _next_pow2(n: int) -> int:
+    p = 1
+    while p < n:
+        p *= 2
+    return p
+
+
+def _check_block_width(width: tuple, is_complex: bool) -> None:
+    bw = [_next_pow2(int(-(-(w + 1) // 1))) for w in width]
+    if is_complex:
+        bw[-1] *= 2
+    if max(bw) > MAX_BLOCK_WIDTH:
+        raise ValueError(
+            f"Necessary block width {bw} has entry which exceeds maximum width "
+            f"{MAX_BLOCK_WIDTH} (kernel width is doubled in last dim if input is complex)"
+        )
+
+
+def _check_dtypes(vals: torch.Tensor, locs: torch.Tensor) -> None:
+    if vals.dtype not in (torch.float32, torch.complex64):
+        raise TypeError(
+            f"interpolation kernels compute in float32/complex64; got {vals.dtype}"
+        )
+    if not locs.is_floating_point():
+        raise TypeError(f"locs must be floating point, got {locs.dtype}")
+
+
+def _launch(adjoint: bool, vals_flat, locs, locs2d, out, nbatch, npts, ndim,
+            grid_size, width, kernel, norm, pad_mode, is_complex, kernel_params):
+    if kernel not in KERNELS:
+        raise ValueError(f"Unrecognized kernel {kernel}")
+    if norm not in (1, 2):
+        raise ValueError(f"Unexpected norm: {norm}.")
+    if pad_mode not in PAD_MODES:
+        raise ValueError(f"Unrecognized pad_mode {pad_mode}")
+    beta = float(kernel_params.get("beta", 1.0)) if kernel == "kaiser_bessel" else 1.0
+    pad_code = PAD_MODES[pad_mode]
+    plan = get_plan(locs, locs2d, ndim, grid_size, width, norm, pad_code)
+    fn = lib().tnl_grid if adjoint else lib().tnl_ungrid
+    with torch.cuda.device(vals_flat.device):
+        check(
+            fn(
+                vals_flat.data_ptr(),
+                locs2d.data_ptr(),
+                out.data_ptr(),
+                nbatch,
+                npts,
+                ndim,
+                i64_array(grid_size),
+                f32_array(width),
+                KERNELS[kernel],
+                int(norm),
+                pad_code,
+                int(is_complex),
+                beta,
+                None if plan is None else plan.data_ptr(),
+                0 if plan is None else plan.numel(),
+                stream_of(vals_flat),
+            )
+        )
+
+
+def ungrid(
+    vals: torch.Tensor,
+    locs: torch.Tensor,
+    width,
+    kernel: str = "kaiser_bessel",
+    norm: int = 1,
+    pad_mode: str = "circular",
+    kernel_params: Optional[dict] = None,
+) -> torch.Tensor:
+    """Interpolate on-grid ``vals[..., *grid]`` to off-grid ``locs[..., D]``.
+
+    Output shape ``[*batch_shape, *locs.shape[:-1]]``; ``locs`` are in grid
+    units and shared by every batch entry (ungrid.py:37-67, 577-621).
+    """
+    kernel_params = get_kernel_params(kernel, kernel_params)
+    ndim = locs.shape[-1]
+    grid_size = tuple(vals.shape[-ndim:])
+    batch_shape = vals.shape[:-ndim]
+    width = _width_tuple(width, ndim)
+    locs_batch_shape = locs.shape[:-1]
+    npts = prod(locs_batch_shape)
+    is_complex = torch.is_complex(vals)
+    _check_block_width(width, is_complex)
+    require_device(vals, locs, op="ungrid")
+    _check_dtypes(vals, locs)
+    nbatch = prod(batch_shape)
+    vals_flat = vals.reshape(nbatch, *grid_size).contiguous()
+    locs2d = locs.reshape(npts, ndim).to(torch.float32).contiguous()
+    out = torch.empty(nbatch, npts, dtype=vals.dtype, device=vals.device)
+    if is_complex:
+        vals_flat = torch.view_as_real(vals_flat)
+        out_r = torch.view_as_real(out)
+    else:
+        out_r = out
+    if nbatch > 0 and npts > 0:
+        _launch(False, vals_flat, locs, locs2d, out_r, nbatch, npts, ndim,
+                grid_size, width, kernel, norm, pad_mode, is_complex, kernel_params)
+    return out.reshape(*batch_shape, *locs_batch_shape)
+
+
+def grid(
+    vals: torch.Tensor,
+    locs: torch.Tensor,
+    grid_size: tuple,
+    width,
+    kernel: str = "kaiser_bessel",
+    norm: int = 1,
+    pad_mode: str = "circular",
+    kernel_params: Optional[dict] = None,
+) -> torch.Tensor:
+    """Grid off-grid ``vals[..., *locs.shape[:-1]]`` onto ``grid_size``.
+
+    Output shape ``[*batch_shape, *grid_size]`` (grid.py:37-72, 522-568).
+    """
+    kernel_params = get_kernel_params(kernel, kernel_params)
+    grid_size = tuple(int(g) for g in grid_size)
+    ndim = len(grid_size)
+    if locs.shape[-1] != ndim:
+        raise ValueError(
+            f"Expected last dim of locs: to equal ndim but got locs.shape[-1] "
+            f"{locs.shape[-1]} != ndim {ndim}"
+        )
+    locs_batch_shape = locs.shape[:-1]
+    npts = prod(locs_batch_shape)
+    nlb = len(locs_batch_shape)
+    batch_shape = vals.shape[: vals.ndim - nlb]
+    width = _width_tuple(width, ndim)
+    is_complex = torch.is_complex(vals)
+    _check_block_width(width, is_complex)
+    require_device(vals, locs, op="grid")
+    _check_dtypes(vals, locs)
+    if tuple(vals.shape[vals.ndim - nlb:]) != tuple(locs_batch_shape):
+        raise RuntimeError(
+            f"grid expected vals with trailing shape {tuple(locs_batch_shape)} "
+            f"but got {tuple(vals.shape)}"
+        )
+    nbatch = prod(batch_shape)
+    vals_flat = vals.reshape(nbatch, npts).contiguous()
+    locs2d = locs.reshape(npts, ndim).to(torch.float32).contiguous()
+    out = torch.empty(nbatch, *grid_size, dtype=vals.dtype, device=vals.device)
+    if is_complex:
+        vals_flat = torch.view_as_real(vals_flat)
+        out_r = torch.view_as_real(out)
+    else:
+        out_r = out
+    if nbatch > 0:
+        _launch(True, vals_flat, locs, locs2d, out_r, nbatch, npts, ndim,
+                grid_size, width, kernel, norm, pad_mode, is_complex, kernel_params)
+    return out.reshape(*batch_shape, *grid_size)
+
+
+class InterpolateFn(Function):
+    """``ungrid`` with ``grid`` as its vals-gradient (interp.py:13-70)."""
+
+    @staticmethod
+    def forward(vals, locs, width, kernel, norm, pad_mode, kernel_params):
+        out = ungrid(vals, locs, width, kernel, norm, pad_mode, kernel_params)
+        out.requires_grad_(vals.requires_grad)
+        return out
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        vals, locs, width, kernel, norm, pad_mode, kernel_params = inputs
+        ndim = locs.shape[-1]
+        ctx.grid_size = tuple(vals.shape[-ndim:])
+        ctx.params = (width, kernel, norm, pad_mode, kernel_params)
+        ctx.save_for_backward(locs)
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        grad_vals = None
+        if ctx.needs_input_grad[0]:
+            (locs,) = ctx.saved_tensors
+            width, kernel, norm, pad_mode, kernel_params = ctx.params
+            grad_vals = grid(grad_output, locs, ctx.grid_size, width, kernel,
+                             norm, pad_mode, kernel_params)
+        return grad_vals, None, None, None, None, None, None
+
+
+def interpolate(
+    vals: torch.Tensor,
+    locs: torch.Tensor,
+    width,
+    kernel: str = "kaiser_bessel",
+    norm: int = 1,
+    pad_mode: str = "circular",
+    kernel_params: Optional[dict] = None,
+) -> torch.Tensor:
+    """Differentiable ungrid: grid values -> nonuniform locations
+    (src/torchlinops/functional/_interp/interp.py:73-114)."""
+    return InterpolateFn.apply(vals, locs, width, kernel, norm, pad_mode, kernel_params)
+
+
+class InterpolateAdjointFn(Function):
+    """``grid`` with ``ungrid`` as its vals-gradient (interp.py:117-181)."""
+
+    @staticmethod
+    def forward(vals, locs, grid_size, width, kernel, norm, pad_mode, kernel_params):
+        out = grid(vals, locs, grid_size, width, kernel, norm, pad_mode, kernel_params)
+        out.requires_grad_(vals.requires_grad)
+        return out
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        vals, locs, grid_size, width, kernel, norm, pad_mode, kernel_params = inputs
+        ctx.params = (width, kernel, norm, pad_mode, kernel_params)
+        ctx.save_for_backward(locs)
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        grad_vals = None
+        if ctx.needs_input_grad[0]:
+            (locs,) = ctx.saved_tensors
+            width, kernel, norm, pad_mode, kernel_params = ctx.params
+            grad_vals = ungrid(grad_output, locs, width, kernel, norm, pad_mode,
+                               kernel_params)
+        return grad_vals, None, None, None, None, None, None, None
+
+
+def interpolate_adjoint(
+    vals: torch.Tensor,
+    locs: torch.Tensor,
+    grid_size: tuple,
+    width,
+    kernel: str = "kaiser_bessel",
+    norm: int = 1,
+    pad_mode: str = "circular",
+    kernel_params: Optional[dict] = None,
+) -> torch.Tensor:
+    """Differentiable grid: nonuniform values -> grid
+    (src/torchlinops/functional/_interp/interp.py:184-236)."""
+    return InterpolateAdjointFn.apply(
+        vals, locs, grid_size, width, kernel, norm, pad_mode, kernel_params
+    )
