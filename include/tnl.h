@@ -52,10 +52,22 @@ int tnl_abi_version(void);
  * No reference counterpart: the reference's Triton kernels walk points in
  * input order (src/torchlinops/functional/_interp/ungrid.py:154-503).
  */
+/* Tuning knobs for the plan (0 = library default).  The autotuner in the
+ * Python package searches these; they are copied into the layout, so a
+ * plan always runs with the tiling it was built for. */
+typedef struct tnl_tuning {
+  int32_t u_tile[3];  /* interpolation bin edge per (left-padded) axis      */
+  int32_t g_tile[3];  /* gridding tile edge per (left-padded) axis          */
+  int32_t max_sub;    /* points per interpolation subproblem                */
+  int32_t g_cap;      /* touches per gridding work item                     */
+  int32_t lds_budget; /* LDS bytes per workgroup for channel chunking       */
+  int32_t reserved;
+} tnl_tuning;
+
 typedef struct tnl_plan_layout {
   int32_t ndim;
   int32_t ntaps[3];      /* ceil(w_d)+1 per dim (dims left-padded to 3)      */
-  /* interpolation bins: each point goes to the bin of its first tap       */
+  /* interpolation bins: each point goes to the bin holding its first tap   */
   int32_t u_tile[3];     /* bin edge per dim (cells)                         */
   int32_t u_ntiles[3];
   int64_t u_nbins;       /* prod(u_ntiles); bin u_nbins holds dead points    */
@@ -63,27 +75,32 @@ typedef struct tnl_plan_layout {
   int32_t g_tile[3];
   int32_t g_ntiles[3];
   int64_t g_ntiles_total;
+  int64_t grid_size[3];  /* left-padded                                       */
+  float width[3];        /* left-padded                                       */
+  int32_t pad_mode;
   int64_t npts;
   int64_t max_sub;       /* points per interpolation subproblem              */
   int64_t sub_capacity;  /* upper bound on interpolation subproblems         */
   int64_t g_cap;         /* touches per gridding work item                   */
   int64_t touch_capacity;/* upper bound on (tile, point) touches             */
   int64_t item_capacity; /* upper bound on gridding work items               */
+  int64_t lds_budget;    /* LDS bytes per workgroup                          */
   /* byte offsets inside the workspace                                      */
   int64_t off_counts, off_starts, off_cursor, off_perm, off_slocs;
-  int64_t off_subk, off_suboff, off_sub;
+  int64_t off_suboff, off_sub;
   int64_t off_gcounts, off_gstarts, off_gcursor, off_tlist;
-  int64_t off_itemk, off_itemoff, off_items;
+  int64_t off_itemoff, off_items;
   int64_t bytes;         /* total workspace bytes                             */
 } tnl_plan_layout;
 
+/* Fill *out for a trajectory of npts points; tuning may be NULL. */
 int tnl_plan_layout_query(int64_t npts, int ndim, const int64_t* grid_size,
                           const float* width, int pad_mode,
-                          tnl_plan_layout* out);
+                          const tnl_tuning* tuning, tnl_plan_layout* out);
 
-/* Build the plan into a caller-owned device workspace of layout.bytes. */
-int tnl_plan_build(const float* locs, int64_t npts, int ndim,
-                   const int64_t* grid_size, const float* width, int pad_mode,
+/* Build the plan described by *layout into a caller-owned device workspace
+ * of at least layout->bytes. */
+int tnl_plan_build(const float* locs, const tnl_plan_layout* layout,
                    void* workspace, int64_t workspace_bytes, void* stream);
 
 /* ---- interpolation (ungrid) -------------------------------------------
@@ -93,13 +110,14 @@ int tnl_plan_build(const float* locs, int64_t npts, int ndim,
  * at src/torchlinops/functional/_interp/ungrid.py:103-117.
  * vals: [nbatch, *grid_size] (x2 floats if complex); locs: [npts, ndim];
  * out: [nbatch, npts] (x2 if complex), fully written by the call.
- * plan: NULL, or a workspace built by tnl_plan_build for the same locs.
+ * layout/plan: both NULL, or the layout and workspace of a plan built by
+ * tnl_plan_build for the same locs, grid_size, width and pad_mode.
  */
 int tnl_ungrid(const float* vals, const float* locs, float* out,
                int64_t nbatch, int64_t npts, int ndim,
                const int64_t* grid_size, const float* width, int kernel,
                int norm, int pad_mode, int is_complex, float beta,
-               const void* plan, int64_t plan_bytes, void* stream);
+               const tnl_plan_layout* layout, const void* plan, void* stream);
 
 /* ---- gridding (interpolate adjoint) ------------------------------------
  * out[b, g] = sum_{p : g in taps(p)} w(t_p - g) * vals[b, p]
@@ -111,8 +129,8 @@ int tnl_ungrid(const float* vals, const float* locs, float* out,
 int tnl_grid(const float* vals, const float* locs, float* out, int64_t nbatch,
              int64_t npts, int ndim, const int64_t* grid_size,
              const float* width, int kernel, int norm, int pad_mode,
-             int is_complex, float beta, const void* plan, int64_t plan_bytes,
-             void* stream);
+             int is_complex, float beta, const tnl_plan_layout* layout,
+             const void* plan, void* stream);
 
 /* ---- array_to_blocks (unfold) -------------------------------------------
  * y[n, b..., j...] = x[n, b*s + j]  (pure copy, bit-exact)

@@ -35,16 +35,23 @@ def test_plan_layout_is_host_only_and_consistent():
     L = plan_layout(10_240_000, 3, (256, 256, 256), (4.0, 4.0, 4.0), 0)
     assert L.u_nbins == 32 ** 3
     assert tuple(L.ntaps) == (5, 5, 5)
-    assert tuple(L.g_tile) == (8, 8, 16)
+    assert tuple(L.g_tile) == (8, 8, 8)
     assert L.touch_capacity == 8 * 10_240_000
     offs = [L.off_counts, L.off_starts, L.off_cursor, L.off_perm, L.off_slocs,
             L.off_suboff, L.off_sub, L.off_gcounts, L.off_gstarts, L.off_gcursor,
             L.off_tlist, L.off_itemoff, L.off_items]
+    assert tuple(L.grid_size) == (256, 256, 256) and L.lds_budget == 40960
     assert all(o % 256 == 0 for o in offs)
     assert offs == sorted(offs) and L.bytes > offs[-1]
     # small grid: a single tile along each axis, one touch per point
     L2 = plan_layout(100, 2, (8, 8), (4.0, 4.0), 0)
     assert tuple(L2.g_ntiles)[1:] == (1, 1) and L2.touch_capacity == 100
+    # tuning knobs reach the layout
+    L3 = plan_layout(1000, 3, (64, 64, 64), (4.0,) * 3, 0,
+                     tuning={"u_tile": (4, 8, 16), "g_cap": 512, "lds_budget": 131072})
+    assert tuple(L3.u_tile) == (4, 8, 16) and L3.g_cap == 512 and L3.lds_budget == 131072
+    with pytest.raises(ValueError, match="160 KiB"):
+        plan_layout(10, 2, (8, 8), (4.0, 4.0), 0, tuning={"lds_budget": 1 << 20})
 
 
 def test_library_validation_maps_to_exceptions():
@@ -54,14 +61,14 @@ def test_library_validation_maps_to_exceptions():
     g = _lib.i64_array([8, 8])
     w = _lib.f32_array([4.0, 4.0])
     with pytest.raises(ValueError, match="kernel"):
-        _lib.check(h.tnl_ungrid(None, None, None, 1, 1, 2, g, w, 7, 1, 0, 0, 1.0, None, 0, None))
+        _lib.check(h.tnl_ungrid(None, None, None, 1, 1, 2, g, w, 7, 1, 0, 0, 1.0, None, None, None))
     with pytest.raises(ValueError, match="norm"):
-        _lib.check(h.tnl_grid(None, None, None, 1, 1, 2, g, w, 0, 3, 0, 0, 1.0, None, 0, None))
+        _lib.check(h.tnl_grid(None, None, None, 1, 1, 2, g, w, 0, 3, 0, 0, 1.0, None, None, None))
     with pytest.raises(ValueError, match="ndim"):
-        _lib.check(h.tnl_ungrid(None, None, None, 1, 1, 4, g, w, 0, 1, 0, 0, 1.0, None, 0, None))
+        _lib.check(h.tnl_ungrid(None, None, None, 1, 1, 4, g, w, 0, 1, 0, 0, 1.0, None, None, None))
     big = _lib.f32_array([4.0, 40.0])
     with pytest.raises(ValueError, match="block width"):
-        _lib.check(h.tnl_ungrid(None, None, None, 1, 1, 2, g, big, 0, 1, 0, 1, 1.0, None, 0, None))
+        _lib.check(h.tnl_ungrid(None, None, None, 1, 1, 2, g, big, 0, 1, 0, 1, 1.0, None, None, None))
     nb = _lib.i64_array([0])
     with pytest.raises(ValueError, match="nblocks"):
         _lib.check(h.tnl_unfold(None, None, 4, 1, 1, nb, _lib.i64_array([3]),

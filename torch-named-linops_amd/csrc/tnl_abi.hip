@@ -77,8 +77,8 @@ static int interp_entry(bool adjoint, const float* vals, const float* locs,
                         float* out, int64_t nbatch, int64_t npts, int ndim,
                         const int64_t* grid_size, const float* width,
                         int kernel, int norm, int pad_mode, int is_complex,
-                        float beta, const void* plan, int64_t plan_bytes,
-                        void* stream) {
+                        float beta, const tnl_plan_layout* layout,
+                        const void* plan, void* stream) {
   InterpGeom G;
   if (int rc = make_geom(ndim, grid_size, width, kernel, norm, pad_mode,
                          is_complex, beta, G))
@@ -95,13 +95,18 @@ static int interp_entry(bool adjoint, const float* vals, const float* locs,
     return TNL_OK;
   }
   if (!vals || !locs) return fail(TNL_EINVAL, "null input pointer");
+  if ((plan == nullptr) != (layout == nullptr))
+    return fail(TNL_EINVAL, "pass both the plan layout and its workspace, or neither");
   if (plan) {
-    tnl_plan_layout L;
-    if (int rc = plan_layout(npts, ndim, grid_size, width, pad_mode, &L))
-      return rc;
-    if (plan_bytes < L.bytes)
-      return fail(TNL_EWORKSPACE, "plan workspace smaller than its layout");
-    if (binned_supported(G, L))
+    const tnl_plan_layout& L = *layout;
+    bool same = L.ndim == ndim && L.npts == npts && L.pad_mode == pad_mode;
+    for (int d = 0; same && d < 3; ++d)
+      same = L.grid_size[d] == G.n[d] && L.ntaps[d] == G.ntap[d] &&
+             L.width[d] == (d >= 3 - ndim ? width[d - (3 - ndim)] : 1.0f);
+    if (!same)
+      return fail(TNL_EINVAL,
+                  "plan was built for another trajectory size, grid, width or pad_mode");
+    if (binned_supported(G, L, adjoint))
       return interp_binned(adjoint, vals, out, nbatch, G, is_complex, L, plan,
                            s);
   }
@@ -124,20 +129,20 @@ extern "C" int tnl_ungrid(const float* vals, const float* locs, float* out,
                           int64_t nbatch, int64_t npts, int ndim,
                           const int64_t* grid_size, const float* width,
                           int kernel, int norm, int pad_mode, int is_complex,
-                          float beta, const void* plan, int64_t plan_bytes,
-                          void* stream) {
+                          float beta, const tnl_plan_layout* layout,
+                          const void* plan, void* stream) {
   return interp_entry(false, vals, locs, out, nbatch, npts, ndim, grid_size,
-                      width, kernel, norm, pad_mode, is_complex, beta, plan,
-                      plan_bytes, stream);
+                      width, kernel, norm, pad_mode, is_complex, beta, layout,
+                      plan, stream);
 }
 
 extern "C" int tnl_grid(const float* vals, const float* locs, float* out,
                         int64_t nbatch, int64_t npts, int ndim,
                         const int64_t* grid_size, const float* width,
                         int kernel, int norm, int pad_mode, int is_complex,
-                        float beta, const void* plan, int64_t plan_bytes,
-                        void* stream) {
+                        float beta, const tnl_plan_layout* layout,
+                        const void* plan, void* stream) {
   return interp_entry(true, vals, locs, out, nbatch, npts, ndim, grid_size,
-                      width, kernel, norm, pad_mode, is_complex, beta, plan,
-                      plan_bytes, stream);
+                      width, kernel, norm, pad_mode, is_complex, beta, layout,
+                      plan, stream);
 }

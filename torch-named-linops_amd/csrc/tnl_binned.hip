@@ -23,7 +23,6 @@ namespace tnl {
 
 constexpr int BIN_THREADS = 256;
 constexpr int NUM_XCD = 8;
-constexpr int64_t LDS_BUDGET = 64 * 1024;
 
 struct BinGeom {
   int tile[3];
@@ -58,7 +57,7 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nwg) {
   return (bid % NUM_XCD) * per + bid / NUM_XCD;
 }
 
-template <int NDIM, bool CPLX, int CC>
+template <int NDIM, bool CPLX, int CC, int NT>
 __global__ __launch_bounds__(BIN_THREADS) void ungrid_binned_kernel(
     const float* __restrict__ vals, float* __restrict__ out, int64_t nbatch,
     InterpGeom G, BinGeom B, const int* __restrict__ perm,
@@ -75,6 +74,14 @@ __global__ __launch_bounds__(BIN_THREADS) void ungrid_binned_kernel(
   const int64_t c0 = (int64_t)blockIdx.y * CC;
   int nc = (int)(nbatch - c0);
   nc = nc < CC ? nc : CC;
+  if (bin == B.nbins) {  // points without taps: zero output
+    T* __restrict__ o = reinterpret_cast<T*>(out);
+    for (int i = threadIdx.x; i < count; i += BIN_THREADS) {
+      const int p = perm[start + i];
+      for (int c = 0; c < nc; ++c) o[(c0 + c) * B.npts + p] = C::zero();
+    }
+    return;
+  }
 
   // tile origin per (padded) axis
   int origin[3];
@@ -86,7 +93,8 @@ __global__ __launch_bounds__(BIN_THREADS) void ungrid_binned_kernel(
       origin[d] = ti * B.tile[d];
     }
   }
-  // ---- stage window: [cell][channel] in LDS, coalesced along z in HBM
+  // ---- stage window: channel planes [c][cell] in LDS (lanes gathering
+  // different cells then fall in different banks), coalesced along z in HBM
   const T* __restrict__ v = reinterpret_cast<const T*>(vals);
   const int w12 = B.win[1] * B.win[2];
   const int total = B.wcells * CC;
@@ -110,59 +118,69 @@ __global__ __launch_bounds__(BIN_THREADS) void ungrid_binned_kernel(
     T val = C::zero();
     if (ok)
       val = v[(c0 + c) * G.ncells + (g[0] * G.n[1] + g[1]) * G.n[2] + g[2]];
-    lds[cell * CC + c] = val;
+    lds[c * B.wcells + cell] = val;
   }
   __syncthreads();
 
-  // ---- gather: one point per lane
+  // ---- gather: one point per lane.  The x taps are a runtime loop (their
+  // window values computed on the fly), y and z taps are unrolled to NT:
+  // the loop body stays small enough for the instruction cache.
   for (int i = threadIdx.x; i < count; i += BIN_THREADS) {
     const int q = start + i;
     const int p = perm[q];
-    float w[3][8];
-    int lo[3], cnt[3];
+    float t[3] = {0.f, 0.f, 0.f};
+    TapRange r[3];
+    int lo[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      if (d < 3 - NDIM) {
-        lo[d] = 0;
-        cnt[d] = 1;
-        w[d][0] = 1.0f;
-        continue;
-      }
-      const float t = slocs[(int64_t)q * NDIM + d - (3 - NDIM)];
-      const TapRange r = tap_range(t, G.half[d], G.ntap[d], G.n[d], G.pad);
-      cnt[d] = r.count;
-      const int64_t gw =
-          G.pad == TNL_PAD_CIRCULAR ? mod_pos(r.g0, G.n[d]) : r.g0;
-      lo[d] = (int)(gw - origin[d]);
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        w[d][k] = 0.0f;
-        if (k < r.count) {
-          const float x =
-              fminf(fabsf((float)(r.g0 + k) - t) * G.inv_half[d], 1.0f);
-          w[d][k] = window(G.kernel, x, G.beta);
-        }
+      r[d].g0 = 0;
+      r[d].count = 1;
+      lo[d] = 0;
+      if (d >= 3 - NDIM) {
+        t[d] = slocs[(int64_t)q * NDIM + d - (3 - NDIM)];
+        r[d] = tap_range(t[d], G.half[d], G.ntap[d], G.n[d], G.pad);
+        const int64_t gw =
+            G.pad == TNL_PAD_CIRCULAR ? mod_pos(r[d].g0, G.n[d]) : r[d].g0;
+        lo[d] = (int)(gw - origin[d]);
       }
     }
+    float wy[NT], wz[NT];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+      wy[k] = 0.0f;
+      wz[k] = 0.0f;
+      if (NDIM >= 2 && k < r[1].count)
+        wy[k] = window(G.kernel,
+                       fminf(fabsf((float)(r[1].g0 + k) - t[1]) * G.inv_half[1], 1.0f),
+                       G.beta);
+      if (k < r[2].count)
+        wz[k] = window(G.kernel,
+                       fminf(fabsf((float)(r[2].g0 + k) - t[2]) * G.inv_half[2], 1.0f),
+                       G.beta);
+    }
+    if (NDIM < 2) wy[0] = 1.0f;
     T acc[CC];
 #pragma unroll
     for (int c = 0; c < CC; ++c) acc[c] = C::zero();
+#pragma unroll 1
+    for (int kx = 0; kx < r[0].count; ++kx) {
+      float wx = 1.0f;
+      if (NDIM >= 3)
+        wx = window(G.kernel,
+                    fminf(fabsf((float)(r[0].g0 + kx) - t[0]) * G.inv_half[0], 1.0f),
+                    G.beta);
+      const int xrow = (lo[0] + kx) * B.win[1];
 #pragma unroll
-    for (int kx = 0; kx < (NDIM >= 3 ? 8 : 1); ++kx) {
-      if (kx >= cnt[0]) break;
+      for (int ky = 0; ky < NT; ++ky) {
+        if (ky >= r[1].count) break;
+        const float wxy = wx * wy[ky];
+        const T* rowp = lds + (xrow + lo[1] + ky) * B.win[2] + lo[2];
 #pragma unroll
-      for (int ky = 0; ky < 8; ++ky) {
-        if (ky >= cnt[1]) break;
-        const float wxy = w[0][kx] * w[1][ky];
-        const int row = ((lo[0] + kx) * B.win[1] + (lo[1] + ky)) * B.win[2] +
-                        lo[2];
+        for (int kz = 0; kz < NT; ++kz) {
+          if (kz >= r[2].count) break;
+          const float wt = wxy * wz[kz];
 #pragma unroll
-        for (int kz = 0; kz < 8; ++kz) {
-          if (kz >= cnt[2]) break;
-          const float wt = wxy * w[2][kz];
-          const T* cellp = lds + (row + kz) * CC;
-#pragma unroll
-          for (int c = 0; c < CC; ++c) C::fma(acc[c], wt, cellp[c]);
+          for (int c = 0; c < CC; ++c) C::fma(acc[c], wt, rowp[c * B.wcells + kz]);
         }
       }
     }
@@ -211,8 +229,19 @@ __global__ void grid_zero_shared_kernel(float* __restrict__ out,
   }
 }
 
-template <int NDIM, bool CPLX, int CC>
-__global__ __launch_bounds__(BIN_THREADS) void grid_binned_kernel(
+// Gridding, "lanes = taps": one wave per workgroup owns a tile accumulator
+// in LDS (channel planes, z rows padded by one cell against bank
+// conflicts).  Touches are processed in batches of 64: first lanes = points
+// compute each point's first tap, tap counts and 1-D window values into LDS;
+// then the wave walks the batch point by point with lanes = taps (a 4x4x4
+// block, plus a second pass for the fifth tap layer of points on integer
+// coordinates).  Within one instruction every lane addresses a different
+// cell and only this wave touches the tile, so the accumulation is a plain
+// read-modify-write: no LDS atomics (which run about one lane per clock).
+constexpr int GW = 64;  // threads per gridding workgroup
+
+template <int NDIM, bool CPLX, int CC, int NT>
+__global__ __launch_bounds__(GW) void grid_binned_kernel(
     const float* __restrict__ vals, float* __restrict__ out, int64_t nbatch,
     InterpGeom G, BinGeom B, const int* __restrict__ perm,
     const float* __restrict__ slocs, const int* __restrict__ tlist,
@@ -220,132 +249,197 @@ __global__ __launch_bounds__(BIN_THREADS) void grid_binned_kernel(
   using C = Ch<CPLX>;
   using T = typename C::T;
   constexpr int COMP = CPLX ? 2 : 1;
+  constexpr int NP = CC * COMP;  // accumulator planes
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  float* acc = reinterpret_cast<float*>(lds_raw);  // [cell][CC][COMP]
-  const int64_t it = xcd_remap(blockIdx.x, gridDim.x);
-  if (it >= *nitems_ptr) return;
-  const int4 item = items[it];
-  const int tile = item.x, start = item.y, count = item.z;
-  const bool shared = item.w != 0;
-  const int64_t c0 = (int64_t)blockIdx.y * CC;
-  int nc = (int)(nbatch - c0);
-  nc = nc < CC ? nc : CC;
-  int origin[3];
-  {
-    int r = tile;
-    for (int d = 2; d >= 0; --d) {
-      origin[d] = (r % B.ntiles[d]) * B.tile[d];
-      r /= B.ntiles[d];
-    }
-  }
-  const int nacc = B.tcells * CC * COMP;
-  for (int i = threadIdx.x; i < nacc; i += BIN_THREADS) acc[i] = 0.0f;
-  __syncthreads();
-
+  const int zp = B.tile[2] + 1;                   // padded z row
+  const int pcells = B.tile[0] * B.tile[1] * zp;  // cells per plane
+  float* acc = reinterpret_cast<float*>(lds_raw);  // [NP][pcells]
+  // point table of the current batch (lanes = points fill it)
+  int* s_lo = reinterpret_cast<int*>(acc + NP * pcells);  // [3][GW] first tap - origin
+  int* s_cnt = s_lo + 3 * GW;                              // [3][GW]
+  float* s_w = reinterpret_cast<float*>(s_cnt + 3 * GW);   // [3][NT][GW]
+  float* s_v = s_w + 3 * NT * GW;                          // [NP][GW]
+  const int lane = threadIdx.x;
+  const int nitems = *nitems_ptr;
   const T* __restrict__ v = reinterpret_cast<const T*>(vals);
-  for (int i = threadIdx.x; i < count; i += BIN_THREADS) {
-    const int q = tlist[start + i];
-    const int p = perm[q];
-    // per axis: weight and local (in-tile) coordinate of every tap; taps
-    // outside this tile get l = -1.  Under circular padding a footprint
-    // wider than the grid wraps, so in-tile taps need not be contiguous.
-    float w[3][8];
-    int l[3][8];
-    int cnt[3];
-    bool live = true;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      if (d < 3 - NDIM) {
-        cnt[d] = 1;
-        w[d][0] = 1.0f;
-        l[d][0] = 0;
-        continue;
-      }
-      const float t = slocs[(int64_t)q * NDIM + d - (3 - NDIM)];
-      const TapRange r = tap_range(t, G.half[d], G.ntap[d], G.n[d], G.pad);
-      cnt[d] = r.count;
-      bool any = false;
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        w[d][k] = 0.0f;
-        l[d][k] = -1;
-        if (k < r.count) {
-          const int64_t g = G.pad == TNL_PAD_CIRCULAR
-                                ? mod_pos(r.g0 + k, G.n[d])
-                                : r.g0 + k;
-          const int64_t lc = g - origin[d];
-          if (lc >= 0 && lc < B.tile[d]) {
-            const float x =
-                fminf(fabsf((float)(r.g0 + k) - t) * G.inv_half[d], 1.0f);
-            w[d][k] = window(G.kernel, x, G.beta);
-            l[d][k] = (int)lc;
-            any = true;
+
+  // This lane's taps.  3-D: main pass (k0,k1,k2) in [0,4)^3 and a second
+  // pass over the 61 taps of [0,5)^3 with some index 4 (points on integer
+  // coordinates).  2-D: one pass over (k1,k2) in [0,8)^2.
+  int mk[3], sk[3];
+  bool shell_ok = false;
+  if (NDIM >= 3) {
+    mk[0] = lane >> 4; mk[1] = (lane >> 2) & 3; mk[2] = lane & 3;
+    sk[0] = sk[1] = sk[2] = 0;
+    int seen = 0;
+    for (int a = 0; a < 5; ++a)
+      for (int b2 = 0; b2 < 5; ++b2)
+        for (int c2 = 0; c2 < 5; ++c2)
+          if (a == 4 || b2 == 4 || c2 == 4) {
+            if (seen == lane) { sk[0] = a; sk[1] = b2; sk[2] = c2; shell_ok = true; }
+            ++seen;
           }
-        }
+  } else {
+    mk[0] = 0; mk[1] = lane >> 3; mk[2] = lane & 7;
+    sk[0] = sk[1] = sk[2] = 0;
+  }
+
+  for (int64_t it = xcd_remap(blockIdx.x, gridDim.x); it < nitems;
+       it += gridDim.x) {
+    const int4 item = items[it];
+    const int tile = item.x, start = item.y, count = item.z;
+    const bool shared = item.w != 0;
+    int origin[3];
+    {
+      int r = tile;
+      for (int d = 2; d >= 0; --d) {
+        origin[d] = (r % B.ntiles[d]) * B.tile[d];
+        r /= B.ntiles[d];
       }
-      live = live && any;
     }
-    if (!live) continue;
-    T val[CC];
-#pragma unroll
-    for (int c = 0; c < CC; ++c)
-      val[c] = c < nc ? v[(c0 + c) * B.npts + p] : C::zero();
-#pragma unroll
-    for (int kx = 0; kx < (NDIM >= 3 ? 8 : 1); ++kx) {
-      if (kx >= cnt[0]) break;
-      if (l[0][kx] < 0) continue;
-#pragma unroll
-      for (int ky = 0; ky < 8; ++ky) {
-        if (ky >= cnt[1]) break;
-        if (l[1][ky] < 0) continue;
-        const float wxy = w[0][kx] * w[1][ky];
-        const int row = (l[0][kx] * B.tile[1] + l[1][ky]) * B.tile[2];
-#pragma unroll
-        for (int kz = 0; kz < 8; ++kz) {
-          if (kz >= cnt[2]) break;
-          if (l[2][kz] < 0) continue;
-          const float wt = wxy * w[2][kz];
-          float* cellp = acc + (row + l[2][kz]) * (CC * COMP);
+    for (int64_t c0 = 0; c0 < nbatch; c0 += CC) {
+      const int nc = (int)(nbatch - c0 < CC ? nbatch - c0 : CC);
+      for (int i = lane; i < NP * pcells; i += GW) acc[i] = 0.0f;
+      for (int b0 = 0; b0 < count; b0 += GW) {
+        const int nb = count - b0 < GW ? count - b0 : GW;
+        __syncthreads();
+        // ---- lanes = points: first tap, counts, windows and values
+        if (lane < nb) {
+          const int q = tlist[start + b0 + lane];
+          const int p = perm[q];
 #pragma unroll
           for (int c = 0; c < CC; ++c) {
+            const T x = c < nc ? v[(c0 + c) * B.npts + p] : C::zero();
             if constexpr (CPLX) {
-              atomicAdd(cellp + 2 * c, wt * val[c].x);
-              atomicAdd(cellp + 2 * c + 1, wt * val[c].y);
+              s_v[(2 * c) * GW + lane] = x.x;
+              s_v[(2 * c + 1) * GW + lane] = x.y;
             } else {
-              atomicAdd(cellp + c, wt * val[c]);
+              s_v[c * GW + lane] = x;
+            }
+          }
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            int lo = 0, cnt = 1;
+            float t = 0.f;
+            TapRange r;
+            r.g0 = 0;
+            r.count = 1;
+            if (d >= 3 - NDIM) {
+              t = slocs[(int64_t)q * NDIM + d - (3 - NDIM)];
+              r = tap_range(t, G.half[d], G.ntap[d], G.n[d], G.pad);
+              const int64_t gw =
+                  G.pad == TNL_PAD_CIRCULAR ? mod_pos(r.g0, G.n[d]) : r.g0;
+              lo = (int)(gw - origin[d]);
+              cnt = r.count;
+            }
+            s_lo[d * GW + lane] = lo;
+            s_cnt[d * GW + lane] = cnt;
+#pragma unroll
+            for (int k = 0; k < NT; ++k) {
+              float w = (d < 3 - NDIM && k == 0) ? 1.0f : 0.0f;
+              if (d >= 3 - NDIM && k < r.count)
+                w = window(G.kernel,
+                           fminf(fabsf((float)(r.g0 + k) - t) * G.inv_half[d], 1.0f),
+                           G.beta);
+              s_w[(d * NT + k) * GW + lane] = w;
+            }
+          }
+        }
+        __syncthreads();
+        // ---- lanes = taps: walk the batch point by point.  The next
+        // point's table entries are loaded before this point's
+        // read-modify-write so their LDS latency overlaps it.
+        int nx_lo[3], nx_n[3];
+        float nx_w[3];
+        float nx_v[NP];
+        auto fetch = [&](int j, const int* k) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            nx_lo[d] = s_lo[d * GW + j];
+            nx_n[d] = s_cnt[d * GW + j];
+            nx_w[d] = s_w[(d * NT + (k[d] < NT ? k[d] : 0)) * GW + j];
+          }
+#pragma unroll
+          for (int c = 0; c < NP; ++c) nx_v[c] = s_v[c * GW + j];
+        };
+        fetch(0, mk);
+        for (int j = 0; j < nb; ++j) {
+          int lo[3], n[3];
+          float w3[3], vv[NP];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            lo[d] = nx_lo[d];
+            n[d] = nx_n[d];
+            w3[d] = nx_w[d];
+          }
+#pragma unroll
+          for (int c = 0; c < NP; ++c) vv[c] = nx_v[c];
+          if (j + 1 < nb) fetch(j + 1, mk);
+          const bool fifth = NDIM >= 3 && (n[0] > 4 || n[1] > 4 || n[2] > 4);
+#pragma unroll
+          for (int pass = 0; pass < (NDIM >= 3 ? 2 : 1); ++pass) {
+            if (pass == 1 && !fifth) break;
+            const int* k = pass ? sk : mk;
+            bool ok = (pass == 0 || shell_ok) && k[0] < n[0] && k[1] < n[1] && k[2] < n[2];
+            int l[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              l[d] = lo[d] + k[d];
+              // footprint wrapping past the end of the grid
+              if (G.pad == TNL_PAD_CIRCULAR && origin[d] + l[d] >= G.n[d])
+                l[d] -= (int)G.n[d];
+              ok = ok && l[d] >= 0 && l[d] < B.tile[d];
+            }
+            if (ok) {
+              float w;
+              if (pass == 0) {
+                w = w3[0] * w3[1] * w3[2];
+              } else {
+                w = s_w[(0 * NT + k[0]) * GW + j] * s_w[(1 * NT + k[1]) * GW + j] *
+                    s_w[(2 * NT + k[2]) * GW + j];
+              }
+              float* cp = acc + (l[0] * B.tile[1] + l[1]) * zp + l[2];
+              float old[NP];
+#pragma unroll
+              for (int c = 0; c < NP; ++c) old[c] = cp[c * pcells];
+#pragma unroll
+              for (int c = 0; c < NP; ++c) cp[c * pcells] = fmaf(w, vv[c], old[c]);
             }
           }
         }
       }
-    }
-  }
-  __syncthreads();
-
-  // ---- write the tile: coalesced along z, one channel plane at a time
-  const int t12 = B.tile[1] * B.tile[2];
-  T* __restrict__ o = reinterpret_cast<T*>(out);
-  const int total = B.tcells * nc;
-  for (int idx = threadIdx.x; idx < total; idx += BIN_THREADS) {
-    const int c = idx / B.tcells;
-    const int cell = idx - c * B.tcells;
-    const int lx = cell / t12, rem = cell - lx * t12;
-    const int ly = rem / B.tile[2], lz = rem - ly * B.tile[2];
-    const int64_t gx = origin[0] + lx, gy = origin[1] + ly, gz = origin[2] + lz;
-    if (gx >= G.n[0] || gy >= G.n[1] || gz >= G.n[2]) continue;
-    const int64_t gi = (c0 + c) * G.ncells + (gx * G.n[1] + gy) * G.n[2] + gz;
-    const float* a = acc + cell * (CC * COMP) + c * COMP;
-    if (!shared) {
-      if constexpr (CPLX)
-        o[gi] = make_float2(a[0], a[1]);
-      else
-        o[gi] = a[0];
-    } else {
-      if constexpr (CPLX) {
-        unsafeAtomicAdd(out + 2 * gi, a[0]);
-        unsafeAtomicAdd(out + 2 * gi + 1, a[1]);
-      } else {
-        unsafeAtomicAdd(out + gi, a[0]);
+      __syncthreads();
+      // ---- write the tile: coalesced along z, one channel plane at a time
+      T* __restrict__ o = reinterpret_cast<T*>(out);
+      const int t12 = B.tile[1] * B.tile[2];
+      const int total = B.tcells * nc;
+      for (int idx = lane; idx < total; idx += GW) {
+        const int c = idx / B.tcells;
+        const int cell = idx - c * B.tcells;
+        const int lx = cell / t12, rem = cell - lx * t12;
+        const int ly = rem / B.tile[2], lz = rem - ly * B.tile[2];
+        const int64_t gx = origin[0] + lx, gy = origin[1] + ly,
+                      gz = origin[2] + lz;
+        if (gx >= G.n[0] || gy >= G.n[1] || gz >= G.n[2]) continue;
+        const int64_t gi = (c0 + c) * G.ncells + (gx * G.n[1] + gy) * G.n[2] + gz;
+        const int pc = (lx * B.tile[1] + ly) * zp + lz;
+        const float a0 = acc[(c * COMP) * pcells + pc];
+        const float a1 = CPLX ? acc[(c * COMP + 1) * pcells + pc] : 0.0f;
+        if (!shared) {
+          if constexpr (CPLX)
+            o[gi] = make_float2(a0, a1);
+          else
+            o[gi] = a0;
+        } else {
+          if constexpr (CPLX) {
+            unsafeAtomicAdd(out + 2 * gi, a0);
+            unsafeAtomicAdd(out + 2 * gi + 1, a1);
+          } else {
+            unsafeAtomicAdd(out + gi, a0);
+          }
+        }
       }
+      __syncthreads();
     }
   }
 }
@@ -367,25 +461,44 @@ static BinGeom make_bin_geom(const tnl_plan_layout& L, bool adjoint) {
   return B;
 }
 
-bool binned_supported(const InterpGeom& G, const tnl_plan_layout& L) {
+static size_t grid_lds_bytes(const BinGeom& B, int planes, int nt) {
+  const size_t pcells = (size_t)B.tile[0] * B.tile[1] * (B.tile[2] + 1);
+  return planes * pcells * 4 + (6 * GW + 3 * nt * GW + planes * GW) * 4;
+}
+
+bool binned_supported(const InterpGeom& G, const tnl_plan_layout& L,
+                      bool adjoint) {
   if (G.ndim < 2 || G.norm != 1) return false;
   for (int d = 0; d < 3; ++d)
     if (G.ntap[d] > 8 || G.ntap[d] != L.ntaps[d]) return false;
-  const BinGeom Bu = make_bin_geom(L, false);
-  const BinGeom Bg = make_bin_geom(L, true);
-  // at least one channel must fit in LDS for both directions
-  return (int64_t)Bu.wcells * 8 <= LDS_BUDGET &&
-         (int64_t)Bg.tcells * 8 <= LDS_BUDGET;
+  if (adjoint) {
+    for (int d = 3 - G.ndim; d < 3; ++d) {
+      if (G.ntap[d] > 5) return false;       // gridding lanes cover 5 taps
+      if (G.n[d] < G.ntap[d]) return false;  // a footprint would self-wrap
+    }
+    return (int64_t)grid_lds_bytes(make_bin_geom(L, true), 2, 5) <= L.lds_budget;
+  }
+  // at least one channel of the window must fit in LDS
+  return (int64_t)make_bin_geom(L, false).wcells * 8 <= L.lds_budget;
 }
 
-static int pick_cc(int64_t nbatch, int64_t cells, int comp_bytes) {
+static int pick_cc(int64_t nbatch, int64_t cells, int comp_bytes,
+                   int64_t budget) {
   int cc = 1;
-  while (cc < 8 && cc < nbatch && (int64_t)cells * comp_bytes * cc * 2 <= LDS_BUDGET)
+  while (cc < 8 && cc < nbatch && (int64_t)cells * comp_bytes * cc * 2 <= budget)
     cc *= 2;
   return cc;
 }
 
-template <int NDIM, bool CPLX, int CC>
+// Dynamic LDS above the default 64 KiB needs an explicit opt-in per kernel.
+static int allow_lds(const void* fn, size_t bytes) {
+  if (bytes <= 65536) return TNL_OK;
+  return check_hip(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)bytes),
+                   "hipFuncSetAttribute(MaxDynamicSharedMemorySize)");
+}
+
+template <int NDIM, bool CPLX, int CC, int NT>
 static int launch_binned(bool adjoint, const float* vals, float* out,
                          int64_t nbatch, const InterpGeom& G,
                          const tnl_plan_layout& L, const char* ws,
@@ -398,11 +511,13 @@ static int launch_binned(bool adjoint, const float* vals, float* out,
   if (!adjoint) {
     const int64_t nwg = (L.sub_capacity + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
     const size_t lds = (size_t)B.wcells * CC * comp_bytes;
-    ungrid_binned_kernel<NDIM, CPLX, CC>
+    if (int rc = allow_lds((const void*)ungrid_binned_kernel<NDIM, CPLX, CC, NT>, lds))
+      return rc;
+    ungrid_binned_kernel<NDIM, CPLX, CC, NT>
         <<<dim3((unsigned)nwg, nchunk), BIN_THREADS, lds, s>>>(
             vals, out, nbatch, G, B, perm, slocs,
             (const int4*)(ws + L.off_sub),
-            (const int*)(ws + L.off_suboff) + L.u_nbins);
+            (const int*)(ws + L.off_suboff) + L.u_nbins + 1);
     TNL_LAUNCH_CHECK("ungrid_binned");
   } else {
     const int* nitems = (const int*)(ws + L.off_itemoff) + L.g_ntiles_total;
@@ -410,30 +525,52 @@ static int launch_binned(bool adjoint, const float* vals, float* out,
     grid_zero_shared_kernel<CPLX><<<1024, 256, 0, s>>>(
         out, nbatch, G, B, items, nitems, (const int*)(ws + L.off_gstarts));
     TNL_LAUNCH_CHECK("grid_zero_shared");
-    const int64_t nwg = (L.item_capacity + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
-    const size_t lds = (size_t)B.tcells * CC * comp_bytes;
-    grid_binned_kernel<NDIM, CPLX, CC>
-        <<<dim3((unsigned)nwg, nchunk), BIN_THREADS, lds, s>>>(
-            vals, out, nbatch, G, B, perm, slocs,
-            (const int*)(ws + L.off_tlist), items, nitems);
+    const size_t lds = grid_lds_bytes(B, CC * (CPLX ? 2 : 1), NT);
+    if (int rc = allow_lds((const void*)grid_binned_kernel<NDIM, CPLX, CC, NT>, lds))
+      return rc;
+    // persistent: a few workgroups per CU walk the work items
+    int64_t nwg = (L.item_capacity + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+    const int64_t cap = 256 * 16;
+    if (nwg > cap) nwg = cap;
+    grid_binned_kernel<NDIM, CPLX, CC, NT><<<(unsigned)nwg, GW, lds, s>>>(
+        vals, out, nbatch, G, B, perm, slocs, (const int*)(ws + L.off_tlist),
+        items, nitems);
     TNL_LAUNCH_CHECK("grid_binned");
   }
   return TNL_OK;
 }
 
-template <int NDIM, bool CPLX>
+template <int NDIM, bool CPLX, int NT>
 static int dispatch_cc(bool adjoint, const float* vals, float* out,
                        int64_t nbatch, const InterpGeom& G,
                        const tnl_plan_layout& L, const char* ws,
                        hipStream_t s) {
   const BinGeom B = make_bin_geom(L, adjoint);
-  const int cc = pick_cc(nbatch, adjoint ? B.tcells : B.wcells, CPLX ? 8 : 4);
-  switch (cc) {
-    case 8: return launch_binned<NDIM, CPLX, 8>(adjoint, vals, out, nbatch, G, L, ws, s);
-    case 4: return launch_binned<NDIM, CPLX, 4>(adjoint, vals, out, nbatch, G, L, ws, s);
-    case 2: return launch_binned<NDIM, CPLX, 2>(adjoint, vals, out, nbatch, G, L, ws, s);
-    default: return launch_binned<NDIM, CPLX, 1>(adjoint, vals, out, nbatch, G, L, ws, s);
+  int cc;
+  if (adjoint) {
+    cc = 1;
+    while (cc < 8 && cc < nbatch &&
+           (int64_t)grid_lds_bytes(B, 2 * cc * (CPLX ? 2 : 1), NT) <= L.lds_budget)
+      cc *= 2;
+  } else {
+    cc = pick_cc(nbatch, B.wcells, CPLX ? 8 : 4, L.lds_budget);
   }
+  switch (cc) {
+    case 8: return launch_binned<NDIM, CPLX, 8, NT>(adjoint, vals, out, nbatch, G, L, ws, s);
+    case 4: return launch_binned<NDIM, CPLX, 4, NT>(adjoint, vals, out, nbatch, G, L, ws, s);
+    case 2: return launch_binned<NDIM, CPLX, 2, NT>(adjoint, vals, out, nbatch, G, L, ws, s);
+    default: return launch_binned<NDIM, CPLX, 1, NT>(adjoint, vals, out, nbatch, G, L, ws, s);
+  }
+}
+
+template <int NDIM, bool CPLX>
+static int dispatch_nt(bool adjoint, const float* vals, float* out,
+                       int64_t nbatch, const InterpGeom& G,
+                       const tnl_plan_layout& L, const char* ws,
+                       hipStream_t s) {
+  const int nt = G.ntap[1] > G.ntap[2] ? G.ntap[1] : G.ntap[2];
+  if (nt <= 5) return dispatch_cc<NDIM, CPLX, 5>(adjoint, vals, out, nbatch, G, L, ws, s);
+  return dispatch_cc<NDIM, CPLX, 8>(adjoint, vals, out, nbatch, G, L, ws, s);
 }
 
 int interp_binned(bool adjoint, const float* vals, float* out, int64_t nbatch,
@@ -441,10 +578,10 @@ int interp_binned(bool adjoint, const float* vals, float* out, int64_t nbatch,
                   const void* plan, hipStream_t s) {
   const char* ws = (const char*)plan;
   if (G.ndim == 3)
-    return is_complex ? dispatch_cc<3, true>(adjoint, vals, out, nbatch, G, L, ws, s)
-                      : dispatch_cc<3, false>(adjoint, vals, out, nbatch, G, L, ws, s);
-  return is_complex ? dispatch_cc<2, true>(adjoint, vals, out, nbatch, G, L, ws, s)
-                    : dispatch_cc<2, false>(adjoint, vals, out, nbatch, G, L, ws, s);
+    return is_complex ? dispatch_nt<3, true>(adjoint, vals, out, nbatch, G, L, ws, s)
+                      : dispatch_nt<3, false>(adjoint, vals, out, nbatch, G, L, ws, s);
+  return is_complex ? dispatch_nt<2, true>(adjoint, vals, out, nbatch, G, L, ws, s)
+                    : dispatch_nt<2, false>(adjoint, vals, out, nbatch, G, L, ws, s);
 }
 
 }  // namespace tnl

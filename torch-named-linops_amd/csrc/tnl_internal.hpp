@@ -13,12 +13,14 @@ int interp_direct(bool adjoint, const float* vals, const float* locs,
 
 // Binned (plan-driven) path; returns TNL_EINVAL with no launch when the
 // problem is outside what the binned kernels cover (caller then falls back).
-bool binned_supported(const InterpGeom& G, const tnl_plan_layout& L);
+bool binned_supported(const InterpGeom& G, const tnl_plan_layout& L,
+                      bool adjoint);
 int interp_binned(bool adjoint, const float* vals, float* out, int64_t nbatch,
                   const InterpGeom& G, int is_complex, const tnl_plan_layout& L,
                   const void* plan, hipStream_t s);
 
 int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
-                const float* width, int pad_mode, tnl_plan_layout* L);
+                const float* width, int pad_mode, const tnl_tuning* tune,
+                tnl_plan_layout* L);
 
 }  // namespace tnl

@@ -30,7 +30,6 @@ struct PlanDev {
   int64_t npts;
   int max_sub;
   int g_cap;
-  int gmaxper[3];  // max distinct gridding tiles touched per dim
 };
 
 static int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
@@ -40,13 +39,14 @@ static int default_u_tile(int ndim) {
 }
 static void default_g_tile(int ndim, int t[3]) {
   // innermost (contiguous) axis longest: coalesced tile writes
-  if (ndim == 3) { t[0] = 8; t[1] = 8; t[2] = 16; }
+  if (ndim == 3) { t[0] = 8; t[1] = 8; t[2] = 8; }
   else if (ndim == 2) { t[0] = 1; t[1] = 16; t[2] = 64; }
   else { t[0] = 1; t[1] = 1; t[2] = 4096; }
 }
 
 int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
-                const float* width, int pad_mode, tnl_plan_layout* L) {
+                const float* width, int pad_mode, const tnl_tuning* tune,
+                tnl_plan_layout* L) {
   if (ndim < 1 || ndim > 3) return fail(TNL_EINVAL, "ndim must be 1, 2 or 3");
   if (npts < 0) return fail(TNL_EINVAL, "npts must be >= 0");
   if (npts > (int64_t(1) << 28))
@@ -56,11 +56,22 @@ int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
   std::memset(L, 0, sizeof(*L));
   L->ndim = ndim;
   L->npts = npts;
+  L->pad_mode = pad_mode;
   int gt[3];
   default_g_tile(ndim, gt);
+  int ut_def[3] = {1, 1, 1};
+  for (int d = 3 - ndim; d < 3; ++d) ut_def[d] = default_u_tile(ndim);
+  if (tune) {
+    for (int d = 0; d < 3; ++d) {
+      if (tune->u_tile[d] > 0) ut_def[d] = tune->u_tile[d];
+      if (tune->g_tile[d] > 0) gt[d] = tune->g_tile[d];
+    }
+  }
   int64_t n[3] = {1, 1, 1};
   int64_t touch_per_pt = 1;
   for (int d = 0; d < 3; ++d) {
+    L->grid_size[d] = 1;
+    L->width[d] = 1.0f;
     L->ntaps[d] = 1;
     L->u_tile[d] = 1;
     L->u_ntiles[d] = 1;
@@ -73,10 +84,12 @@ int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
     if (!(width[d] > 0.0f) || !std::isfinite(width[d]))
       return fail(TNL_EINVAL, "kernel width must be positive and finite");
     n[gd] = grid_size[d];
+    L->grid_size[gd] = grid_size[d];
+    L->width[gd] = width[d];
     const int ntap = (int)std::ceil((double)width[d]) + 1;
     if (ntap > TNL_MAX_TAPS) return fail(TNL_EINVAL, "kernel width too large");
     L->ntaps[gd] = ntap;
-    int64_t ut = default_u_tile(ndim);
+    int64_t ut = ut_def[gd];
     if (ut > n[gd]) ut = n[gd];
     L->u_tile[gd] = (int32_t)ut;
     L->u_ntiles[gd] = (int32_t)((n[gd] + ut - 1) / ut);
@@ -89,9 +102,12 @@ int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
   }
   L->u_nbins = (int64_t)L->u_ntiles[0] * L->u_ntiles[1] * L->u_ntiles[2];
   L->g_ntiles_total = (int64_t)L->g_ntiles[0] * L->g_ntiles[1] * L->g_ntiles[2];
-  L->max_sub = 2048;
+  L->max_sub = (tune && tune->max_sub > 0) ? tune->max_sub : 2048;
   L->sub_capacity = L->u_nbins + (npts + L->max_sub - 1) / L->max_sub + 1;
-  L->g_cap = 8192;
+  L->g_cap = (tune && tune->g_cap > 0) ? tune->g_cap : 2048;
+  L->lds_budget = (tune && tune->lds_budget > 0) ? tune->lds_budget : 40960;
+  if (L->lds_budget > 160 * 1024)
+    return fail(TNL_EINVAL, "lds_budget above the 160 KiB LDS of a CU");
   L->touch_capacity = npts * touch_per_pt;
   L->item_capacity =
       L->g_ntiles_total + (L->touch_capacity + L->g_cap - 1) / L->g_cap + 1;
@@ -107,14 +123,12 @@ int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
   L->off_cursor = take(4 * nb1);
   L->off_perm = take(4 * std::max<int64_t>(npts, 1));
   L->off_slocs = take(4 * std::max<int64_t>(npts, 1) * ndim);
-  L->off_subk = take(4 * L->u_nbins);
-  L->off_suboff = take(4 * (L->u_nbins + 1));
+  L->off_suboff = take(4 * (L->u_nbins + 2));
   L->off_sub = take(16 * L->sub_capacity);
   L->off_gcounts = take(4 * L->g_ntiles_total);
   L->off_gstarts = take(4 * (L->g_ntiles_total + 1));
   L->off_gcursor = take(4 * L->g_ntiles_total);
   L->off_tlist = take(4 * std::max<int64_t>(L->touch_capacity, 1));
-  L->off_itemk = take(4 * L->g_ntiles_total);
   L->off_itemoff = take(4 * (L->g_ntiles_total + 1));
   L->off_items = take(16 * L->item_capacity);
   L->bytes = off;
@@ -303,13 +317,15 @@ __global__ __launch_bounds__(1024) void scan_kernel(const int* __restrict__ in,
   if (tid == 1023) out[n] = part[1023];
 }
 
-// Interpolation subproblems: <= max_sub points of one bin each.
+// Interpolation subproblems: <= max_sub points of one bin each.  The dead
+// bin (index u_nbins: points with no taps) gets subproblems too, so that
+// interpolation writes their zero outputs.
 __global__ void sub_fill_kernel(const int* __restrict__ counts,
                                 const int* __restrict__ starts,
                                 const int* __restrict__ suboff, PlanDev P,
                                 int4* __restrict__ sub) {
   for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-       b < P.u_nbins; b += (int64_t)gridDim.x * blockDim.x) {
+       b <= P.u_nbins; b += (int64_t)gridDim.x * blockDim.x) {
     const int c = counts[b];
     const int o = suboff[b];
     for (int j = 0; j * P.max_sub < c; ++j) {
@@ -338,8 +354,7 @@ __global__ void item_fill_kernel(const int* __restrict__ gcounts,
   }
 }
 
-PlanDev make_plan_dev(const tnl_plan_layout& L, const int64_t* grid_size,
-                      const float* width, int pad_mode) {
+PlanDev make_plan_dev(const tnl_plan_layout& L) {
   PlanDev P;
   std::memset(&P, 0, sizeof(P));
   P.ndim = L.ndim;
@@ -352,13 +367,12 @@ PlanDev make_plan_dev(const tnl_plan_layout& L, const int64_t* grid_size,
     P.gtile[d] = L.g_tile[d];
     P.gntiles[d] = L.g_ntiles[d];
   }
-  for (int d = 0; d < L.ndim; ++d) {
-    const int gd = 3 - L.ndim + d;
-    P.n[gd] = grid_size[d];
-    P.half[gd] = width[d] * 0.5f;
+  for (int gd = 3 - L.ndim; gd < 3; ++gd) {
+    P.n[gd] = L.grid_size[gd];
+    P.half[gd] = L.width[gd] * 0.5f;
     P.ntap[gd] = L.ntaps[gd];
   }
-  P.pad = pad_mode;
+  P.pad = L.pad_mode;
   P.u_nbins = L.u_nbins;
   P.g_ntiles_total = L.g_ntiles_total;
   P.npts = L.npts;
@@ -381,19 +395,20 @@ using namespace tnl;
 extern "C" int tnl_plan_layout_query(int64_t npts, int ndim,
                                      const int64_t* grid_size,
                                      const float* width, int pad_mode,
+                                     const tnl_tuning* tuning,
                                      tnl_plan_layout* out) {
   if (!out || !grid_size || !width) return fail(TNL_EINVAL, "null pointer");
-  return plan_layout(npts, ndim, grid_size, width, pad_mode, out);
+  return plan_layout(npts, ndim, grid_size, width, pad_mode, tuning, out);
 }
 
-extern "C" int tnl_plan_build(const float* locs, int64_t npts, int ndim,
-                              const int64_t* grid_size, const float* width,
-                              int pad_mode, void* workspace,
-                              int64_t workspace_bytes, void* stream) {
-  if (!grid_size || !width) return fail(TNL_EINVAL, "null pointer");
-  tnl_plan_layout L;
-  if (int rc = plan_layout(npts, ndim, grid_size, width, pad_mode, &L))
-    return rc;
+extern "C" int tnl_plan_build(const float* locs, const tnl_plan_layout* layout,
+                              void* workspace, int64_t workspace_bytes,
+                              void* stream) {
+  if (!layout) return fail(TNL_EINVAL, "null layout");
+  const tnl_plan_layout L = *layout;
+  if (L.ndim < 1 || L.ndim > 3 || L.npts < 0 || L.bytes <= 0)
+    return fail(TNL_EINVAL, "invalid plan layout");
+  const int64_t npts = L.npts;
   if (!workspace) return fail(TNL_EINVAL, "null workspace");
   if (workspace_bytes < L.bytes)
     return fail(TNL_EWORKSPACE, "plan workspace too small");
@@ -405,7 +420,6 @@ extern "C" int tnl_plan_build(const float* locs, int64_t npts, int ndim,
   int* cursor = (int*)(ws + L.off_cursor);
   int* perm = (int*)(ws + L.off_perm);
   float* slocs = (float*)(ws + L.off_slocs);
-  int* subk = (int*)(ws + L.off_subk);
   int* suboff = (int*)(ws + L.off_suboff);
   int4* sub = (int4*)(ws + L.off_sub);
   int* gcounts = (int*)(ws + L.off_gcounts);
@@ -414,8 +428,7 @@ extern "C" int tnl_plan_build(const float* locs, int64_t npts, int ndim,
   int* tlist = (int*)(ws + L.off_tlist);
   int* itemoff = (int*)(ws + L.off_itemoff);
   int4* items = (int4*)(ws + L.off_items);
-  (void)subk;
-  const PlanDev P = make_plan_dev(L, grid_size, width, pad_mode);
+  const PlanDev P = make_plan_dev(L);
   const int64_t nb1 = L.u_nbins + 1;
 
   zero_ints_kernel<<<grid_for(nb1), PLAN_THREADS, 0, s>>>(counts, nb1);
@@ -439,10 +452,9 @@ extern "C" int tnl_plan_build(const float* locs, int64_t npts, int ndim,
         slocs, P, gcounts, nullptr, nullptr);
     TNL_LAUNCH_CHECK("plan touch count");
   }
-  // subproblems: k_b = ceil(count_b / max_sub) (dead bin excluded)
-  scan_kernel<<<1, 1024, 0, s>>>(counts, suboff, L.u_nbins, (int)L.max_sub,
-                                 0);
-  sub_fill_kernel<<<grid_for(L.u_nbins), PLAN_THREADS, 0, s>>>(
+  // subproblems: k_b = ceil(count_b / max_sub), dead bin included
+  scan_kernel<<<1, 1024, 0, s>>>(counts, suboff, nb1, (int)L.max_sub, 0);
+  sub_fill_kernel<<<grid_for(nb1), PLAN_THREADS, 0, s>>>(
       counts, starts, suboff, P, sub);
   TNL_LAUNCH_CHECK("plan subproblems");
   scan_kernel<<<1, 1024, 0, s>>>(gcounts, gstarts, L.g_ntiles_total, 1, 0);

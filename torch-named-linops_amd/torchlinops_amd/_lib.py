@@ -19,6 +19,7 @@ __all__ = [
     "library_path",
     "TNLError",
     "PlanLayout",
+    "Tuning",
     "KERNELS",
     "PAD_MODES",
 ]
@@ -48,6 +49,19 @@ class TNLError(RuntimeError):
     """A HIP-level failure reported by libtnl."""
 
 
+class Tuning(ctypes.Structure):
+    """Mirror of tnl_tuning (include/tnl.h); 0 means library default."""
+
+    _fields_ = [
+        ("u_tile", ctypes.c_int32 * 3),
+        ("g_tile", ctypes.c_int32 * 3),
+        ("max_sub", ctypes.c_int32),
+        ("g_cap", ctypes.c_int32),
+        ("lds_budget", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 class PlanLayout(ctypes.Structure):
     """Mirror of tnl_plan_layout (include/tnl.h)."""
 
@@ -60,30 +74,35 @@ class PlanLayout(ctypes.Structure):
         ("g_tile", ctypes.c_int32 * 3),
         ("g_ntiles", ctypes.c_int32 * 3),
         ("g_ntiles_total", ctypes.c_int64),
+        ("grid_size", ctypes.c_int64 * 3),
+        ("width", ctypes.c_float * 3),
+        ("pad_mode", ctypes.c_int32),
         ("npts", ctypes.c_int64),
         ("max_sub", ctypes.c_int64),
         ("sub_capacity", ctypes.c_int64),
         ("g_cap", ctypes.c_int64),
         ("touch_capacity", ctypes.c_int64),
         ("item_capacity", ctypes.c_int64),
+        ("lds_budget", ctypes.c_int64),
         ("off_counts", ctypes.c_int64),
         ("off_starts", ctypes.c_int64),
         ("off_cursor", ctypes.c_int64),
         ("off_perm", ctypes.c_int64),
         ("off_slocs", ctypes.c_int64),
-        ("off_subk", ctypes.c_int64),
         ("off_suboff", ctypes.c_int64),
         ("off_sub", ctypes.c_int64),
         ("off_gcounts", ctypes.c_int64),
         ("off_gstarts", ctypes.c_int64),
         ("off_gcursor", ctypes.c_int64),
         ("off_tlist", ctypes.c_int64),
-        ("off_itemk", ctypes.c_int64),
         ("off_itemoff", ctypes.c_int64),
         ("off_items", ctypes.c_int64),
         ("bytes", ctypes.c_int64),
     ]
 
+
+_layout_p = ctypes.POINTER(PlanLayout)
+_tuning_p = ctypes.POINTER(Tuning)
 
 # (name, restype, argtypes) for every symbol in include/tnl.h
 SIGNATURES = {
@@ -91,21 +110,18 @@ SIGNATURES = {
     "tnl_abi_version": (_i32, []),
     "tnl_plan_layout_query": (
         _i32,
-        [_i64, _i32, _i64p, _f32p, _i32, ctypes.POINTER(PlanLayout)],
+        [_i64, _i32, _i64p, _f32p, _i32, _tuning_p, _layout_p],
     ),
-    "tnl_plan_build": (
-        _i32,
-        [_vp, _i64, _i32, _i64p, _f32p, _i32, _vp, _i64, _vp],
-    ),
+    "tnl_plan_build": (_i32, [_vp, _layout_p, _vp, _i64, _vp]),
     "tnl_ungrid": (
         _i32,
         [_vp, _vp, _vp, _i64, _i64, _i32, _i64p, _f32p, _i32, _i32, _i32,
-         _i32, _f32, _vp, _i64, _vp],
+         _i32, _f32, _layout_p, _vp, _vp],
     ),
     "tnl_grid": (
         _i32,
         [_vp, _vp, _vp, _i64, _i64, _i32, _i64p, _f32p, _i32, _i32, _i32,
-         _i32, _f32, _vp, _i64, _vp],
+         _i32, _f32, _layout_p, _vp, _vp],
     ),
     "tnl_unfold": (
         _i32,

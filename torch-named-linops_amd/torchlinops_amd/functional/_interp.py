@@ -124,8 +124,8 @@ def _launch(adjoint: bool, vals_flat, locs, locs2d, out, nbatch, npts, ndim,
                 pad_code,
                 int(is_complex),
                 beta,
-                None if plan is None else plan.data_ptr(),
-                0 if plan is None else plan.numel(),
+                None if plan is None else plan.layout,
+                None if plan is None else plan.ptr,
                 stream_of(vals_flat),
             )
         )

@@ -1,11 +1,11 @@
 """Point-binning plans for the 2-D/3-D interpolation kernels, cached per
 trajectory tensor.
 
-A plan (see tnl_plan_build in include/tnl.h) depends only on the locations,
-the grid size, the kernel width and the padding mode.  It is keyed on the
-identity and version counter of the caller's ``locs`` tensor, so in-place
-edits of ``locs`` invalidate it, and a weak reference guards against a new
-tensor reusing a freed tensor's id.
+A plan (tnl_plan_layout_query / tnl_plan_build in include/tnl.h) depends
+only on the locations, grid size, kernel width, padding mode and tiling.  It
+is keyed on the identity and version counter of the caller's ``locs``
+tensor, so in-place edits of ``locs`` invalidate it, and a weak reference
+guards against a new tensor reusing a freed tensor's id.
 """
 
 from __future__ import annotations
@@ -13,26 +13,91 @@ from __future__ import annotations
 import threading
 import weakref
 from collections import OrderedDict
+from dataclasses import dataclass
+from typing import Optional
 
 import torch
 
 from .. import config
-from .._lib import PlanLayout, check, f32_array, i64_array, lib, stream_of
+from .._lib import PlanLayout, Tuning, check, f32_array, i64_array, lib, stream_of
 
-__all__ = ["get_plan", "plan_layout", "clear_plan_cache"]
+__all__ = ["Plan", "get_plan", "build_plan", "plan_layout", "clear_plan_cache",
+           "set_tuning", "get_tuning"]
 
 _MAX_PLANS = 4
 _AUTO_MIN_POINTS = 1 << 16
 
+# per-ndim tuning overrides (see autotune.py); {} = library defaults
+_tuning: dict = {}
+_tuning_lock = threading.Lock()
 
-def plan_layout(npts, ndim, grid_size, width, pad_mode_code) -> PlanLayout:
+
+def set_tuning(ndim: int, **knobs) -> None:
+    """Set tiling knobs for ndim-D plans: u_tile, g_tile (3-tuples over the
+    left-padded axes), max_sub, g_cap, lds_budget.  Cached plans are
+    dropped so the next call builds with the new tiling."""
+    allowed = {"u_tile", "g_tile", "max_sub", "g_cap", "lds_budget"}
+    bad = set(knobs) - allowed
+    if bad:
+        raise ValueError(f"unknown tuning knobs {bad}")
+    with _tuning_lock:
+        cur = dict(_tuning.get(ndim, {}))
+        cur.update({k: v for k, v in knobs.items() if v is not None})
+        _tuning[ndim] = cur
+    clear_plan_cache()
+
+
+def get_tuning(ndim: int) -> dict:
+    with _tuning_lock:
+        return dict(_tuning.get(ndim, {}))
+
+
+def _tuning_struct(ndim: int, overrides: Optional[dict] = None) -> Tuning:
+    knobs = get_tuning(ndim)
+    if overrides:
+        knobs.update(overrides)
+    t = Tuning()
+    for k in ("u_tile", "g_tile"):
+        if k in knobs:
+            v = tuple(int(x) for x in knobs[k])
+            if len(v) != 3:
+                raise ValueError(f"{k} must have 3 entries (left-padded axes)")
+            getattr(t, k)[:] = v
+    for k in ("max_sub", "g_cap", "lds_budget"):
+        if k in knobs:
+            setattr(t, k, int(knobs[k]))
+    return t
+
+
+def plan_layout(npts, ndim, grid_size, width, pad_mode_code, tuning=None) -> PlanLayout:
     L = PlanLayout()
+    t = _tuning_struct(ndim, tuning)
     check(
         lib().tnl_plan_layout_query(
-            npts, ndim, i64_array(grid_size), f32_array(width), pad_mode_code, L
+            npts, ndim, i64_array(grid_size), f32_array(width), pad_mode_code, t, L
         )
     )
     return L
+
+
+@dataclass
+class Plan:
+    layout: PlanLayout
+    workspace: torch.Tensor
+
+    @property
+    def ptr(self) -> int:
+        return self.workspace.data_ptr()
+
+
+def build_plan(locs2d: torch.Tensor, ndim, grid_size, width, pad_code, tuning=None) -> Plan:
+    npts = locs2d.shape[0]
+    L = plan_layout(npts, ndim, grid_size, width, pad_code, tuning)
+    ws = torch.empty(int(L.bytes), dtype=torch.uint8, device=locs2d.device)
+    with torch.cuda.device(locs2d.device):
+        check(lib().tnl_plan_build(locs2d.data_ptr(), L, ws.data_ptr(), int(L.bytes),
+                                   stream_of(locs2d)))
+    return Plan(L, ws)
 
 
 class _PlanCache:
@@ -42,7 +107,7 @@ class _PlanCache:
         self._lock = threading.Lock()
 
     @staticmethod
-    def _key(locs, ndim, grid_size, width, pad_code):
+    def key(locs, ndim, grid_size, width, pad_code):
         return (
             id(locs),
             locs._version,
@@ -93,29 +158,8 @@ def clear_plan_cache() -> None:
     _cache.clear()
 
 
-def build_plan(locs2d: torch.Tensor, ndim, grid_size, width, pad_code) -> torch.Tensor:
-    npts = locs2d.shape[0]
-    L = plan_layout(npts, ndim, grid_size, width, pad_code)
-    ws = torch.empty(int(L.bytes), dtype=torch.uint8, device=locs2d.device)
-    with torch.cuda.device(locs2d.device):
-        check(
-            lib().tnl_plan_build(
-                locs2d.data_ptr(),
-                npts,
-                ndim,
-                i64_array(grid_size),
-                f32_array(width),
-                pad_code,
-                ws.data_ptr(),
-                int(L.bytes),
-                stream_of(locs2d),
-            )
-        )
-    return ws
-
-
-def get_plan(locs, locs2d, ndim, grid_size, width, norm, pad_code):
-    """Plan workspace for this trajectory, or None to use direct kernels."""
+def get_plan(locs, locs2d, ndim, grid_size, width, norm, pad_code) -> Optional[Plan]:
+    """Plan for this trajectory, or None to use the direct kernels."""
     mode = config.interp_plan
     if mode not in ("auto", "always", "never"):
         raise ValueError(f"config.interp_plan must be auto/always/never, got {mode!r}")
@@ -124,7 +168,7 @@ def get_plan(locs, locs2d, ndim, grid_size, width, norm, pad_code):
     npts = locs2d.shape[0]
     if npts == 0 or npts > (1 << 28):
         return None
-    key = _cache._key(locs, ndim, grid_size, width, pad_code)
+    key = _cache.key(locs, ndim, grid_size, width, pad_code)
     plan = _cache.lookup(locs, key)
     if plan is not None:
         return plan
