@@ -1,0 +1,162 @@
+"""GPU checks of the linop layer: the linop contract (adjoint, normal,
+level-2, autograd, no input mutation) on Interpolate, ArrayToBlocks and
+NUFFT; NUFFT values against the float64 oracle and the exact nonuniform
+DFT; the Toeplitz normal against A^H A; CG solvers against a dense solve."""
+
+from math import prod
+
+import numpy as np
+import pytest
+import torch
+
+import torchlinops_amd as tl
+import torchlinops_amd.functional as F
+from oracle import oracle as O
+from torchlinops_amd.testing import BaseNamedLinopTests
+from tests._util import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("GPU is required but not available")
+    return torch.device("cuda:0")
+
+
+class TestInterp(BaseNamedLinopTests):
+    equality_check = "approx"
+    isclose_kwargs = {"rtol": 1e-4, "atol": 1e-4}
+
+    @pytest.fixture(scope="class")
+    def linop_input_output(self):
+        dev = _gpu()
+        g = torch.Generator().manual_seed(0)
+        grid = (10, 8, 9)
+        locs = (torch.rand(3, 5, 3, generator=g) * torch.tensor(grid)).to(dev)
+        A = tl.Interpolate(locs, grid, batch_shape=("C",), locs_batch_shape=("R", "K"),
+                           grid_shape=("Nx", "Ny", "Nz"), width=4.0, kernel="kaiser_bessel",
+                           kernel_params={"beta": 3.0})
+        x = torch.randn(2, *grid, dtype=torch.complex64, generator=g).to(dev)
+        y = torch.randn(2, 3, 5, dtype=torch.complex64, generator=g).to(dev)
+        return A, x, y
+
+
+class TestArrayToBlocks(BaseNamedLinopTests):
+    equality_check = "approx"
+    isclose_kwargs = {"rtol": 1e-4, "atol": 1e-4}
+
+    @pytest.fixture(scope="class")
+    def linop_input_output(self):
+        dev = _gpu()
+        mask = torch.zeros(3, 3, 3, dtype=torch.bool)
+        mask[1, 1, :] = mask[1, :, 1] = mask[:, 1, 1] = True
+        A = tl.ArrayToBlocks((15, 15, 15), (3, 3, 3), (1, 1, 1), mask.to(dev))
+        nb = F.get_nblocks((15, 15, 15), (3, 3, 3), (1, 1, 1))
+        x = torch.randn(1, 15, 15, 15, dtype=torch.complex64, device=dev)
+        y = torch.randn(1, *nb, int(mask.sum()), dtype=torch.complex64, device=dev)
+        return A, x, y
+
+
+class TestNUFFT(BaseNamedLinopTests):
+    equality_check = "approx"
+    isclose_kwargs = {"rtol": 1e-3, "atol": 1e-4}
+
+    @pytest.fixture(scope="class")
+    def linop_input_output(self):
+        dev = _gpu()
+        g = torch.Generator().manual_seed(1)
+        grid = (32, 32, 32)
+        locs = ((torch.rand(3, 5, 3, generator=g) - 0.5) * 28).to(dev)
+        A = tl.NUFFT(locs.clone(), grid, output_shape=("R", "K"), width=4.0, oversamp=1.25)
+        x = (0.5 * torch.rand(2, 1, *grid, dtype=torch.complex64, generator=g) + 1).to(dev)
+        y = (0.5 * torch.rand(2, 1, 3, 5, dtype=torch.complex64, generator=g) + 1).to(dev)
+        return A, x, y
+
+    def test_normal_fn(self, linop_input_output):
+        A, x, y = linop_input_output
+        assert torch.isclose(A.N(x), A.normal_fn(A, x.clone()), rtol=1e-2, atol=1e-3).all()
+
+
+@pytest.mark.parametrize("grid_size,npts,os_", [((24, 20), 300, 2.0), ((40, 36), 900, 1.25),
+                                                ((16, 18, 14), 500, 1.25)])
+def test_nufft_vs_oracle_and_dft(grid_size, npts, os_):
+    dev = _gpu()
+    rng = np.random.default_rng(3)
+    ndim = len(grid_size)
+    locs = (rng.uniform(-0.5, 0.5, (npts, ndim)) * np.array(grid_size)).astype(np.float32)
+    x = (rng.standard_normal((2, *grid_size)) + 1j * rng.standard_normal((2, *grid_size))).astype(np.complex64)
+    y = (rng.standard_normal((2, npts)) + 1j * rng.standard_normal((2, npts))).astype(np.complex64)
+    A = tl.NUFFT(torch.from_numpy(locs).to(dev), grid_size, output_shape=("K",),
+                 oversamp=os_, width=4.0)
+    xt, yt = torch.from_numpy(x).to(dev), torch.from_numpy(y).to(dev)
+    got = A(xt).cpu().numpy()
+    got_h = A.H(yt).cpu().numpy()
+    want = O.nufft(x, locs.astype(np.float64), oversamp=os_, width=4.0)
+    want_h = O.nufft_adjoint(y, locs.astype(np.float64), grid_size, oversamp=os_, width=4.0)
+    e, eh = rel_l2(got, want), rel_l2(got_h, want_h)
+    print(f"NUFFT {grid_size} os={os_}: fwd rel-L2 vs f64 oracle {e:.2e}, adj {eh:.2e}")
+    assert e < 1e-5 and eh < 1e-5
+    # the approximation itself: close to the exact DFT
+    dft = O.dft(x, locs.astype(np.float64))
+    assert rel_l2(got, dft) < (5e-3 if os_ >= 2 else 2e-2)
+    # functional interface gives the same numbers
+    f = F.nufft(xt, torch.from_numpy(locs).to(dev), oversamp=os_, width=4.0).cpu().numpy()
+    fh = F.nufft_adjoint(yt, torch.from_numpy(locs).to(dev), grid_size, oversamp=os_, width=4.0).cpu().numpy()
+    assert rel_l2(f, got) < 1e-6 and rel_l2(fh, got_h) < 1e-6
+
+
+def test_toeplitz_normal_matches_explicit_normal():
+    dev = _gpu()
+    rng = np.random.default_rng(4)
+    grid = (24, 24, 24)
+    locs = torch.from_numpy((rng.uniform(-0.5, 0.5, (2000, 3)) * 24).astype(np.float32)).to(dev)
+    A = tl.NUFFT(locs, grid, output_shape=("K",), batch_shape=("C",), oversamp=2.0)
+    At = tl.NUFFT(locs, grid, output_shape=("K",), batch_shape=("C",), oversamp=2.0, toeplitz=True)
+    x = torch.randn(2, *grid, dtype=torch.complex64, device=dev)
+    explicit = A.H(A(x))
+    toep = At.N(x)
+    err = rel_l2(toep.cpu().numpy(), explicit.cpu().numpy())
+    print("toeplitz vs explicit normal rel-L2:", err)
+    assert err < 2e-3
+    # Toeplitz chain: Pad, FFT, K, FFT^H, Crop
+    assert [type(l).__name__ for l in At.N.linops] == ["Pad", "FFT", "Diagonal", "FFT", "Pad"]
+
+
+def test_cg_on_nufft_normal():
+    dev = _gpu()
+    rng = np.random.default_rng(5)
+    grid = (16, 16)
+    locs = torch.from_numpy((rng.uniform(-0.5, 0.5, (600, 2)) * 16).astype(np.float32)).to(dev)
+    A = tl.NUFFT(locs, grid, output_shape=("K",), oversamp=2.0)
+    xt = torch.randn(*grid, dtype=torch.complex64, device=dev)
+    lam = 1e-2
+    b = A.H(A(xt)) + lam * xt
+    s = tl.DeviceCG(A.N, lam=lam, check_every=5)
+    x = s.solve(b, tol=1e-6, max_iters=300)
+    print("DeviceCG iters", s.iters, "rel residual", s.history[-1])
+    assert s.history[-1] <= 1e-6
+    assert rel_l2(x.cpu().numpy(), xt.cpu().numpy()) < 1e-3
+    x2 = tl.conjugate_gradients(lambda v: A.N(v) + lam * v, b, max_num_iters=100,
+                                disable_tracking=True)
+    assert rel_l2(x2.cpu().numpy(), xt.cpu().numpy()) < 1e-3
+
+
+def test_cg_graph_replays_match_eager():
+    """A captured block of CG iterations, replayed several times with eager
+    work and synchronisations in between, matches eager CG step for step."""
+    dev = _gpu()
+    rng = np.random.default_rng(6)
+    grid = (16, 16)
+    locs = torch.from_numpy((rng.uniform(-0.5, 0.5, (500, 2)) * 16).astype(np.float32)).to(dev)
+    A = tl.NUFFT(locs, grid, output_shape=("K",), oversamp=2.0)
+    b = torch.randn(*grid, dtype=torch.complex64, device=dev)
+    eager = tl.DeviceCG(A.N, lam=1e-2, check_every=4, use_graph=False)
+    graph = tl.DeviceCG(A.N, lam=1e-2, check_every=4, use_graph=True)
+    xe = eager.solve(b, tol=0.0, max_iters=16)
+    xg = graph.solve(b, tol=0.0, max_iters=16)
+    torch.cuda.synchronize()
+    assert graph.iters == eager.iters == 16
+    assert rel_l2(xg.cpu().numpy(), xe.cpu().numpy()) < 1e-4
+    for he, hg in zip(eager.history, graph.history):
+        assert abs(he - hg) <= 1e-3 * max(he, 1e-12)

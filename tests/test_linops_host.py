@@ -1,0 +1,142 @@
+"""Host-side (CPU) checks of the named-dimension core and the linops that
+are plain torch ops (Pad, FFT, Diagonal, Scalar, Chain algebra, NUFFT
+construction and constants).  The interpolation linops themselves run only
+on the GPU (tests/test_gpu_linops.py)."""
+
+import numpy as np
+import pytest
+import torch
+
+import torchlinops_amd as tl
+from torchlinops_amd import config
+from torchlinops_amd.nameddim import ND, NS, Dim, isequal, resolve_wildcards
+from oracle import oracle as O
+
+
+def test_dim_parsing_examples():
+    assert Dim("ABCD") == ("A", "B", "C", "D")
+    assert Dim("NxNyNz") == ("Nx", "Ny", "Nz")
+    assert Dim("A1B2Kx1Ky2") == ("A1", "B2", "Kx1", "Ky2")
+    assert Dim("()A()B") == ("()", "A", "()", "B")
+    assert Dim("(1)A(23)") == ("(1)", "A", "(23)")
+    assert Dim("") == ()
+
+
+def test_named_dimension():
+    assert repr(ND("A")) == "A" and repr(ND("A", 1)) == "A1"
+    assert ND("A") == "A" and ND.infer("B2") == ND("B", 2)
+    assert ND("A").next_unused(("A", "A1")) == "A2"
+    assert ND("...").next_unused(("...",)) == "..."
+
+
+@pytest.mark.parametrize("a,b,want", [
+    (("A", "B"), ("A", "B"), True), (("A", "C"), ("A",), False), (("A", "C"), (), False),
+    (("A", "C"), ("...",), True), (("A", "C", "..."), ("...",), True),
+    (("A", "B", "C"), ("A", "...", "C"), True), (("...", "A", "C", "..."), ("...",), True),
+    (("...", "A", "C"), ("B", "C"), False), (("A", "B"), ("A", "()"), True),
+    (("A",), ("()", "()"), False),
+])
+def test_isequal_table(a, b, want):
+    assert isequal(a, b)[0] is want
+
+
+def test_shared_dim_rename_and_shapes():
+    s = NS(("A", "B"), ("B", "C"))
+    s.ishape = ("D", "E")
+    assert s.ishape == (ND("D"), ND("E")) and s.oshape == (ND("E"), ND("C"))
+    s2 = NS(("...", "Nx"), ("...", "Kx"), extra=("Nx",))
+    s2.ishape = ("C", "Nx")
+    assert s2.ishape == ("C", "Nx") and s2.oshape == ("...", "Kx")
+    assert NS(("A", "B"), ("C",)).N.oshape == ("A1", "B1")
+    assert NS(("A", "B"), ("B",)).N.oshape == ("A1", "B")
+    assert NS(("A",), ("B",)).H.ishape == ("B",)
+    assert resolve_wildcards(("C", "..."), ("C", "Kx", "Ky")) == ("C", "Kx", "Ky")
+    assert resolve_wildcards(("X", "..."), ("A", "B")) == ("X", "B")
+
+
+def test_pad_rules_and_numerics():
+    assert tl.linops.pad_to_size((4, 5), (9, 10)) == [3, 2, 2, 3]
+    p = tl.Pad((9, 10), (4, 5))
+    x = torch.randn(2, 4, 5)
+    y = p(x)
+    assert y.shape == (2, 9, 10)
+    want = O.center_pad(x.numpy(), (4, 5), (9, 10))
+    assert np.array_equal(y.numpy(), want)
+    assert torch.equal(p.H(y), x)
+    assert torch.allclose(p.N(x), x)
+    with pytest.raises(ValueError):
+        p(torch.randn(3, 3))
+
+
+def test_fft_unitary_and_centered():
+    f = tl.FFT(2, centered=True)
+    x = torch.randn(3, 8, 6, dtype=torch.complex64)
+    want = O.cfftn(x.numpy(), 2)
+    assert np.allclose(f(x).numpy(), want, atol=1e-5)
+    assert torch.allclose(f.H(f(x)), x, atol=1e-5)
+    assert isinstance(f.N, tl.Identity)
+
+
+def test_diagonal_scalar_algebra():
+    w = torch.randn(4, 5, dtype=torch.complex64)
+    d = tl.Diagonal(w, ("Nx", "Ny"))
+    x = torch.randn(4, 5, dtype=torch.complex64)
+    assert torch.allclose(d.H(x), x * w.conj())
+    assert torch.allclose(d.N(x), x * w.abs() ** 2)
+    s = 3.0 * d
+    assert torch.allclose(s(x), 3 * w * x)
+    a = d + d
+    assert torch.allclose(a(x), 2 * w * x)
+    assert torch.allclose((d - d)(x), torch.zeros_like(x))
+    with config.using(cache_adjoint_normal=True):
+        assert d.H is d.H
+    with pytest.raises(ValueError):
+        tl.Diagonal(torch.randn(2, 3, 4), ("A", "B"))
+
+
+def test_chain_mismatch_raises():
+    a = tl.Diagonal(torch.randn(3), ("A",))
+    b = tl.Diagonal(torch.randn(3), ("B",))
+    with pytest.raises(ValueError, match="Mismatched shape"):
+        tl.Chain(a, b)
+
+
+def test_nufft_structure_and_constants():
+    locs = torch.rand(4, 5, 3) * 32 - 16
+    A = tl.NUFFT(locs, (32, 32, 32), output_shape=("R", "K"), oversamp=1.25)
+    assert [l.name for l in A.linops] == ["Apodize", "Pad", "FFT", "Interpolate", "Scalar"]
+    assert [l.repr_name for l in A.N.linops] == [
+        "Apodize", "Pad", "FFT", "Interpolate", "Scalar", "Interpolate.H", "FFT.H", "Crop.H", "Apodize"]
+    beta = float(tl.NUFFT.beta(4.0, 1.25))
+    assert abs(beta - O.nufft_beta(4.0, 1.25)) < 1e-12
+    apod = tl.NUFFT.apodize_weights((32, 32, 32), (40, 40, 40), 1.25, 4.0, beta)
+    want = O.apodize_weights((32, 32, 32), (40, 40, 40), 4.0, beta)
+    assert np.allclose(apod.numpy(), want, rtol=1e-5)
+    lp = tl.NUFFT.prep_locs(locs, (32, 32, 32), (40, 40, 40))
+    assert np.allclose(lp.numpy(), O.prep_locs(locs.numpy(), (32, 32, 32), (40, 40, 40)), atol=1e-4)
+    assert float(lp.min()) >= 0 and float(lp.max()) < 40
+    assert A.size("R") == 4 and A.size("K") == 5
+    with pytest.raises(NotImplementedError):
+        tl.NUFFT(locs, (32, 32, 32), output_shape=("R", "K"), mode="sampling")
+    with pytest.raises(ValueError, match="Nan/Inf"):
+        tl.NUFFT(locs, (32, 32, 32), output_shape=("R", "K"), oversamp=1.0, width=4.0)
+
+
+def test_cg_reference_semantics_cpu():
+    y = torch.randn(7, dtype=torch.complex64)
+    x = tl.conjugate_gradients(lambda v: 2 * v, y, max_num_iters=10)
+    assert torch.allclose(x, y / 2, atol=1e-6)
+    M = torch.randn(6, 6, dtype=torch.complex128)
+    A = M.conj().T @ M + 0.5 * torch.eye(6, dtype=torch.complex128)
+    b = torch.randn(6, dtype=torch.complex128)
+    x = tl.conjugate_gradients(lambda v: A @ v, b, max_num_iters=12, disable_tracking=True)
+    assert torch.allclose(x, torch.linalg.solve(A, b), atol=1e-8)
+    s = tl.DeviceCG(lambda v: A @ v, lam=0.0, check_every=2)
+    x2 = s.solve(b, tol=1e-10, max_iters=40)
+    assert torch.allclose(x2, torch.linalg.solve(A, b), atol=1e-8)
+    assert s.history[-1] <= 1e-10
+    # preconditioned with the exact inverse: one step
+    Ainv = torch.linalg.inv(A)
+    s3 = tl.DeviceCG(lambda v: A @ v, M=lambda r: Ainv @ r, check_every=1)
+    s3.solve(b, tol=1e-9, max_iters=5)
+    assert s3.iters <= 2

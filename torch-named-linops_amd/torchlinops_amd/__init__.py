@@ -5,9 +5,33 @@ and the NUFFT built on them, with the arithmetic in hand-written HIP
 kernels (libtnl.so, C ABI in include/tnl.h).
 """
 
-from . import config, functional
+from . import alg, config, functional, linops, nameddim
 from ._lib import library_path
+from .alg import DeviceCG, conjugate_gradients
+from .linops import (
+    FFT,
+    NUFFT,
+    Add,
+    ArrayToBlocks,
+    BlocksToArray,
+    Chain,
+    Dense,
+    Diagonal,
+    Identity,
+    Interpolate,
+    NamedLinop,
+    Pad,
+    Scalar,
+    toeplitz_psf,
+)
+from .nameddim import ANY, ELLIPSES, Dim, NamedDimension, NamedShape
 
 __version__ = "0.1.0"
 
-__all__ = ["config", "functional", "library_path"]
+__all__ = [
+    "config", "functional", "linops", "alg", "nameddim", "library_path",
+    "NamedLinop", "Chain", "Add", "Identity", "Diagonal", "Scalar", "Pad", "FFT", "Dense",
+    "Interpolate", "ArrayToBlocks", "BlocksToArray", "NUFFT", "toeplitz_psf",
+    "conjugate_gradients", "DeviceCG", "Dim", "NamedDimension", "NamedShape",
+    "ELLIPSES", "ANY",
+]

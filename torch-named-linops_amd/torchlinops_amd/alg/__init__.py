@@ -1,0 +1,3 @@
+from .cg import CGRun, DeviceCG, conjugate_gradients, inner
+
+__all__ = ["conjugate_gradients", "CGRun", "DeviceCG", "inner"]

@@ -30,7 +30,7 @@ interp_plan: str = "auto"
 
 
 def inner_not_relevant(inner) -> bool:
-    from .linops.identity import Identity
+    from .linops.basic import Identity
 
     return inner is None or (isinstance(inner, Identity) and reduce_identity_in_normal)
 

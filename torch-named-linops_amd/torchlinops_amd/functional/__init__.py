@@ -1,6 +1,7 @@
 """Functional API of the gridding path (mirrors torchlinops.functional)."""
 
 from ._interp import grid, interpolate, interpolate_adjoint, ungrid
+from ._nufft import center_crop, center_pad, nufft, nufft_adjoint
 from ._plan import clear_plan_cache
 from ._unfold import (
     array_to_blocks,
@@ -23,4 +24,8 @@ __all__ = [
     "get_nblocks",
     "get_norm_weights",
     "clear_plan_cache",
+    "nufft",
+    "nufft_adjoint",
+    "center_pad",
+    "center_crop",
 ]

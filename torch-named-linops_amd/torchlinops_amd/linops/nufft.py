@@ -1,0 +1,194 @@
+"""NUFFT linop (type II, "interpolate" mode) and its Toeplitz normal.
+
+Follows src/torchlinops/linops/nufft.py:
+  * chain Apodize -> Pad(int(N*os)) -> FFT(centred, ortho) ->
+    Interpolate(KB, beta) -> Scalar(1 / (w^D sqrt(prod N / prod N_pad)))
+    (nufft.py:123-184);
+  * prep_locs (241-334), beta (336-346), apodize_weights (348-365);
+  * adjoint (217-224); normal with ``toeplitz=True`` (226-239) builds
+    Crop . F^H . (K *) . F . Pad with K the Fourier transform of the point
+    spread function on a 2x grid (toeplitz_psf, 395-494).
+The interpolation runs on the HIP kernels through Interpolate.
+"""
+
+from __future__ import annotations
+
+from copy import copy
+from math import prod
+from typing import Optional
+
+import torch
+import torch.nn as nn
+
+from ..nameddim import ND, NamedShape as NS, get_nd_shape
+from .basic import FFT, Diagonal, Identity, Pad, Scalar
+from .chain import Chain
+from .interp import Interpolate
+
+__all__ = ["NUFFT", "toeplitz_psf", "cfftn", "cifftn"]
+
+
+def cfftn(x, dim=None, norm="ortho"):
+    """Centred n-D FFT: ifftshift -> fftn -> fftshift (utils/_fft.py:7-31)."""
+    x = torch.fft.ifftshift(x, dim=dim)
+    x = torch.fft.fftn(x, dim=dim, norm=norm)
+    return torch.fft.fftshift(x, dim=dim)
+
+
+def cifftn(x, dim=None, norm="ortho"):
+    x = torch.fft.ifftshift(x, dim=dim)
+    x = torch.fft.ifftn(x, dim=dim, norm=norm)
+    return torch.fft.fftshift(x, dim=dim)
+
+
+class NUFFT(Chain):
+    def __init__(self, locs, grid_size, output_shape, input_shape=None, input_kshape=None,
+                 batch_shape=None, oversamp: float = 1.25, width: float = 4.0,
+                 mode: str = "interpolate", do_prep_locs: bool = True,
+                 apodize_weights: Optional[torch.Tensor] = None, **options):
+        if mode != "interpolate":
+            raise NotImplementedError(
+                f"NUFFT mode {mode!r}: only 'interpolate' is implemented on this path"
+            )
+        device = locs.device
+        self.mode = mode
+        self.options = options
+        self.input_shape = ND.infer(tuple(input_shape) if input_shape is not None
+                                    else get_nd_shape(len(grid_size)))
+        self.input_kshape = ND.infer(tuple(input_kshape) if input_kshape is not None
+                                     else get_nd_shape(len(grid_size), kspace=True))
+        self.output_shape = ND.infer(tuple(output_shape))
+        self.batch_shape = ND.infer(tuple(batch_shape) if batch_shape is not None else ("...",))
+        batched_input = NS(self.batch_shape) + NS(self.input_shape)
+        ndim = len(grid_size)
+        padded_size = tuple(int(i * oversamp) for i in grid_size)
+        pad = Pad(padded_size, grid_size, in_shape=self.input_shape, batch_shape=self.batch_shape)
+        fft = FFT(ndim=locs.shape[-1], centered=True, norm="ortho", batch_shape=self.batch_shape,
+                  grid_shapes=(pad.out_im_shape, self.input_kshape))
+        grid_shape = fft._shape.output_grid_shape
+        locs_p = self.prep_locs(locs, grid_size, padded_size) if do_prep_locs else locs
+        beta = self.beta(width, oversamp)
+        weight = (self.apodize_weights(grid_size, padded_size, oversamp, width, beta).to(device)
+                  if apodize_weights is None else apodize_weights)
+        if weight.isnan().any() or weight.isinf().any():
+            raise ValueError(
+                f"Nan/Inf values detected in apodization weight (width={width}, oversamp={oversamp})."
+            )
+        apodize = Diagonal(weight, batched_input.ishape)
+        apodize.name = "Apodize"
+        interp = Interpolate(locs_p, padded_size, batch_shape=self.batch_shape,
+                             locs_batch_shape=self.output_shape, grid_shape=grid_shape,
+                             width=width, kernel="kaiser_bessel", kernel_params=dict(beta=beta))
+        scale_factor = width**ndim * (prod(grid_size) / prod(padded_size)) ** 0.5
+        scale = Scalar(weight=1.0 / scale_factor, ioshape=interp.oshape)
+        super().__init__(apodize, pad, fft, interp, scale, name="NUFFT")
+        self.locs = locs
+        self.grid_size = tuple(grid_size)
+        self.oversamp = oversamp
+        self.width = width
+        self.pad = pad
+        self.fft = fft
+        self.interp = interp
+
+    def adjoint(self):
+        adj = copy(self)
+        adj._shape = adj._shape.H
+        adj.linops = nn.ModuleList([l.adjoint() for l in reversed(self.linops)])
+        return adj
+
+    def normal(self, inner=None):
+        if self.options.get("toeplitz", False):
+            kernel = toeplitz_psf(self, inner, dtype=self.options.get("toeplitz_dtype"),
+                                  oversamp=self.options.get("toeplitz_oversamp", 2.0))
+            pad = Pad(scale_int(self.grid_size, self.options.get("toeplitz_oversamp", 2.0)),
+                      self.grid_size, in_shape=self.input_shape, batch_shape=self.batch_shape)
+            return pad.normal(self.fft.normal(kernel))
+        return super().normal(inner)
+
+    @staticmethod
+    def prep_locs(locs, grid_size, padded_size, pad_mode="circular", nufft_mode="interpolate"):
+        """Centred coordinates in [-N/2, N/2] -> padded-grid units in
+        [0, N_pad) (scale by N_pad/N, shift by N_pad//2, wrap or clamp)."""
+        out = locs.clone()
+        for i in range(-len(grid_size), 0):
+            out[..., i] *= padded_size[i] / grid_size[i]
+            out[..., i] += padded_size[i] // 2
+            if pad_mode == "zero":
+                out[..., i] = torch.clamp(out[..., i], 0, padded_size[i] - 1)
+            elif pad_mode == "circular":
+                out[..., i] = torch.remainder(out[..., i], torch.tensor(padded_size[i]))
+            else:
+                raise ValueError(f"Unrecognized padding mode during prep: {pad_mode}")
+        return out
+
+    @staticmethod
+    def beta(width, oversamp):
+        """Beatty et al. 2005: pi * sqrt(((w / os) (os - 1/2))^2 - 0.8)."""
+        return torch.pi * (((width / oversamp) * (oversamp - 0.5)) ** 2 - 0.8) ** 0.5
+
+    @staticmethod
+    def apodize_weights(grid_size, padded_size, oversamp, width, beta):
+        """prod_d a_d / sinh(a_d), a = sqrt(beta^2 - (pi w (n - N//2) / N_pad)^2)."""
+        gs = torch.tensor(grid_size)
+        ps = torch.tensor(padded_size)
+        g = torch.stack(torch.meshgrid(*(torch.arange(s) for s in grid_size), indexing="ij"), -1)
+        apod = (beta**2 - (torch.pi * width * (g - gs // 2) / ps) ** 2) ** 0.5
+        apod = apod / torch.sinh(apod)
+        return torch.prod(apod, dim=-1)
+
+    def flatten(self):
+        return [self]
+
+    @property
+    def device(self):
+        return self.interp.locs.device
+
+
+def scale_int(t, s):
+    return tuple(int(s * x) for x in t)
+
+
+def _rescale_locs(locs, c0, w0, c1, w1):
+    cols = [(locs[..., d] - c0[d]) * w1[d] / w0[d] + c1[d] for d in range(locs.shape[-1])]
+    return torch.stack(cols, dim=-1)
+
+
+def toeplitz_psf(nufft: NUFFT, inner=None, dtype=None, oversamp: float = 2.0):
+    """Fourier-domain point spread kernel K with A^H inner A =
+    Crop F^H (K *) F Pad on a ``oversamp``-times larger grid.
+
+    The PSF is one adjoint NUFFT of ``inner(ones)`` on the enlarged grid
+    (nufft.py:439-491).  Supported inners: None / Identity and linops that
+    do not couple batch entries (all batch dims broadcast); the kernel is
+    then shared by every batch entry and applied as a Diagonal.
+    """
+    dtype = dtype or torch.complex64
+    device = nufft.device
+    grid_size = nufft.pad.im_size
+    new_grid = scale_int(grid_size, oversamp)
+    width0 = nufft.pad.pad_im_size
+    width1 = scale_int(width0, oversamp)
+    new_locs = _rescale_locs(nufft.interp.locs.detach().clone(),
+                             tuple(w // 2 for w in width0), width0,
+                             tuple(w // 2 for w in width1), width1)
+    nufft_os = NUFFT(new_locs, grid_size=new_grid, output_shape=nufft.output_shape,
+                     input_shape=nufft.input_shape, input_kshape=nufft.input_kshape,
+                     batch_shape=("...",), oversamp=nufft.oversamp, width=nufft.width,
+                     do_prep_locs=False)
+    out_sizes = tuple(nufft.interp.locs.shape[:-1])
+    ones = torch.ones(out_sizes, dtype=dtype, device=device)
+    if inner is not None and not isinstance(inner, Identity):
+        for d in inner.ishape:
+            if d in nufft.output_shape or str(d) == "...":
+                continue
+            if inner.size(d) not in (None, 1):
+                raise NotImplementedError(
+                    "toeplitz_psf: inner linops that couple batch entries are not supported"
+                )
+        ones = inner(ones)
+    otf = nufft_os.H(ones)
+    dims = tuple(range(-len(new_grid), 0))
+    kernel = cfftn(otf, dim=dims, norm=None) * (oversamp ** len(grid_size) / prod(new_grid) ** 0.5)
+    kernel = kernel.to(dtype)
+    kshape = tuple(nufft.fft._shape.output_grid_shape)
+    return Diagonal(kernel, ioshape=tuple(nufft.batch_shape) + kshape)
