@@ -94,6 +94,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--small", action="store_true", help="reduced size (debug only)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one elementwise copy of the 8 x 256^3 complex64 grid "
+                         "(1 GiB read + 1 GiB written) to calibrate HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -149,6 +152,11 @@ def main() -> int:
     for i in range(args.warmup):
         step()
         log(f"warmup {i + 1}/{args.warmup}")
+    if args.calibrate_copy:
+        cal = vals.mul(1.0)  # one kernel: reads and writes C*n^3*8 bytes
+        torch.cuda.synchronize()
+        del cal
+        log(f"calibration copy: {C * n**3 * 8} bytes read and written")
     torch.cuda.synchronize()
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
     torch.cuda.synchronize()
@@ -165,6 +173,15 @@ def main() -> int:
     ms_step = wall / args.steps * 1e3
     value = 2 * C * M / (wall / args.steps)
     dom, t_dom = ("grid", t_g) if t_g >= t_u else ("ungrid", t_u)
+    traffic = None
+    tfile = ROOT / "profiles" / "interp_c2_radial_hbm_traffic.json"
+    if tfile.exists() and not args.small:
+        # HBM bytes per dispatch from the FETCH_SIZE / WRITE_SIZE passes
+        # recorded in profiles/ (rocprofv3 cannot run inside this process)
+        tb = json.loads(tfile.read_text())["traffic_bytes"]
+        key = "grid_binned_kernel" if dom == "grid" else "ungrid_binned_kernel"
+        hits = [v for k, v in tb.items() if key in k and "true, 1," not in k]
+        traffic = float(hits[0]) if hits else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -200,7 +217,7 @@ def main() -> int:
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": bytes_call / t_dom / HBM_PEAK,
-            "traffic": None,
+            "traffic": traffic,
             "algorithmic_bytes": bytes_call,
             "ungrid_frac": bytes_call / t_u / HBM_PEAK,
             "grid_frac": bytes_call / t_g / HBM_PEAK,

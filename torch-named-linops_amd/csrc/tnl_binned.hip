@@ -94,31 +94,49 @@ __global__ __launch_bounds__(BIN_THREADS) void ungrid_binned_kernel(
     }
   }
   // ---- stage window: channel planes [c][cell] in LDS (lanes gathering
-  // different cells then fall in different banks), coalesced along z in HBM
+  // different cells then fall in different banks), coalesced along z in
+  // HBM.  A table of row bases (one per (channel, x, y) row of the window)
+  // keeps the per-element index math to one division by the row length.
   const T* __restrict__ v = reinterpret_cast<const T*>(vals);
-  const int w12 = B.win[1] * B.win[2];
-  const int total = B.wcells * CC;
-  for (int idx = threadIdx.x; idx < total; idx += BIN_THREADS) {
-    const int c = idx / B.wcells;
-    const int cell = idx - c * B.wcells;
-    const int wx = cell / w12;
-    const int rem = cell - wx * w12;
-    const int wy = rem / B.win[2];
-    const int wz = rem - wy * B.win[2];
-    int64_t g[3] = {origin[0] + wx, origin[1] + wy, origin[2] + wz};
+  const int w2 = B.win[2];
+  const int nrows = CC * B.win[0] * B.win[1];
+  int64_t* rowbase = reinterpret_cast<int64_t*>(
+      lds_raw + ((size_t)CC * B.wcells * sizeof(T) + 7) / 8 * 8);
+  for (int r = threadIdx.x; r < nrows; r += BIN_THREADS) {
+    const int c = r / (B.win[0] * B.win[1]);
+    const int rem = r - c * (B.win[0] * B.win[1]);
+    const int wx = rem / B.win[1], wy = rem - wx * B.win[1];
+    int64_t gx = origin[0] + wx, gy = origin[1] + wy;
     bool ok = c < nc;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      if (G.pad == TNL_PAD_CIRCULAR) {
-        if (g[d] >= G.n[d]) g[d] = mod_pos(g[d], G.n[d]);
-      } else {
-        ok = ok && g[d] < G.n[d];
-      }
+    if (G.pad == TNL_PAD_CIRCULAR) {
+      gx = mod_pos(gx, G.n[0]);
+      gy = mod_pos(gy, G.n[1]);
+    } else {
+      ok = ok && gx < G.n[0] && gy < G.n[1];
+    }
+    rowbase[r] = ok ? (c0 + c) * G.ncells + (gx * G.n[1] + gy) * G.n[2] : -1;
+  }
+  __syncthreads();
+  const float inv_w2 = 1.0f / (float)w2;
+  const int total = nrows * w2;
+  for (int idx = threadIdx.x; idx < total; idx += BIN_THREADS) {
+    int r = (int)(((float)idx + 0.5f) * inv_w2);
+    if (r * w2 > idx) --r;              // correct the float quotient
+    else if ((r + 1) * w2 <= idx) ++r;
+    const int wz = idx - r * w2;
+    int64_t gz = origin[2] + wz;
+    const int64_t base = rowbase[r];
+    bool ok = base >= 0;
+    if (G.pad == TNL_PAD_CIRCULAR) {
+      if (gz >= G.n[2]) gz = mod_pos(gz, G.n[2]);
+    } else {
+      ok = ok && gz < G.n[2];
     }
     T val = C::zero();
-    if (ok)
-      val = v[(c0 + c) * G.ncells + (g[0] * G.n[1] + g[1]) * G.n[2] + g[2]];
-    lds[c * B.wcells + cell] = val;
+    if (ok) val = v[base + gz];
+    // channel c of row r lives in plane c; rows of one channel are
+    // contiguous, so the LDS index is simply idx
+    lds[idx] = val;
   }
   __syncthreads();
 
@@ -461,6 +479,11 @@ static BinGeom make_bin_geom(const tnl_plan_layout& L, bool adjoint) {
   return B;
 }
 
+static size_t ungrid_lds_bytes(const BinGeom& B, int cc, int comp_bytes) {
+  return ((size_t)B.wcells * cc * comp_bytes + 7) / 8 * 8 +
+         (size_t)cc * B.win[0] * B.win[1] * sizeof(int64_t);
+}
+
 static size_t grid_lds_bytes(const BinGeom& B, int planes, int nt) {
   const size_t pcells = (size_t)B.tile[0] * B.tile[1] * (B.tile[2] + 1);
   return planes * pcells * 4 + (6 * GW + 3 * nt * GW + planes * GW) * 4;
@@ -479,15 +502,7 @@ bool binned_supported(const InterpGeom& G, const tnl_plan_layout& L,
     return (int64_t)grid_lds_bytes(make_bin_geom(L, true), 2, 5) <= L.lds_budget;
   }
   // at least one channel of the window must fit in LDS
-  return (int64_t)make_bin_geom(L, false).wcells * 8 <= L.lds_budget;
-}
-
-static int pick_cc(int64_t nbatch, int64_t cells, int comp_bytes,
-                   int64_t budget) {
-  int cc = 1;
-  while (cc < 8 && cc < nbatch && (int64_t)cells * comp_bytes * cc * 2 <= budget)
-    cc *= 2;
-  return cc;
+  return (int64_t)ungrid_lds_bytes(make_bin_geom(L, false), 1, 8) <= L.lds_budget;
 }
 
 // Dynamic LDS above the default 64 KiB needs an explicit opt-in per kernel.
@@ -510,7 +525,7 @@ static int launch_binned(bool adjoint, const float* vals, float* out,
   const float* slocs = (const float*)(ws + L.off_slocs);
   if (!adjoint) {
     const int64_t nwg = (L.sub_capacity + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
-    const size_t lds = (size_t)B.wcells * CC * comp_bytes;
+    const size_t lds = ungrid_lds_bytes(B, CC, comp_bytes);
     if (int rc = allow_lds((const void*)ungrid_binned_kernel<NDIM, CPLX, CC, NT>, lds))
       return rc;
     ungrid_binned_kernel<NDIM, CPLX, CC, NT>
@@ -553,7 +568,10 @@ static int dispatch_cc(bool adjoint, const float* vals, float* out,
            (int64_t)grid_lds_bytes(B, 2 * cc * (CPLX ? 2 : 1), NT) <= L.lds_budget)
       cc *= 2;
   } else {
-    cc = pick_cc(nbatch, B.wcells, CPLX ? 8 : 4, L.lds_budget);
+    cc = 1;
+    while (cc < 8 && cc < nbatch &&
+           (int64_t)ungrid_lds_bytes(B, 2 * cc, CPLX ? 8 : 4) <= L.lds_budget)
+      cc *= 2;
   }
   switch (cc) {
     case 8: return launch_binned<NDIM, CPLX, 8, NT>(adjoint, vals, out, nbatch, G, L, ws, s);

@@ -33,7 +33,7 @@ int check_hip(hipError_t e, const char* what);
 __device__ __forceinline__ float kb_window(float x, float beta) {
   const float z = beta * sqrtf(fmaxf(1.0f - x * x, 0.0f));
   if (z < 3.75f) {
-    const float t = z / 3.75f;
+    const float t = z * (1.0f / 3.75f);
     const float q = t * t;
     return 1.0f +
            q * (3.5156229f +
@@ -41,7 +41,9 @@ __device__ __forceinline__ float kb_window(float x, float beta) {
                      q * (1.2067492f +
                           q * (0.2659732f + q * (0.0360768f + q * 0.0045813f)))));
   }
-  const float r = 3.75f / z;  // 1/t
+  // z >= 3.75 here, so the reciprocal and rsqrt are of well-scaled values
+  const float rz = __frcp_rn(z);
+  const float r = 3.75f * rz;  // 1/t
   const float poly =
       0.39894228f +
       r * (0.01328592f +
@@ -51,7 +53,7 @@ __device__ __forceinline__ float kb_window(float x, float beta) {
                           r * (-0.02057706f +
                                r * (0.02635537f +
                                     r * (-0.01647633f + r * 0.00392377f)))))));
-  return expf(z) / sqrtf(z) * poly;
+  return expf(z) * sqrtf(rz) * poly;
 }
 
 // Linear-interpolation hat max(1 - |x|, 0) (kernels.py:92-94).

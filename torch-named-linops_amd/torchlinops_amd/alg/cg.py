@@ -149,8 +149,9 @@ class DeviceCG:
             n = min(self.check_every, max_iters - self.iters)
             if self.use_graph and y.is_cuda and n == self.check_every:
                 if graph is None:
-                    graph = self._capture(n)
-                graph.replay()
+                    graph = self._capture(n)  # runs this block eagerly, then records
+                else:
+                    graph.replay()
             else:
                 for _ in range(n):
                     self._iteration()
@@ -159,14 +160,15 @@ class DeviceCG:
         return self.x
 
     def _capture(self, n):
-        """Capture n iterations (warm-up first so allocations and plans exist)."""
+        """Run n iterations eagerly on a side stream (so plans and
+        allocations exist), then record n iterations into a graph without
+        executing them."""
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
             for _ in range(n):
                 self._iteration()
         torch.cuda.current_stream().wait_stream(s)
-        self.iters += n
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             for _ in range(n):
