@@ -123,12 +123,16 @@ def test_nufft_structure_and_constants():
 
 
 def test_cg_reference_semantics_cpu():
-    y = torch.randn(7, dtype=torch.complex64)
-    x = tl.conjugate_gradients(lambda v: 2 * v, y, max_num_iters=10)
+    # A = 2I: one step lands on y/2 with r == 0 exactly.  The loss change of
+    # that step is above ltol, so a second step would run and divide 0/0,
+    # in the reference's loop (pcg.py:67-87) as here: run one step.
+    g = torch.Generator().manual_seed(5)
+    y = torch.randn(7, dtype=torch.complex64, generator=g)
+    x = tl.conjugate_gradients(lambda v: 2 * v, y, max_num_iters=1)
     assert torch.allclose(x, y / 2, atol=1e-6)
-    M = torch.randn(6, 6, dtype=torch.complex128)
+    M = torch.randn(6, 6, dtype=torch.complex128, generator=g)
     A = M.conj().T @ M + 0.5 * torch.eye(6, dtype=torch.complex128)
-    b = torch.randn(6, dtype=torch.complex128)
+    b = torch.randn(6, dtype=torch.complex128, generator=g)
     x = tl.conjugate_gradients(lambda v: A @ v, b, max_num_iters=12, disable_tracking=True)
     assert torch.allclose(x, torch.linalg.solve(A, b), atol=1e-8)
     s = tl.DeviceCG(lambda v: A @ v, lam=0.0, check_every=2)

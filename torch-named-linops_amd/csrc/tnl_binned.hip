@@ -30,6 +30,8 @@ struct BinGeom {
   int win[3];  // tile + ntap - 1 (ungrid window)
   int wcells;
   int tcells;
+  int gys;  // gridding accumulator: cells between y rows
+  int gxs;  // gridding accumulator: cells between x planes
   int64_t npts;
   int64_t nbins;
 };
@@ -247,19 +249,70 @@ __global__ void grid_zero_shared_kernel(float* __restrict__ out,
   }
 }
 
-// Gridding, "lanes = taps": one wave per workgroup owns a tile accumulator
-// in LDS (channel planes, z rows padded by one cell against bank
-// conflicts).  Touches are processed in batches of 64: first lanes = points
-// compute each point's first tap, tap counts and 1-D window values into LDS;
-// then the wave walks the batch point by point with lanes = taps (a 4x4x4
-// block, plus a second pass for the fifth tap layer of points on integer
-// coordinates).  Within one instruction every lane addresses a different
-// cell and only this wave touches the tile, so the accumulation is a plain
-// read-modify-write: no LDS atomics (which run about one lane per clock).
-constexpr int GW = 64;  // threads per gridding workgroup
+// Gridding, "lanes = taps".  A workgroup of NW waves owns one tile
+// accumulator in LDS: NP channel planes (row and plane strides padded so the
+// 64 taps of one instruction spread over the LDS banks), split
+// evenly over the waves, so no two waves ever write the same word.  Touches
+// are taken in batches of PB points, in three phases:
+//   1. lanes = points, by role: one wave per spatial axis computes the
+//      first tap, the in-tile tap mask and the 1-D window values; one wave
+//      stores the channel values.  Right after, the raw inputs of the next
+//      batch are loaded into registers, so their HBM latency overlaps the
+//      rest of this batch.
+//   2. all waves: the 64 tap weights of every point (products of the 1-D
+//      windows, stored tap-major) and a per-point record: LDS offset of tap
+//      (0,0,0), the 64-bit mask of live taps and flags.
+//   3. lanes = taps: every wave walks the batch point by point over its own
+//      planes.  A lane holds one tap of a 4x4x4 block (3-D) or of an 8x8
+//      block (2-D); it is live when its bit is set in the point's mask, and
+//      its cell is the point's offset plus a per-lane constant.  Within an
+//      instruction every lane addresses a distinct cell and only one wave
+//      owns a plane, so the accumulation is a plain read-modify-write (LDS
+//      float atomics run about one lane per clock).  Two rare cases take a
+//      general path: points on integer coordinates in 3-D (a fifth tap,
+//      covered by a second pass over the 61 taps with an index 4) and
+//      footprints that wrap around the grid inside the tile.
+// The tile then goes to HBM once, coalesced along z.
+constexpr int GW = 64;  // wave size
+constexpr int PB = 32;  // points per batch
 
-template <int NDIM, bool CPLX, int CC, int NT>
-__global__ __launch_bounds__(GW) void grid_binned_kernel(
+constexpr int GF_SHELL = 1;  // some axis has a live fifth tap (3-D)
+constexpr int GF_WRAP = 2;   // the footprint wraps around the grid in this tile
+
+// LDS carve-up of the gridding kernel (host and device agree on it).
+struct GridLds {
+  int off_dim;   // int2 [3][PB]: {lo * stride, mask | wrap << 8 | lo << 16}
+  int off_w1;    // float [PB][w1s]: 1-D window values
+  int off_v;     // float [PB][NP]: channel values
+  int off_w64;   // float [64][PB + 1]: tap weights, tap-major
+  int off_meta;  // int4 [PB]: {offset of tap (0,0,0), mask lo, mask hi, flags}
+  int w1s;
+  int bytes;
+};
+
+__host__ __device__ inline int align16(int x) { return (x + 15) / 16 * 16; }
+
+__host__ __device__ inline GridLds grid_lds_layout(const BinGeom& B, int planes, int nt) {
+  GridLds g;
+  const int pcells = B.tile[0] * B.gxs;
+  g.w1s = (3 * nt) | 1;  // odd row stride: conflict-free per-point access
+  g.off_dim = align16(planes * pcells * 4);
+  g.off_w1 = g.off_dim + 3 * PB * 8;
+  g.off_v = align16(g.off_w1 + PB * g.w1s * 4);
+  g.off_w64 = g.off_v + PB * planes * 4;
+  g.off_meta = align16(g.off_w64 + GW * (PB + 1) * 4);
+  g.bytes = g.off_meta + PB * 16;
+  return g;
+}
+
+template <int N>
+struct VecF;
+template <> struct VecF<1> { using T = float; };
+template <> struct VecF<2> { using T = float2; };
+template <> struct VecF<4> { using T = float4; };
+
+template <int NDIM, bool CPLX, int CC, int NT, int NW>
+__global__ __launch_bounds__(GW * NW) void grid_binned_kernel(
     const float* __restrict__ vals, float* __restrict__ out, int64_t nbatch,
     InterpGeom G, BinGeom B, const int* __restrict__ perm,
     const float* __restrict__ slocs, const int* __restrict__ tlist,
@@ -268,22 +321,34 @@ __global__ __launch_bounds__(GW) void grid_binned_kernel(
   using T = typename C::T;
   constexpr int COMP = CPLX ? 2 : 1;
   constexpr int NP = CC * COMP;  // accumulator planes
+  constexpr int PPW = NP / NW;   // planes per wave
+  constexpr int NR = 4 / NW;     // phase-1 roles per wave
+  constexpr int NTH = GW * NW;
+  constexpr int VW = PPW % 4 == 0 ? 4 : (PPW % 2 == 0 ? 2 : 1);
+  constexpr int W64S = PB + 1;  // tap-major weight row stride
+  static_assert(NP % NW == 0 && 4 % NW == 0, "planes and roles split evenly");
+  static_assert(NT <= 8, "tap masks hold 8 bits per axis");
+  static_assert(NTH % PB == 0, "phase 2 maps threads to (point, tap group)");
+  using V = typename VecF<VW>::T;
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  const int zp = B.tile[2] + 1;                   // padded z row
-  const int pcells = B.tile[0] * B.tile[1] * zp;  // cells per plane
+  const int pcells = B.tile[0] * B.gxs;  // cells per plane
+  const GridLds LG = grid_lds_layout(B, NP, NT);
   float* acc = reinterpret_cast<float*>(lds_raw);  // [NP][pcells]
-  // point table of the current batch (lanes = points fill it)
-  int* s_lo = reinterpret_cast<int*>(acc + NP * pcells);  // [3][GW] first tap - origin
-  int* s_cnt = s_lo + 3 * GW;                              // [3][GW]
-  float* s_w = reinterpret_cast<float*>(s_cnt + 3 * GW);   // [3][NT][GW]
-  float* s_v = s_w + 3 * NT * GW;                          // [NP][GW]
-  const int lane = threadIdx.x;
+  int2* s_dim = reinterpret_cast<int2*>(lds_raw + LG.off_dim);
+  float* s_w1 = reinterpret_cast<float*>(lds_raw + LG.off_w1);
+  float* s_v = reinterpret_cast<float*>(lds_raw + LG.off_v);
+  float* s_w64 = reinterpret_cast<float*>(lds_raw + LG.off_w64);
+  int4* s_meta = reinterpret_cast<int4*>(lds_raw + LG.off_meta);
+  const int w1s = LG.w1s;
+  const int tid = threadIdx.x;
+  const int lane = tid & (GW - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(tid / GW);
   const int nitems = *nitems_ptr;
   const T* __restrict__ v = reinterpret_cast<const T*>(vals);
+  const int dstride[3] = {B.gxs, B.gys, 1};
 
-  // This lane's taps.  3-D: main pass (k0,k1,k2) in [0,4)^3 and a second
-  // pass over the 61 taps of [0,5)^3 with some index 4 (points on integer
-  // coordinates).  2-D: one pass over (k1,k2) in [0,8)^2.
+  // This lane's tap in the main block and its cell offset from a point's
+  // tap (0,0,0); the 3-D shell tap (an index 4) of the second pass.
   int mk[3], sk[3];
   bool shell_ok = false;
   if (NDIM >= 3) {
@@ -301,6 +366,9 @@ __global__ __launch_bounds__(GW) void grid_binned_kernel(
     mk[0] = 0; mk[1] = lane >> 3; mk[2] = lane & 7;
     sk[0] = sk[1] = sk[2] = 0;
   }
+  const int mkoff = mk[0] * dstride[0] + mk[1] * dstride[1] + mk[2];
+  float* __restrict__ accw = acc + wv * PPW * pcells;  // this wave's planes
+  const float* __restrict__ w64l = s_w64 + lane * W64S;
 
   for (int64_t it = xcd_remap(blockIdx.x, gridDim.x); it < nitems;
        it += gridDim.x) {
@@ -317,149 +385,330 @@ __global__ __launch_bounds__(GW) void grid_binned_kernel(
     }
     for (int64_t c0 = 0; c0 < nbatch; c0 += CC) {
       const int nc = (int)(nbatch - c0 < CC ? nbatch - c0 : CC);
-      for (int i = lane; i < NP * pcells; i += GW) acc[i] = 0.0f;
-      for (int b0 = 0; b0 < count; b0 += GW) {
-        const int nb = count - b0 < GW ? count - b0 : GW;
-        __syncthreads();
-        // ---- lanes = points: first tap, counts, windows and values
+      // raw inputs of one batch, per role: a coordinate or the values
+      float pt[NR];
+      T pv[CC];
+      auto load = [&](int b0, int nb) {
         if (lane < nb) {
           const int q = tlist[start + b0 + lane];
-          const int p = perm[q];
 #pragma unroll
-          for (int c = 0; c < CC; ++c) {
-            const T x = c < nc ? v[(c0 + c) * B.npts + p] : C::zero();
-            if constexpr (CPLX) {
-              s_v[(2 * c) * GW + lane] = x.x;
-              s_v[(2 * c + 1) * GW + lane] = x.y;
+          for (int sr = 0; sr < NR; ++sr) {
+            const int role = wv + sr * NW;
+            if (role < 3) {
+              pt[sr] = role >= 3 - NDIM ? slocs[(int64_t)q * NDIM + role - (3 - NDIM)] : 0.f;
             } else {
-              s_v[c * GW + lane] = x;
+              const int p = perm[q];
+#pragma unroll
+              for (int c = 0; c < CC; ++c)
+                pv[c] = c < nc ? v[(c0 + c) * B.npts + p] : C::zero();
             }
           }
+        }
+      };
+      {
+        float4* a4 = reinterpret_cast<float4*>(acc);
+        const int n4 = NP * pcells / 4;
+        for (int i = tid; i < n4; i += NTH) a4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int i = n4 * 4 + tid; i < NP * pcells; i += NTH) acc[i] = 0.f;
+      }
+      load(0, count < PB ? count : PB);
+      for (int b0 = 0; b0 < count; b0 += PB) {
+        const int nb = count - b0 < PB ? count - b0 : PB;
+        __syncthreads();
+        // ---- phase 1, lanes = points: per-axis records and values
+        if (lane < nb) {
 #pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            int lo = 0, cnt = 1;
-            float t = 0.f;
-            TapRange r;
-            r.g0 = 0;
-            r.count = 1;
-            if (d >= 3 - NDIM) {
-              t = slocs[(int64_t)q * NDIM + d - (3 - NDIM)];
-              r = tap_range(t, G.half[d], G.ntap[d], G.n[d], G.pad);
-              const int64_t gw =
-                  G.pad == TNL_PAD_CIRCULAR ? mod_pos(r.g0, G.n[d]) : r.g0;
-              lo = (int)(gw - origin[d]);
-              cnt = r.count;
+          for (int sr = 0; sr < NR; ++sr) {
+            const int role = wv + sr * NW;
+            if (role < 3) {
+              const int d = role;
+              int lo = 0, cnt = 1;
+              float t = 0.f;
+              int64_t g0 = 0;
+              if (d >= 3 - NDIM) {
+                t = pt[sr];
+                const TapRange r = tap_range(t, G.half[d], G.ntap[d], G.n[d], G.pad);
+                const int64_t gw =
+                    G.pad == TNL_PAD_CIRCULAR ? mod_pos(r.g0, G.n[d]) : r.g0;
+                lo = (int)(gw - origin[d]);
+                cnt = r.count;
+                g0 = r.g0;
+              }
+              // in-tile mask over taps k < cnt, and whether a tap wraps
+              // past the end of the grid into this tile
+              int m = 0, wrap = 0;
+#pragma unroll
+              for (int k = 0; k < NT; ++k) {
+                if (k < cnt) {
+                  int l = lo + k;
+                  if (G.pad == TNL_PAD_CIRCULAR && origin[d] + l >= G.n[d]) {
+                    l -= (int)G.n[d];
+                    wrap = 1;
+                  }
+                  if (l >= 0 && l < B.tile[d]) m |= 1 << k;
+                }
+              }
+              s_dim[d * PB + lane] =
+                  make_int2(lo * dstride[d], m | (wrap << 8) | (int)((unsigned)lo << 16));
+#pragma unroll
+              for (int k = 0; k < NT; ++k) {
+                float w = (d < 3 - NDIM && k == 0) ? 1.0f : 0.0f;
+                if (d >= 3 - NDIM && k < cnt)
+                  w = window(G.kernel,
+                             fminf(fabsf((float)(g0 + k) - t) * G.inv_half[d], 1.0f),
+                             G.beta);
+                s_w1[lane * w1s + d * NT + k] = w;
+              }
+            } else {
+              V* dst = reinterpret_cast<V*>(s_v + lane * NP);
+              float tmp[NP];
+#pragma unroll
+              for (int c = 0; c < CC; ++c) {
+                if constexpr (CPLX) {
+                  tmp[2 * c] = pv[c].x;
+                  tmp[2 * c + 1] = pv[c].y;
+                } else {
+                  tmp[c] = pv[c];
+                }
+              }
+#pragma unroll
+              for (int i = 0; i < NP / VW; ++i) {
+                V x;
+                if constexpr (VW == 4) x = make_float4(tmp[4 * i], tmp[4 * i + 1], tmp[4 * i + 2], tmp[4 * i + 3]);
+                else if constexpr (VW == 2) x = make_float2(tmp[2 * i], tmp[2 * i + 1]);
+                else x = tmp[i];
+                dst[i] = x;
+              }
             }
-            s_lo[d * GW + lane] = lo;
-            s_cnt[d * GW + lane] = cnt;
+          }
+        }
+        if (b0 + PB < count) load(b0 + PB, count - b0 - PB < PB ? count - b0 - PB : PB);
+        __syncthreads();
+        // ---- phase 2: tap weights (tap-major) and point records
+        {
+          const int j = tid % PB;
+          if (j < nb) {
+            const float* w1 = s_w1 + j * w1s;
+            // 8 groups of 8 taps: 3-D group g = (k0 = g / 2, k1 in 2(g % 2) + {0, 1},
+            // k2 in 0..3); 2-D group g = (k1 = g, k2 in 0..7)
+            for (int g = tid / PB; g < 8; g += NTH / PB) {
+              float* dst = s_w64 + (g * 8) * W64S + j;
+              if (NDIM >= 3) {
+                const float wx = w1[g >> 1];
+                const float wxy0 = wx * w1[NT + 2 * (g & 1)];
+                const float wxy1 = wx * w1[NT + 2 * (g & 1) + 1];
 #pragma unroll
-            for (int k = 0; k < NT; ++k) {
-              float w = (d < 3 - NDIM && k == 0) ? 1.0f : 0.0f;
-              if (d >= 3 - NDIM && k < r.count)
-                w = window(G.kernel,
-                           fminf(fabsf((float)(r.g0 + k) - t) * G.inv_half[d], 1.0f),
-                           G.beta);
-              s_w[(d * NT + k) * GW + lane] = w;
+                for (int k2 = 0; k2 < 4; ++k2) {
+                  const float wz = w1[2 * NT + k2];
+                  dst[k2 * W64S] = wxy0 * wz;
+                  dst[(4 + k2) * W64S] = wxy1 * wz;
+                }
+              } else {
+                const float wy = w1[NT + g];
+#pragma unroll
+                for (int k2 = 0; k2 < 8; ++k2)
+                  dst[k2 * W64S] = k2 < NT ? wy * w1[2 * NT + k2] : 0.0f;
+              }
+            }
+            if (tid < PB) {
+              const int2 a0 = s_dim[j], a1 = s_dim[PB + j], a2 = s_dim[2 * PB + j];
+              const unsigned m0 = a0.y & 255, m1 = a1.y & 255, m2 = a2.y & 255;
+              uint64_t m64 = 0;
+              if (NDIM >= 3) {
+                unsigned m12 = 0;
+#pragma unroll
+                for (int k1 = 0; k1 < 4; ++k1)
+                  if ((m1 >> k1) & 1) m12 |= (m2 & 15u) << (4 * k1);
+#pragma unroll
+                for (int k0 = 0; k0 < 4; ++k0)
+                  if ((m0 >> k0) & 1) m64 |= (uint64_t)m12 << (16 * k0);
+              } else {
+#pragma unroll
+                for (int k1 = 0; k1 < 8; ++k1)
+                  if ((m1 >> k1) & 1) m64 |= (uint64_t)m2 << (8 * k1);
+              }
+              const int mall = a0.y | a1.y | a2.y;
+              const int flags = (NDIM >= 3 && (mall & 16) ? GF_SHELL : 0) |
+                                (mall & 256 ? GF_WRAP : 0);
+              s_meta[j] = make_int4(a0.x + a1.x + a2.x, (int)(unsigned)m64,
+                                    (int)(unsigned)(m64 >> 32), flags);
             }
           }
         }
         __syncthreads();
-        // ---- lanes = taps: walk the batch point by point.  The next
-        // point's table entries are loaded before this point's
-        // read-modify-write so their LDS latency overlaps it.
-        int nx_lo[3], nx_n[3];
-        float nx_w[3];
-        float nx_v[NP];
-        auto fetch = [&](int j, const int* k) {
+        // ---- phase 3, lanes = taps: this wave's planes, point by point.
+        // The next point's record, values and weight are loaded during this
+        // point's read-modify-write (one wave's LDS operations complete in
+        // order, so its reads see its earlier writes).
+        const int nbu = __builtin_amdgcn_readfirstlane(nb);
+        int4 n_meta = s_meta[0];
+        float n_w = w64l[0];
+        V n_v[PPW / VW];
+        {
+          const V* src = reinterpret_cast<const V*>(s_v + wv * PPW);
 #pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            nx_lo[d] = s_lo[d * GW + j];
-            nx_n[d] = s_cnt[d * GW + j];
-            nx_w[d] = s_w[(d * NT + (k[d] < NT ? k[d] : 0)) * GW + j];
-          }
+          for (int i = 0; i < PPW / VW; ++i) n_v[i] = src[i];
+        }
+        for (int j = 0; j < nbu; ++j) {
+          const int4 meta = n_meta;
+          const float w = n_w;
+          float vv[PPW];
 #pragma unroll
-          for (int c = 0; c < NP; ++c) nx_v[c] = s_v[c * GW + j];
-        };
-        fetch(0, mk);
-        for (int j = 0; j < nb; ++j) {
-          int lo[3], n[3];
-          float w3[3], vv[NP];
-#pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            lo[d] = nx_lo[d];
-            n[d] = nx_n[d];
-            w3[d] = nx_w[d];
-          }
-#pragma unroll
-          for (int c = 0; c < NP; ++c) vv[c] = nx_v[c];
-          if (j + 1 < nb) fetch(j + 1, mk);
-          const bool fifth = NDIM >= 3 && (n[0] > 4 || n[1] > 4 || n[2] > 4);
-#pragma unroll
-          for (int pass = 0; pass < (NDIM >= 3 ? 2 : 1); ++pass) {
-            if (pass == 1 && !fifth) break;
-            const int* k = pass ? sk : mk;
-            bool ok = (pass == 0 || shell_ok) && k[0] < n[0] && k[1] < n[1] && k[2] < n[2];
-            int l[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-              l[d] = lo[d] + k[d];
-              // footprint wrapping past the end of the grid
-              if (G.pad == TNL_PAD_CIRCULAR && origin[d] + l[d] >= G.n[d])
-                l[d] -= (int)G.n[d];
-              ok = ok && l[d] >= 0 && l[d] < B.tile[d];
+          for (int i = 0; i < PPW / VW; ++i) {
+            const V x = n_v[i];
+            if constexpr (VW == 4) {
+              vv[4 * i] = x.x; vv[4 * i + 1] = x.y; vv[4 * i + 2] = x.z; vv[4 * i + 3] = x.w;
+            } else if constexpr (VW == 2) {
+              vv[2 * i] = x.x; vv[2 * i + 1] = x.y;
+            } else {
+              vv[i] = x;
             }
-            if (ok) {
-              float w;
-              if (pass == 0) {
-                w = w3[0] * w3[1] * w3[2];
-              } else {
-                w = s_w[(0 * NT + k[0]) * GW + j] * s_w[(1 * NT + k[1]) * GW + j] *
-                    s_w[(2 * NT + k[2]) * GW + j];
+          }
+          if (j + 1 < nbu) {
+            n_meta = s_meta[j + 1];
+            n_w = w64l[j + 1];
+            const V* src = reinterpret_cast<const V*>(s_v + (j + 1) * NP + wv * PPW);
+#pragma unroll
+            for (int i = 0; i < PPW / VW; ++i) n_v[i] = src[i];
+          }
+          const unsigned mlo = __builtin_amdgcn_readfirstlane(meta.y);
+          const unsigned mhi = __builtin_amdgcn_readfirstlane(meta.z);
+          const int flags = __builtin_amdgcn_readfirstlane(meta.w);
+          const uint64_t m64 = ((uint64_t)mhi << 32) | mlo;
+          if ((m64 >> lane) & 1) {
+            int cell = meta.x + mkoff;
+            if (flags & GF_WRAP) {  // footprint wraps around the grid in this tile
+              cell = 0;
+#pragma unroll
+              for (int d = 0; d < 3; ++d) {
+                int l = (s_dim[d * PB + j].y >> 16) + mk[d];
+                if (origin[d] + l >= G.n[d]) l -= (int)G.n[d];
+                cell += l * dstride[d];
               }
-              float* cp = acc + (l[0] * B.tile[1] + l[1]) * zp + l[2];
-              float old[NP];
+            }
+            float* cp = accw + cell;
+            float old[PPW];
 #pragma unroll
-              for (int c = 0; c < NP; ++c) old[c] = cp[c * pcells];
+            for (int c = 0; c < PPW; ++c) old[c] = cp[c * pcells];
 #pragma unroll
-              for (int c = 0; c < NP; ++c) cp[c * pcells] = fmaf(w, vv[c], old[c]);
+            for (int c = 0; c < PPW; ++c) cp[c * pcells] = fmaf(w, vv[c], old[c]);
+          }
+          if (NDIM >= 3 && (flags & GF_SHELL)) {
+            // second pass: taps with some index 4 (points on integer coordinates)
+            const int2 a0 = s_dim[j], a1 = s_dim[PB + j], a2 = s_dim[2 * PB + j];
+            const bool ok = shell_ok && ((a0.y >> sk[0]) & (a1.y >> sk[1]) & (a2.y >> sk[2]) & 1);
+            if (ok) {
+              const int lo3[3] = {a0.y >> 16, a1.y >> 16, a2.y >> 16};
+              int cell = 0;
+#pragma unroll
+              for (int d = 0; d < 3; ++d) {
+                int l = lo3[d] + sk[d];
+                if (G.pad == TNL_PAD_CIRCULAR && origin[d] + l >= G.n[d]) l -= (int)G.n[d];
+                cell += l * dstride[d];
+              }
+              const float* w1 = s_w1 + __builtin_amdgcn_readfirstlane(j * w1s);
+              const float ws3 = w1[sk[0]] * w1[NT + sk[1]] * w1[2 * NT + sk[2]];
+              float* cp = accw + cell;
+              float old[PPW];
+#pragma unroll
+              for (int c = 0; c < PPW; ++c) old[c] = cp[c * pcells];
+#pragma unroll
+              for (int c = 0; c < PPW; ++c) cp[c * pcells] = fmaf(ws3, vv[c], old[c]);
             }
           }
         }
       }
       __syncthreads();
-      // ---- write the tile: coalesced along z, one channel plane at a time
+      // ---- write the tile: coalesced along z, one channel at a time
       T* __restrict__ o = reinterpret_cast<T*>(out);
       const int t12 = B.tile[1] * B.tile[2];
-      const int total = B.tcells * nc;
-      for (int idx = lane; idx < total; idx += GW) {
-        const int c = idx / B.tcells;
-        const int cell = idx - c * B.tcells;
+      for (int cell = tid; cell < B.tcells; cell += NTH) {
         const int lx = cell / t12, rem = cell - lx * t12;
         const int ly = rem / B.tile[2], lz = rem - ly * B.tile[2];
         const int64_t gx = origin[0] + lx, gy = origin[1] + ly,
                       gz = origin[2] + lz;
         if (gx >= G.n[0] || gy >= G.n[1] || gz >= G.n[2]) continue;
-        const int64_t gi = (c0 + c) * G.ncells + (gx * G.n[1] + gy) * G.n[2] + gz;
-        const int pc = (lx * B.tile[1] + ly) * zp + lz;
-        const float a0 = acc[(c * COMP) * pcells + pc];
-        const float a1 = CPLX ? acc[(c * COMP + 1) * pcells + pc] : 0.0f;
-        if (!shared) {
-          if constexpr (CPLX)
-            o[gi] = make_float2(a0, a1);
-          else
-            o[gi] = a0;
-        } else {
-          if constexpr (CPLX) {
-            unsafeAtomicAdd(out + 2 * gi, a0);
-            unsafeAtomicAdd(out + 2 * gi + 1, a1);
+        const int64_t g0 = (gx * G.n[1] + gy) * G.n[2] + gz;
+        const int pc = lx * dstride[0] + ly * dstride[1] + lz;
+#pragma unroll
+        for (int c = 0; c < CC; ++c) {
+          if (c >= nc) break;
+          const int64_t gi = (c0 + c) * G.ncells + g0;
+          const float a0 = acc[(c * COMP) * pcells + pc];
+          const float a1 = CPLX ? acc[(c * COMP + 1) * pcells + pc] : 0.0f;
+          if (!shared) {
+            if constexpr (CPLX)
+              o[gi] = make_float2(a0, a1);
+            else
+              o[gi] = a0;
           } else {
-            unsafeAtomicAdd(out + gi, a0);
+            if constexpr (CPLX) {
+              unsafeAtomicAdd(out + 2 * gi, a0);
+              unsafeAtomicAdd(out + 2 * gi + 1, a1);
+            } else {
+              unsafeAtomicAdd(out + gi, a0);
+            }
           }
         }
       }
       __syncthreads();
     }
   }
+}
+
+// Bank cost of one gridding RMW instruction: lane l touches the cell at
+// offset X*k0 + Y*k1 + k2 from the point's first tap (3-D lanes: 4x4x4 taps,
+// 2-D: 8x8).  Sum of the worst bank multiplicities, for LDS serving a whole
+// wave over 64 banks and for LDS serving half waves over 32 banks.
+static int grid_bank_cost(int ndim, int X, int Y) {
+  int off[64];
+  for (int l = 0; l < 64; ++l)
+    off[l] = ndim >= 3 ? X * (l >> 4) + Y * ((l >> 2) & 3) + (l & 3)
+                       : Y * (l >> 3) + (l & 7);
+  int cost = 0;
+  for (int banks = 32; banks <= 64; banks *= 2) {
+    for (int g = 0; g < 64; g += banks) {
+      int cnt[64] = {0}, worst = 0;
+      for (int l = g; l < g + banks; ++l) {
+        const int b = ++cnt[off[l] % banks];
+        worst = b > worst ? b : worst;
+      }
+      cost += worst;
+    }
+  }
+  return cost;
+}
+
+// Row (Y) and plane (X) strides of the gridding accumulator: the padding
+// that minimises bank cost x cells (LDS time per instruction against LDS
+// space, i.e. channels per workgroup).  Memoised per tile shape.
+static void pick_grid_strides(int ndim, const int tile[3], int* gys, int* gxs) {
+  struct Memo { int ndim, t0, t1, t2, y, x; };
+  static thread_local Memo memo[8];
+  static thread_local int nmemo = 0;
+  for (int i = 0; i < nmemo; ++i)
+    if (memo[i].ndim == ndim && memo[i].t0 == tile[0] && memo[i].t1 == tile[1] &&
+        memo[i].t2 == tile[2]) {
+      *gys = memo[i].y;
+      *gxs = memo[i].x;
+      return;
+    }
+  int64_t best = INT64_MAX;
+  *gys = tile[2];
+  *gxs = tile[1] * tile[2];
+  for (int Y = tile[2]; Y <= tile[2] + 16; ++Y) {
+    const int xmax = ndim >= 3 ? tile[1] * Y + 32 : tile[1] * Y;
+    for (int X = tile[1] * Y; X <= xmax; ++X) {
+      const int64_t score = (int64_t)grid_bank_cost(ndim, X, Y) * tile[0] * X;
+      if (score < best) {
+        best = score;
+        *gys = Y;
+        *gxs = X;
+      }
+    }
+  }
+  memo[nmemo % 8] = Memo{ndim, tile[0], tile[1], tile[2], *gys, *gxs};
+  ++nmemo;
 }
 
 static BinGeom make_bin_geom(const tnl_plan_layout& L, bool adjoint) {
@@ -474,6 +723,7 @@ static BinGeom make_bin_geom(const tnl_plan_layout& L, bool adjoint) {
     B.wcells *= B.win[d];
     B.tcells *= B.tile[d];
   }
+  if (adjoint) pick_grid_strides(L.ndim, B.tile, &B.gys, &B.gxs);
   B.npts = L.npts;
   B.nbins = L.u_nbins;
   return B;
@@ -485,8 +735,7 @@ static size_t ungrid_lds_bytes(const BinGeom& B, int cc, int comp_bytes) {
 }
 
 static size_t grid_lds_bytes(const BinGeom& B, int planes, int nt) {
-  const size_t pcells = (size_t)B.tile[0] * B.tile[1] * (B.tile[2] + 1);
-  return planes * pcells * 4 + (6 * GW + 3 * nt * GW + planes * GW) * 4;
+  return (size_t)grid_lds_layout(B, planes, nt).bytes;
 }
 
 bool binned_supported(const InterpGeom& G, const tnl_plan_layout& L,
@@ -495,11 +744,14 @@ bool binned_supported(const InterpGeom& G, const tnl_plan_layout& L,
   for (int d = 0; d < 3; ++d)
     if (G.ntap[d] > 8 || G.ntap[d] != L.ntaps[d]) return false;
   if (adjoint) {
+    const BinGeom B = make_bin_geom(L, true);
     for (int d = 3 - G.ndim; d < 3; ++d) {
-      if (G.ntap[d] > 5) return false;       // gridding lanes cover 5 taps
+      // gridding lanes cover 5 taps per axis in 3-D, 8 in 2-D
+      if (G.ntap[d] > (G.ndim == 3 ? 5 : 8)) return false;
       if (G.n[d] < G.ntap[d]) return false;  // a footprint would self-wrap
+      if (B.tile[d] >= (1 << 15)) return false;  // lo is packed in 16 bits
     }
-    return (int64_t)grid_lds_bytes(make_bin_geom(L, true), 2, 5) <= L.lds_budget;
+    return (int64_t)grid_lds_bytes(B, 1, G.ndim == 3 ? 5 : 8) <= L.lds_budget;
   }
   // at least one channel of the window must fit in LDS
   return (int64_t)ungrid_lds_bytes(make_bin_geom(L, false), 1, 8) <= L.lds_budget;
@@ -534,22 +786,35 @@ static int launch_binned(bool adjoint, const float* vals, float* out,
             (const int4*)(ws + L.off_sub),
             (const int*)(ws + L.off_suboff) + L.u_nbins + 1);
     TNL_LAUNCH_CHECK("ungrid_binned");
+  } else if constexpr (NDIM == 3 && NT > 5) {
+    return fail(TNL_EINVAL, "3-D gridding covers at most 5 taps per axis");
   } else {
     const int* nitems = (const int*)(ws + L.off_itemoff) + L.g_ntiles_total;
     const int4* items = (const int4*)(ws + L.off_items);
     grid_zero_shared_kernel<CPLX><<<1024, 256, 0, s>>>(
         out, nbatch, G, B, items, nitems, (const int*)(ws + L.off_gstarts));
     TNL_LAUNCH_CHECK("grid_zero_shared");
-    const size_t lds = grid_lds_bytes(B, CC * (CPLX ? 2 : 1), NT);
-    if (int rc = allow_lds((const void*)grid_binned_kernel<NDIM, CPLX, CC, NT>, lds))
-      return rc;
-    // persistent: a few workgroups per CU walk the work items
-    int64_t nwg = (L.item_capacity + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
-    const int64_t cap = 256 * 16;
-    if (nwg > cap) nwg = cap;
-    grid_binned_kernel<NDIM, CPLX, CC, NT><<<(unsigned)nwg, GW, lds, s>>>(
-        vals, out, nbatch, G, B, perm, slocs, (const int*)(ws + L.off_tlist),
-        items, nitems);
+    constexpr int NP = CC * (CPLX ? 2 : 1);
+    constexpr int NW = NP >= 4 ? 4 : NP;
+    const auto kern = grid_binned_kernel<NDIM, CPLX, CC, NT, NW>;
+    const size_t lds = grid_lds_bytes(B, NP, NT);
+    if (int rc = allow_lds((const void*)kern, lds)) return rc;
+    // persistent: as many workgroups as can be resident walk the items
+    int per_cu = 0, dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+      hipDeviceProp_t prop;
+      if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
+    }
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kern, GW * NW,
+                                                     lds) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    int64_t nwg = (int64_t)per_cu * ncu;
+    nwg = (nwg + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+    const int64_t need = (L.item_capacity + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+    if (nwg > need) nwg = need;
+    kern<<<(unsigned)nwg, GW * NW, lds, s>>>(vals, out, nbatch, G, B, perm, slocs,
+                                             (const int*)(ws + L.off_tlist), items, nitems);
     TNL_LAUNCH_CHECK("grid_binned");
   }
   return TNL_OK;

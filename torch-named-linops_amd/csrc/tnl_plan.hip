@@ -105,7 +105,7 @@ int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
   L->max_sub = (tune && tune->max_sub > 0) ? tune->max_sub : 2048;
   L->sub_capacity = L->u_nbins + (npts + L->max_sub - 1) / L->max_sub + 1;
   L->g_cap = (tune && tune->g_cap > 0) ? tune->g_cap : 2048;
-  L->lds_budget = (tune && tune->lds_budget > 0) ? tune->lds_budget : 40960;
+  L->lds_budget = (tune && tune->lds_budget > 0) ? tune->lds_budget : 53248;
   if (L->lds_budget > 160 * 1024)
     return fail(TNL_EINVAL, "lds_budget above the 160 KiB LDS of a CU");
   L->touch_capacity = npts * touch_per_pt;
