@@ -132,6 +132,26 @@ int tnl_grid(const float* vals, const float* locs, float* out, int64_t nbatch,
              int is_complex, float beta, const tnl_plan_layout* layout,
              const void* plan, void* stream);
 
+/* ---- trajectory gradient of interpolate / interpolate_adjoint ----------
+ * grad_locs[p, d] = sum_b sum_{g in taps(p)}
+ *                     Re(conj(pt_vals[b, p]) * grid_vals[b, g]) * dW(g, t_p)/dt_{p,d}
+ * with W the window weight of tnl_ungrid / tnl_grid and its derivative
+ * taken analytically (Kaiser-Bessel: -beta^2 u I1(z)/z with the
+ * Abramowitz & Stegun 9.8.3/9.8.4 approximations; spline: -1 on the
+ * support).  For ungrid (y = A(t) x) pass grid_vals = x, pt_vals = dL/dy;
+ * for grid (X = A(t)^H v) pass grid_vals = dL/dX, pt_vals = v.  locs and
+ * grad_locs are float32 [npts, ndim]; grad_locs is fully written.
+ * No counterpart in the reference: its autograd pair returns None for the
+ * locs gradient (src/torchlinops/functional/_interp/interp.py:46-70,
+ * 157-181).  Kernel widths up to 15.
+ */
+int tnl_interp_grad_locs(const float* grid_vals, const float* pt_vals,
+                         const float* locs, float* grad_locs, int64_t nbatch,
+                         int64_t npts, int ndim, const int64_t* grid_size,
+                         const float* width, int kernel, int norm,
+                         int pad_mode, int is_complex, float beta,
+                         void* stream);
+
 /* ---- array_to_blocks (unfold) -------------------------------------------
  * y[n, b..., j...] = x[n, b*s + j]  (pure copy, bit-exact)
  * Replaces UNFOLD[ndim][grid](x, y, nbatch, *nblocks, *block_size,

@@ -15,6 +15,10 @@
  *   grid_torch               src/torchlinops/functional/_interp/grid.py:571-639
  *   _unfold_torch            src/torchlinops/functional/_unfold/unfold.py:531-561
  *   _fold_torch              src/torchlinops/functional/_unfold/fold.py:489-517
+ *   orc_grad_locs_f64        no reference counterpart (the reference returns
+ *                            no locs gradient, interp.py:46-70, 157-181): the
+ *                            analytic d/dt of the weights above, pinned by
+ *                            scipy's I1 and by float64 finite differences
  *
  * Order of operations follows the reference: taps enumerated as
  * meshgrid(arange(w+1), indexing="ij") (x slowest), per-axis window values
@@ -84,6 +88,29 @@ void orc_kb_vec_f32(const float* x, float* out, int64_t n, float beta) {
 }
 void orc_kb_vec_f64(const double* x, double* out, int64_t n, double beta) {
   for (int64_t i = 0; i < n; ++i) out[i] = orc_kb_f64(x[i], beta);
+}
+
+/* d/du I0(beta sqrt(1 - u^2)) = -beta^2 u I1(z)/z, z = beta sqrt(1 - u^2),
+ * with I1(z)/z from Abramowitz & Stegun 9.8.3 (z < 3.75) and 9.8.4. */
+double orc_kb_deriv_f64(double u, double beta) {
+  double z = beta * sqrt(1.0 - u * u);
+  double t = z / 3.75, g;
+  if (z < 3.75) {
+    double t2 = t * t;
+    g = 0.5 + 0.87890594 * t2 + 0.51498869 * pow(t2, 2) + 0.15084934 * pow(t2, 3) +
+        0.02658733 * pow(t2, 4) + 0.00301532 * pow(t2, 5) + 0.00032411 * pow(t2, 6);
+  } else {
+    double r = 1.0 / t;
+    g = exp(z) / (z * sqrt(z)) *
+        (0.39894228 - 0.03988024 * r - 0.00362018 * pow(r, 2) +
+         0.00163801 * pow(r, 3) - 0.01031555 * pow(r, 4) +
+         0.02282967 * pow(r, 5) - 0.02895312 * pow(r, 6) +
+         0.01787654 * pow(r, 7) - 0.00420059 * pow(r, 8));
+  }
+  return -beta * beta * u * g;
+}
+void orc_kb_deriv_vec_f64(const double* u, double* out, int64_t n, double beta) {
+  for (int64_t i = 0; i < n; ++i) out[i] = orc_kb_deriv_f64(u[i], beta);
 }
 
 static float win_f32(int kernel, float d, float beta) {
@@ -427,4 +454,79 @@ void orc_fold_f64(const double* x, double* y, int64_t nbatch, int ndim,
         const int64_t o = n * g.imsz + img_off(&g, k, j);
         y[o] = y[o] + x[(n * g.nblk + k) * g.bsz + j];
       }
+}
+
+/* ------------------------------------------------- trajectory gradient
+ * grad[p, d] = sum_b sum_taps Re(conj(h[b, p]) x[b, cell]) dW/dt_{p,d},
+ * W the tap weight of taps_f64: norm 1 prod_d win(dn_d), norm 2
+ * win(|dn|), dn_d = (t_d - g_d) / (w_d / 2).  win is even in dn, so
+ * d win(dn_d)/dt_d = win'(|dn_d|) sign(dn_d) / (w_d / 2). */
+static double dwin_f64(int kernel, double u, double beta) {
+  if (kernel == KB) return orc_kb_deriv_f64(u, beta);
+  return u <= 1.0 ? -1.0 : 0.0;
+}
+
+int orc_grad_locs_f64(const double* x, const double* h, const double* locs,
+                      double* grad, int64_t nbatch, int64_t npts, int ndim,
+                      const int64_t* grid, const float* width, int kernel,
+                      int norm, int pad, int is_complex, double beta,
+                      int nthreads) {
+  geom_t G;
+  make_geom(&G, ndim, grid, width, kernel, norm, pad);
+  const int comp = is_complex ? 2 : 1;
+  const int64_t nc = ncells_of(&G);
+  (void)nthreads;
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int64_t p = 0; p < npts; ++p) {
+    const double* t = locs + p * ndim;
+    double r[3];
+    int64_t lower[3];
+    for (int d = 0; d < ndim; ++d) {
+      r[d] = (double)width[d] / 2.0;
+      lower[d] = (int64_t)ceil(t[d] - r[d]);
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int64_t j = 0; j < G.ntaps_total; ++j) {
+      int64_t rem = j, k[3];
+      for (int d = ndim - 1; d >= 0; --d) {
+        k[d] = rem % G.ntap[d];
+        rem /= G.ntap[d];
+      }
+      int keep = 1;
+      double dn[3], s2 = 0.0;
+      int64_t c = 0;
+      for (int d = 0; d < ndim; ++d) {
+        int64_t g = lower[d] + k[d];
+        double diff = t[d] - (double)g;
+        dn[d] = diff / r[d];
+        if (!(fabs(diff) <= r[d])) keep = 0;
+        if (G.pad == ZERO && (g < 0 || g >= G.n[d])) keep = 0;
+        s2 += dn[d] * dn[d];
+        c = c * G.n[d] + mod_pos(g, G.n[d]);
+      }
+      if (!keep) continue;
+      double dW[3] = {0.0, 0.0, 0.0};
+      if (norm == 1) {
+        for (int d = 0; d < ndim; ++d) {
+          double prod = 1.0;
+          for (int e = 0; e < ndim; ++e)
+            if (e != d) prod *= win_f64(kernel, dn[e], beta);
+          double sg = dn[d] > 0 ? 1.0 : (dn[d] < 0 ? -1.0 : 0.0);
+          dW[d] = dwin_f64(kernel, fabs(dn[d]), beta) * sg / r[d] * prod;
+        }
+      } else {
+        double rr = sqrt(s2);
+        if (rr <= 1.0 && rr > 0.0)
+          for (int d = 0; d < ndim; ++d)
+            dW[d] = dwin_f64(kernel, rr, beta) * dn[d] / (rr * r[d]);
+      }
+      double s = 0.0;
+      for (int64_t b = 0; b < nbatch; ++b)
+        for (int cc = 0; cc < comp; ++cc)
+          s += h[(b * npts + p) * comp + cc] * x[(b * nc + c) * comp + cc];
+      for (int d = 0; d < ndim; ++d) acc[d] += s * dW[d];
+    }
+    for (int d = 0; d < ndim; ++d) grad[p * ndim + d] = acc[d];
+  }
+  return 0;
 }

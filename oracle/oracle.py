@@ -62,6 +62,9 @@ def lib():
         h.orc_grid_f64.argtypes = common + [ctypes.c_double]
         for f in (h.orc_ungrid_f32, h.orc_ungrid_f64, h.orc_grid_f32, h.orc_grid_f64):
             f.restype = _i32
+        h.orc_kb_deriv_vec_f64.argtypes = [_vp, _vp, _i64, ctypes.c_double]
+        h.orc_grad_locs_f64.argtypes = [_vp] + common + [ctypes.c_double, _i32]
+        h.orc_grad_locs_f64.restype = _i32
         h.orc_unfold.argtypes = [_vp, _vp, _i32, _i64, _i32, _vp, _vp, _vp, _vp]
         h.orc_fold_f32.argtypes = [_vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp]
         h.orc_fold_f64.argtypes = [_vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp]
@@ -92,6 +95,14 @@ def kb(x, beta: float, precision: str = "f32") -> np.ndarray:
         x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
         out = np.empty_like(x)
         lib().orc_kb_vec_f64(_ptr(x), _ptr(out), x.size, float(beta))
+    return out
+
+
+def kb_deriv(u, beta: float) -> np.ndarray:
+    """d/du of the Kaiser-Bessel window, float64 (A&S 9.8.3 / 9.8.4)."""
+    u = np.ascontiguousarray(np.asarray(u, dtype=np.float64))
+    out = np.empty_like(u)
+    lib().orc_kb_deriv_vec_f64(_ptr(u), _ptr(out), u.size, float(beta))
     return out
 
 
@@ -168,6 +179,38 @@ def grid(vals, locs, grid_size, width, kernel="kaiser_bessel", norm=1,
 
 
 # ------------------------------------------------------- unfold / fold
+def grad_locs(grid_vals, pt_vals, locs, width, kernel="kaiser_bessel", norm=1,
+              pad_mode="circular", beta=1.0, nthreads=1):
+    """float64 trajectory gradient: sum_b Re(conj(pt_vals) * dW/dlocs x grid_vals).
+
+    grid_vals [B..., *grid], pt_vals [B..., L...], locs [L..., D] -> [L..., D].
+    For y = ungrid(x, locs) and a real loss with upstream gradient gy this is
+    dL/dlocs = grad_locs(x, gy, locs); for X = grid(v, locs) with upstream gX
+    it is grad_locs(gX, v, locs).
+    """
+    locs = np.asarray(locs)
+    ndim = locs.shape[-1]
+    grid_size = grid_vals.shape[grid_vals.ndim - ndim:]
+    batch = grid_vals.shape[: grid_vals.ndim - ndim]
+    lb = locs.shape[:-1]
+    nb, npts = prod(batch), prod(lb)
+    width = _width(width, ndim)
+    is_complex = np.iscomplexobj(grid_vals) or np.iscomplexobj(pt_vals)
+    if is_complex:
+        grid_vals = grid_vals.astype(np.complex128)
+        pt_vals = pt_vals.astype(np.complex128)
+    x, _, dt, _ = _as_interleaved(grid_vals.reshape(nb, *grid_size), "f64")
+    h, _, _, _ = _as_interleaved(pt_vals.reshape(nb, npts), "f64")
+    l = np.ascontiguousarray(locs.reshape(npts, ndim).astype(np.float64))
+    out = np.empty((npts, ndim), dtype=np.float64)
+    rc = lib().orc_grad_locs_f64(
+        _ptr(x), _ptr(h), _ptr(l), _ptr(out), nb, npts, ndim, _ptr(_i64arr(grid_size)),
+        _ptr(_f32arr(width)), KERNELS[kernel], norm, PADS[pad_mode], int(is_complex),
+        float(beta), int(nthreads))
+    assert rc == 0
+    return out.reshape(*lb, ndim)
+
+
 def get_nblocks(im_size, block_size, stride=None):
     stride = stride if stride is not None else (1,) * len(block_size)
     return tuple((i - b) // s + 1 for i, b, s in zip(im_size, block_size, stride))

@@ -29,3 +29,33 @@ def radial_3d(nspokes: int, nread: int, n: int, rng: np.random.Generator) -> np.
 
 def to_dev(x, dev):
     return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+
+def spiral_2d(ninter: int, nsamp: int, n: int) -> np.ndarray:
+    """Archimedean spiral in centred grid units [-n/2, n/2): ninter
+    interleaves of nsamp samples, radius growing linearly to n/2 with
+    n / (2 * ninter) turns per interleave (Nyquist spacing between turns)."""
+    s = np.arange(nsamp) / nsamp
+    turns = n / (2.0 * ninter)
+    r = (n / 2.0) * s
+    out = np.empty((ninter, nsamp, 2), dtype=np.float64)
+    for l in range(ninter):
+        th = 2 * np.pi * (turns * s + l / ninter)
+        out[l, :, 0] = r * np.cos(th)
+        out[l, :, 1] = r * np.sin(th)
+    return out.astype(np.float32)
+
+
+def ellipse_phantom(n: int, rng: np.random.Generator, nell: int = 10) -> np.ndarray:
+    """Sum of random ellipses with random complex amplitudes on an n x n grid."""
+    yy, xx = np.meshgrid(np.linspace(-1, 1, n), np.linspace(-1, 1, n), indexing="ij")
+    img = np.zeros((n, n), dtype=np.complex128)
+    for _ in range(nell):
+        cx, cy = rng.uniform(-0.5, 0.5, 2)
+        a, b = rng.uniform(0.1, 0.5, 2)
+        th = rng.uniform(0, np.pi)
+        u = (xx - cx) * np.cos(th) + (yy - cy) * np.sin(th)
+        v = -(xx - cx) * np.sin(th) + (yy - cy) * np.cos(th)
+        amp = rng.uniform(0.2, 1.0) * np.exp(1j * rng.uniform(0, 2 * np.pi))
+        img[(u / a) ** 2 + (v / b) ** 2 <= 1] += amp
+    return img.astype(np.complex64)

@@ -281,3 +281,113 @@ def test_cg_known_answers():
     A = M.conj().T @ M + 0.5 * np.eye(6)
     x = O.conjugate_gradients(lambda v: A @ v, y[:6], max_num_iters=6)
     assert np.allclose(x, np.linalg.solve(A, y[:6]), rtol=1e-8, atol=1e-10)
+
+
+# ------------------------------------------------- trajectory gradient
+# The oracle's analytic window derivative and trajectory gradient have no
+# reference counterpart; they are pinned by scipy's I1 and by float64
+# central differences of the oracle's own (reference-following) ungrid.
+
+@pytest.mark.parametrize("beta", [1.0, 2.5, BETA_NUFFT, 14.0])
+def test_kb_deriv_matches_scipy_i1(beta):
+    from scipy.special import i1
+
+    u = np.linspace(0, 1, 2001)
+    z = beta * np.sqrt(1 - u * u)
+    ratio = np.where(z > 0, i1(z) / np.where(z > 0, z, 1.0), 0.5)  # I1(z)/z -> 1/2
+    ref = -beta * beta * u * ratio
+    got = O.kb_deriv(u, beta)
+    # A&S 9.8.3 / 9.8.4: |rel err| of I1 below 2.2e-7
+    assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300 + beta * beta * 1e-3)) < 3e-7
+
+
+@pytest.mark.parametrize("beta", [2.5, BETA_NUFFT])
+def test_kb_deriv_matches_fd_of_kb(beta):
+    u = np.linspace(0.01, 0.99, 99)
+    h = 1e-6
+    fd = (O.kb(u + h, beta, "f64") - O.kb(u - h, beta, "f64")) / (2 * h)
+    got = O.kb_deriv(u, beta)
+    # I0 and I1 are separate A&S fits: their derivative relation holds to ~1e-7
+    assert np.max(np.abs(got - fd) / np.abs(fd)) < 1e-6
+
+
+def _away_from_tap_changes(t, width, margin):
+    """True where moving t by +-margin changes no tap set: some tap sits at
+    |g - t| = w/2 exactly when t - w/2 is an integer."""
+    f = np.mod(t - np.asarray(width) / 2.0, 1.0)
+    return (f > margin) & (f < 1 - margin)
+
+
+@pytest.mark.parametrize("ndim,kernel,norm,pad,cplx", [
+    (1, "kaiser_bessel", 1, "circular", True),
+    (2, "kaiser_bessel", 1, "circular", True),
+    (2, "kaiser_bessel", 2, "zero", False),
+    (2, "spline", 1, "zero", True),
+    (3, "kaiser_bessel", 1, "zero", True),
+    (3, "spline", 2, "circular", False),
+])
+def test_oracle_grad_locs_matches_fd(ndim, kernel, norm, pad, cplx):
+    rng = np.random.default_rng(40 + ndim + norm)
+    grid = (12, 10, 9)[:ndim]
+    width = (4.0, 3.0, 3.5)[:ndim]
+    beta = 5.0
+    B, M = 2, 40
+    x = rng.standard_normal((B, *grid)) + (1j * rng.standard_normal((B, *grid)) if cplx else 0)
+    h = rng.standard_normal((B, M)) + (1j * rng.standard_normal((B, M)) if cplx else 0)
+    locs = rng.uniform(-1.0, np.array(grid) + 1.0, (M, ndim))
+    kw = dict(kernel=kernel, norm=norm, pad_mode=pad, beta=beta)
+
+    def loss(t):  # L(t) = sum Re(conj(h) * ungrid(x, t)), linear in x
+        y = O.ungrid(x, t, width, precision="f64", **kw)
+        return float(np.sum((np.conj(h) * y).real))
+
+    g = O.grad_locs(x, h, locs, width, **kw)
+    step = 1e-6
+    checked = 0
+    for p in range(M):
+        for d in range(ndim):
+            if not _away_from_tap_changes(locs[p, d], width[d], 10 * step):
+                continue
+            if norm == 2 or kernel == "spline":
+                # kinks where a tap crosses |u| = 1 radially or u = 0 (hat)
+                frac = np.mod(locs[p, d], 1.0)
+                if min(frac, 1 - frac) < 10 * step:
+                    continue
+            tp, tm = locs.copy(), locs.copy()
+            tp[p, d] += step
+            tm[p, d] -= step
+            fd = (loss(tp) - loss(tm)) / (2 * step)
+            assert abs(g[p, d] - fd) <= 1e-5 * (1 + abs(fd)), (p, d, g[p, d], fd)
+            checked += 1
+    assert checked > M * ndim // 2
+    # the grid form: X = grid(v, t), L = sum Re(conj(G) X)  ->  grad_locs(G, v, t)
+    G = x
+    v = h
+
+    def loss_grid(t):
+        X = O.grid(v, t, grid, width, precision="f64", **kw)
+        return float(np.sum((np.conj(G) * X).real))
+
+    p, d = 3, ndim - 1
+    if _away_from_tap_changes(locs[p, d], width[d], 1e-4):
+        tp, tm = locs.copy(), locs.copy()
+        tp[p, d] += step
+        tm[p, d] -= step
+        fd = (loss_grid(tp) - loss_grid(tm)) / (2 * step)
+        assert abs(O.grad_locs(G, v, locs, width, **kw)[p, d] - fd) <= 1e-5 * (1 + abs(fd))
+
+
+def test_oracle_grad_locs_planted_sign_error_is_caught():
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((1, 16, 16)) + 1j * rng.standard_normal((1, 16, 16))
+    h = rng.standard_normal((1, 5)) + 1j * rng.standard_normal((1, 5))
+    locs = rng.uniform(2, 14, (5, 2)) + 0.25
+    g = O.grad_locs(x, h, locs, 4.0, beta=5.0)
+    step = 1e-6
+    tp, tm = locs.copy(), locs.copy()
+    tp[0, 1] += step
+    tm[0, 1] -= step
+    f = lambda t: float(np.sum((np.conj(h) * O.ungrid(x, t, 4.0, beta=5.0, precision="f64")).real))
+    fd = (f(tp) - f(tm)) / (2 * step)
+    assert abs(g[0, 1] - fd) < 1e-5 * (1 + abs(fd))
+    assert abs(-g[0, 1] - fd) > 1e-3 * (1 + abs(fd))  # a flipped sign would fail

@@ -146,3 +146,24 @@ extern "C" int tnl_grid(const float* vals, const float* locs, float* out,
                       width, kernel, norm, pad_mode, is_complex, beta, layout,
                       plan, stream);
 }
+
+extern "C" int tnl_interp_grad_locs(const float* grid_vals, const float* pt_vals,
+                                    const float* locs, float* grad_locs,
+                                    int64_t nbatch, int64_t npts, int ndim,
+                                    const int64_t* grid_size, const float* width,
+                                    int kernel, int norm, int pad_mode,
+                                    int is_complex, float beta, void* stream) {
+  InterpGeom G;
+  if (int rc = make_geom(ndim, grid_size, width, kernel, norm, pad_mode,
+                         is_complex, beta, G))
+    return rc;
+  if (nbatch < 0 || npts < 0)
+    return fail(TNL_EINVAL, "nbatch and npts must be >= 0");
+  if (npts == 0) return TNL_OK;
+  if (!grad_locs || !locs) return fail(TNL_EINVAL, "null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (nbatch == 0) return fill_zero(grad_locs, npts * ndim, s);
+  if (!grid_vals || !pt_vals) return fail(TNL_EINVAL, "null input pointer");
+  return interp_grad_locs(grid_vals, pt_vals, locs, grad_locs, nbatch, npts, G,
+                          is_complex, s);
+}

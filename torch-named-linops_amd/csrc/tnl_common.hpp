@@ -66,6 +66,49 @@ __device__ __forceinline__ float window(int kernel, float x, float beta) {
                                             : spline_window(x);
 }
 
+// d/du of the Kaiser-Bessel window I0(beta * sqrt(1 - u^2)) for u in [0, 1]:
+//   -beta^2 * u * I1(z) / z,  z = beta * sqrt(1 - u^2),
+// which stays finite at u = 1 (I1(z)/z -> 1/2).  I1(z)/z is the Abramowitz &
+// Stegun 9.8.3 polynomial for z < 3.75 and 9.8.4 (z^1/2 e^-z I1(z)) above,
+// the companions of the I0 approximations used by kb_window.
+__device__ __forceinline__ float kb_window_deriv(float u, float beta) {
+  const float z = beta * sqrtf(fmaxf(1.0f - u * u, 0.0f));
+  float g;  // I1(z) / z
+  if (z < 3.75f) {
+    const float t = z * (1.0f / 3.75f);
+    const float q = t * t;
+    g = 0.5f +
+        q * (0.87890594f +
+             q * (0.51498869f +
+                  q * (0.15084934f +
+                       q * (0.02658733f + q * (0.00301532f + q * 0.00032411f)))));
+  } else {
+    const float rz = __frcp_rn(z);
+    const float r = 3.75f * rz;
+    const float poly =
+        0.39894228f +
+        r * (-0.03988024f +
+             r * (-0.00362018f +
+                  r * (0.00163801f +
+                       r * (-0.01031555f +
+                            r * (0.02282967f +
+                                 r * (-0.02895312f +
+                                      r * (0.01787654f + r * -0.00420059f)))))));
+    g = expf(z) * sqrtf(rz) * rz * poly;
+  }
+  return -beta * beta * u * g;
+}
+
+// d/du of the hat max(1 - u, 0) on its support u in [0, 1].
+__device__ __forceinline__ float spline_window_deriv(float u) {
+  return u <= 1.0f ? -1.0f : 0.0f;
+}
+
+__device__ __forceinline__ float window_deriv(int kernel, float u, float beta) {
+  return kernel == TNL_KERNEL_KAISER_BESSEL ? kb_window_deriv(u, beta)
+                                            : spline_window_deriv(u);
+}
+
 __device__ __forceinline__ int64_t mod_pos(int64_t g, int64_t n) {
   int64_t r = g % n;
   return r < 0 ? r + n : r;

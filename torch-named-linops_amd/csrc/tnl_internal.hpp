@@ -19,6 +19,11 @@ int interp_binned(bool adjoint, const float* vals, float* out, int64_t nbatch,
                   const InterpGeom& G, int is_complex, const tnl_plan_layout& L,
                   const void* plan, hipStream_t s);
 
+int interp_grad_locs(const float* grid_vals, const float* pt_vals,
+                     const float* locs, float* grad, int64_t nbatch,
+                     int64_t npts, const InterpGeom& G, int is_complex,
+                     hipStream_t s);
+
 int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
                 const float* width, int pad_mode, const tnl_tuning* tune,
                 tnl_plan_layout* L);

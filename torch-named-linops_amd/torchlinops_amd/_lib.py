@@ -123,6 +123,11 @@ SIGNATURES = {
         [_vp, _vp, _vp, _i64, _i64, _i32, _i64p, _f32p, _i32, _i32, _i32,
          _i32, _f32, _layout_p, _vp, _vp],
     ),
+    "tnl_interp_grad_locs": (
+        _i32,
+        [_vp, _vp, _vp, _vp, _i64, _i64, _i32, _i64p, _f32p, _i32, _i32, _i32,
+         _i32, _f32, _vp],
+    ),
     "tnl_unfold": (
         _i32,
         [_vp, _vp, _i32, _i64, _i32, _i64p, _i64p, _i64p, _i64p, _vp],

@@ -1,6 +1,6 @@
 """Functional API of the gridding path (mirrors torchlinops.functional)."""
 
-from ._interp import grid, interpolate, interpolate_adjoint, ungrid
+from ._interp import grid, interp_grad_locs, interpolate, interpolate_adjoint, ungrid
 from ._nufft import center_crop, center_pad, nufft, nufft_adjoint
 from ._plan import clear_plan_cache
 from ._unfold import (
@@ -17,6 +17,7 @@ __all__ = [
     "ungrid",
     "interpolate",
     "interpolate_adjoint",
+    "interp_grad_locs",
     "array_to_blocks",
     "blocks_to_array",
     "fold",

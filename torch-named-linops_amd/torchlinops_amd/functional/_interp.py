@@ -10,6 +10,11 @@ functional API:
   * autograd pair                  interp.py:13-236
 The arithmetic runs in the HIP kernels of libtnl (tnl_ungrid / tnl_grid);
 tensors must live on a HIP device.
+
+Beyond the reference, the autograd pair also differentiates with respect to
+``locs`` (the reference returns None there, interp.py:46-70, 157-181): the
+gradient comes from the analytic window derivative in the
+``tnl_interp_grad_locs`` kernel (``interp_grad_locs`` below).
 """
 
 from __future__ import annotations
@@ -35,6 +40,7 @@ from ._plan import get_plan
 __all__ = [
     "ungrid",
     "grid",
+    "interp_grad_locs",
     "interpolate",
     "interpolate_adjoint",
     "get_kernel_params",
@@ -222,13 +228,77 @@ def grid(
     return out.reshape(*batch_shape, *grid_size)
 
 
+def interp_grad_locs(
+    grid_vals: torch.Tensor,
+    pt_vals: torch.Tensor,
+    locs: torch.Tensor,
+    width,
+    kernel: str = "kaiser_bessel",
+    norm: int = 1,
+    pad_mode: str = "circular",
+    kernel_params: Optional[dict] = None,
+) -> torch.Tensor:
+    """Trajectory gradient shared by ungrid and grid.
+
+    ``grid_vals[*batch, *grid]`` and ``pt_vals[*batch, *locs.shape[:-1]]``;
+    returns ``sum_b Re(conj(pt_vals) * dW/dlocs applied to grid_vals)`` with
+    the shape and dtype of ``locs``.  For ``y = ungrid(x, locs)`` and upstream
+    gradient ``gy`` it is ``dL/dlocs = interp_grad_locs(x, gy, locs)``; for
+    ``X = grid(v, locs)`` with upstream ``gX`` it is
+    ``interp_grad_locs(gX, v, locs)``.
+    """
+    kernel_params = get_kernel_params(kernel, kernel_params)
+    ndim = locs.shape[-1]
+    grid_size = tuple(grid_vals.shape[-ndim:])
+    batch_shape = grid_vals.shape[:-ndim]
+    width = _width_tuple(width, ndim)
+    npts = prod(locs.shape[:-1])
+    nbatch = prod(batch_shape)
+    if tuple(pt_vals.shape) != tuple(batch_shape) + tuple(locs.shape[:-1]):
+        raise RuntimeError(
+            f"interp_grad_locs: pt_vals shape {tuple(pt_vals.shape)} does not match "
+            f"batch {tuple(batch_shape)} + points {tuple(locs.shape[:-1])}"
+        )
+    is_complex = torch.is_complex(grid_vals)
+    if torch.is_complex(pt_vals) != is_complex:
+        raise TypeError("grid_vals and pt_vals must both be real or both complex")
+    if kernel not in KERNELS:
+        raise ValueError(f"Unrecognized kernel {kernel}")
+    if norm not in (1, 2):
+        raise ValueError(f"Unexpected norm: {norm}.")
+    if pad_mode not in PAD_MODES:
+        raise ValueError(f"Unrecognized pad_mode {pad_mode}")
+    require_device(grid_vals, pt_vals, locs, op="interp_grad_locs")
+    _check_dtypes(grid_vals, locs)
+    _check_dtypes(pt_vals, locs)
+    g = grid_vals.reshape(nbatch, *grid_size).contiguous()
+    h = pt_vals.reshape(nbatch, npts).contiguous()
+    if is_complex:
+        g = torch.view_as_real(g)
+        h = torch.view_as_real(h)
+    locs2d = locs.detach().reshape(npts, ndim).to(torch.float32).contiguous()
+    out = torch.empty(npts, ndim, dtype=torch.float32, device=locs.device)
+    beta = float(kernel_params.get("beta", 1.0)) if kernel == "kaiser_bessel" else 1.0
+    if npts > 0:
+        with torch.cuda.device(locs.device):
+            check(
+                lib().tnl_interp_grad_locs(
+                    g.data_ptr(), h.data_ptr(), locs2d.data_ptr(), out.data_ptr(),
+                    nbatch, npts, ndim, i64_array(grid_size), f32_array(width),
+                    KERNELS[kernel], int(norm), PAD_MODES[pad_mode], int(is_complex),
+                    beta, stream_of(locs2d),
+                )
+            )
+    return out.reshape(locs.shape).to(locs.dtype)
+
+
 class InterpolateFn(Function):
     """``ungrid`` with ``grid`` as its vals-gradient (interp.py:13-70)."""
 
     @staticmethod
     def forward(vals, locs, width, kernel, norm, pad_mode, kernel_params):
         out = ungrid(vals, locs, width, kernel, norm, pad_mode, kernel_params)
-        out.requires_grad_(vals.requires_grad)
+        out.requires_grad_(vals.requires_grad or locs.requires_grad)
         return out
 
     @staticmethod
@@ -237,17 +307,24 @@ class InterpolateFn(Function):
         ndim = locs.shape[-1]
         ctx.grid_size = tuple(vals.shape[-ndim:])
         ctx.params = (width, kernel, norm, pad_mode, kernel_params)
-        ctx.save_for_backward(locs)
+        if ctx.needs_input_grad[1]:
+            ctx.save_for_backward(locs, vals)
+        else:
+            ctx.save_for_backward(locs)
 
     @staticmethod
     def backward(ctx, grad_output):
-        grad_vals = None
+        grad_vals = grad_locs = None
+        locs = ctx.saved_tensors[0]
+        width, kernel, norm, pad_mode, kernel_params = ctx.params
         if ctx.needs_input_grad[0]:
-            (locs,) = ctx.saved_tensors
-            width, kernel, norm, pad_mode, kernel_params = ctx.params
             grad_vals = grid(grad_output, locs, ctx.grid_size, width, kernel,
                              norm, pad_mode, kernel_params)
-        return grad_vals, None, None, None, None, None, None
+        if ctx.needs_input_grad[1]:
+            vals = ctx.saved_tensors[1]
+            grad_locs = interp_grad_locs(vals, grad_output, locs, width, kernel,
+                                         norm, pad_mode, kernel_params)
+        return grad_vals, grad_locs, None, None, None, None, None
 
 
 def interpolate(
@@ -270,24 +347,31 @@ class InterpolateAdjointFn(Function):
     @staticmethod
     def forward(vals, locs, grid_size, width, kernel, norm, pad_mode, kernel_params):
         out = grid(vals, locs, grid_size, width, kernel, norm, pad_mode, kernel_params)
-        out.requires_grad_(vals.requires_grad)
+        out.requires_grad_(vals.requires_grad or locs.requires_grad)
         return out
 
     @staticmethod
     def setup_context(ctx, inputs, output):
         vals, locs, grid_size, width, kernel, norm, pad_mode, kernel_params = inputs
         ctx.params = (width, kernel, norm, pad_mode, kernel_params)
-        ctx.save_for_backward(locs)
+        if ctx.needs_input_grad[1]:
+            ctx.save_for_backward(locs, vals)
+        else:
+            ctx.save_for_backward(locs)
 
     @staticmethod
     def backward(ctx, grad_output):
-        grad_vals = None
+        grad_vals = grad_locs = None
+        locs = ctx.saved_tensors[0]
+        width, kernel, norm, pad_mode, kernel_params = ctx.params
         if ctx.needs_input_grad[0]:
-            (locs,) = ctx.saved_tensors
-            width, kernel, norm, pad_mode, kernel_params = ctx.params
             grad_vals = ungrid(grad_output, locs, width, kernel, norm, pad_mode,
                                kernel_params)
-        return grad_vals, None, None, None, None, None, None, None
+        if ctx.needs_input_grad[1]:
+            vals = ctx.saved_tensors[1]
+            grad_locs = interp_grad_locs(grad_output, vals, locs, width, kernel,
+                                         norm, pad_mode, kernel_params)
+        return grad_vals, grad_locs, None, None, None, None, None, None
 
 
 def interpolate_adjoint(
