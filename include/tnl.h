@@ -132,6 +132,97 @@ int tnl_grid(const float* vals, const float* locs, float* out, int64_t nbatch,
              int is_complex, float beta, const tnl_plan_layout* layout,
              const void* plan, void* stream);
 
+/* ---- many problems in one persistent launch ----------------------------
+ * A batch is a set of independent interpolation problems of one ndim,
+ * kernel, norm, pad_mode and dtype, each with its own grid size, width,
+ * beta, channel count, trajectory and plan (tnl_plan_build).  Grid-side
+ * data of all problems live
+ * in one buffer, point-side data in another, each problem at its own
+ * element offset:
+ *   grid buffer   problem i: [nbatch_i, *grid_size_i] at grid_offset_i
+ *   point buffer  problem i: [nbatch_i, npts_i]       at pts_offset_i
+ * tnl_batch_build copies a problem table into a device workspace once;
+ * tnl_ungrid_batched / tnl_grid_batched then run every problem's work in
+ * one persistent launch each (no host work, capturable in a graph).
+ * Results equal tnl_ungrid / tnl_grid called per problem with its plan.
+ * No counterpart in the reference, which launches per problem
+ * (src/torchlinops/functional/_interp/ungrid.py:86-119, grid.py:91-124).
+ */
+typedef struct tnl_batch_problem {
+  int64_t nbatch;         /* channels (coils) */
+  int64_t npts;
+  int64_t grid_size[3];   /* first ndim entries */
+  float width[3];         /* first ndim entries */
+  float beta;             /* Kaiser-Bessel beta */
+  int64_t grid_offset;    /* element offset in the grid buffer */
+  int64_t pts_offset;     /* element offset in the point buffer */
+  const tnl_plan_layout* layout;  /* this problem's plan */
+  const void* plan;
+} tnl_batch_problem;
+
+typedef struct tnl_batch_layout {
+  int32_t nprob, ndim, kernel, norm, pad_mode, is_complex;
+  int32_t nt;             /* tap capacity per axis of the launches */
+  int32_t ungrid_cc;      /* channels per ungrid workgroup */
+  int32_t grid_cc;        /* channels per gridding accumulator */
+  int32_t ungrid_chunks;  /* max over problems of ceil(nbatch / ungrid_cc) */
+  int64_t total_sub;      /* sum of the plans' subproblem capacities */
+  int64_t total_items;    /* sum of the plans' gridding work-item capacities */
+  int64_t ungrid_lds;     /* LDS bytes per ungrid workgroup */
+  int64_t grid_lds;       /* LDS bytes per gridding workgroup */
+  int64_t bytes;          /* device workspace for the problem table */
+} tnl_batch_layout;
+
+int tnl_batch_layout_query(const tnl_batch_problem* probs, int nprob, int ndim,
+                           int kernel, int norm, int pad_mode, int is_complex,
+                           tnl_batch_layout* layout);
+/* Blocking (synchronises `stream`): call once, outside graph capture. */
+int tnl_batch_build(const tnl_batch_problem* probs,
+                    const tnl_batch_layout* layout, void* ws, void* stream);
+int tnl_ungrid_batched(const float* grid_vals, float* pts_out,
+                       const tnl_batch_layout* layout, const void* ws,
+                       void* stream);
+int tnl_grid_batched(const float* pts_vals, float* grid_out,
+                     const tnl_batch_layout* layout, const void* ws,
+                     void* stream);
+
+/* ---- segmented solver kernels -------------------------------------------
+ * For conjugate gradients over many independent problems held in one flat
+ * float buffer (complex as interleaved floats).  Segment s is the float
+ * range of problem s; seg_len[s] is its length in floats.  Dot products
+ * are Re(conj(a) . b) per segment, accumulated in float64 in a fixed order
+ * (deterministic, no atomics, no memsets: capturable in a graph).  The CG
+ * step follows src/torchlinops/alg/pcg.py:67-87 per segment:
+ *   tnl_seg_cg_update:    alpha = rs / pAp;  x += alpha p;  r -= alpha Ap;
+ *                         rs_new = |r|^2
+ *   tnl_seg_cg_direction: beta = rs_new / rs; p = r + beta p; rs = rs_new
+ * (alpha, beta are 0 where their denominator is 0).  No counterpart in the
+ * reference, whose CG (pcg.py:13-88) runs one problem with host scalars.
+ */
+typedef struct tnl_seg_layout {
+  int32_t nseg;
+  int32_t reserved;
+  int64_t numel;         /* floats in the buffer */
+  int64_t block_floats;  /* floats per reduction block */
+  int64_t nblocks;
+  int64_t off_blocks, off_blkseg, off_segblk, off_partial;
+  int64_t bytes;         /* device workspace */
+} tnl_seg_layout;
+
+int tnl_seg_layout_query(const int64_t* seg_len, int nseg, int64_t block_floats,
+                         tnl_seg_layout* layout);
+/* Blocking (synchronises `stream`): call once, outside graph capture. */
+int tnl_seg_build(const int64_t* seg_len, const tnl_seg_layout* layout, void* ws,
+                  void* stream);
+int tnl_seg_dot(const float* a, const float* b, double* out,
+                const tnl_seg_layout* layout, void* ws, void* stream);
+int tnl_seg_cg_update(float* x, float* r, const float* p, const float* Ap,
+                      const double* rs, const double* pAp, double* rs_new,
+                      const tnl_seg_layout* layout, void* ws, void* stream);
+int tnl_seg_cg_direction(float* p, const float* r, double* rs,
+                         const double* rs_new, const tnl_seg_layout* layout,
+                         void* ws, void* stream);
+
 /* ---- trajectory gradient of interpolate / interpolate_adjoint ----------
  * grad_locs[p, d] = sum_b sum_{g in taps(p)}
  *                     Re(conj(pt_vals[b, p]) * grid_vals[b, g]) * dW(g, t_p)/dt_{p,d}

@@ -59,3 +59,20 @@ def ellipse_phantom(n: int, rng: np.random.Generator, nell: int = 10) -> np.ndar
         amp = rng.uniform(0.2, 1.0) * np.exp(1j * rng.uniform(0, 2 * np.pi))
         img[(u / a) ** 2 + (v / b) ** 2 <= 1] += amp
     return img.astype(np.complex64)
+
+
+def vd_spiral_2d(npts: int, n: int, rng: np.random.Generator, power: float = 1.5) -> np.ndarray:
+    """Variable-density spiral in centred grid units: a random number of
+    interleaves, radius (n/2) * s^power (denser centre for power > 1)."""
+    ninter = int(rng.integers(4, 33))
+    per = max(1, npts // ninter)
+    s = (np.arange(per) + 0.5) / per
+    r = (n / 2.0) * s**power
+    turns = max(1.0, n / (4.0 * ninter))
+    phase0 = rng.uniform(0, 2 * np.pi)
+    out = []
+    for l in range(ninter):
+        th = 2 * np.pi * (turns * s + l / ninter) + phase0
+        out.append(np.stack([r * np.cos(th), r * np.sin(th)], -1))
+    k = np.concatenate(out, 0)[:npts]
+    return np.clip(k, -n / 2, n / 2 - 1e-3).astype(np.float32)

@@ -1,5 +1,6 @@
 // extern "C" entry points for interpolation / gridding and error plumbing.
 #include <cstring>
+#include <vector>
 
 #include "tnl_common.hpp"
 #include "tnl_internal.hpp"
@@ -166,4 +167,72 @@ extern "C" int tnl_interp_grad_locs(const float* grid_vals, const float* pt_vals
   if (!grid_vals || !pt_vals) return fail(TNL_EINVAL, "null input pointer");
   return interp_grad_locs(grid_vals, pt_vals, locs, grad_locs, nbatch, npts, G,
                           is_complex, s);
+}
+
+// ------------------------------------------------------------- batches
+static int batch_geoms(const tnl_batch_problem* probs, int nprob, int ndim,
+                       int kernel, int norm, int pad_mode, int is_complex,
+                       InterpGeom* G) {
+  for (int i = 0; i < nprob; ++i) {
+    const tnl_batch_problem& p = probs[i];
+    float width[3] = {p.width[0], p.width[1], p.width[2]};
+    if (int rc = make_geom(ndim, p.grid_size, width, kernel, norm, pad_mode,
+                           is_complex, p.beta, G[i]))
+      return rc;
+    if (p.nbatch < 1 || p.npts < 0 || p.grid_offset < 0 || p.pts_offset < 0)
+      return fail(TNL_EINVAL, "batch problem: nbatch >= 1, npts and offsets >= 0");
+    if (!p.layout || !p.plan) return fail(TNL_EINVAL, "batch problem without a plan");
+    const tnl_plan_layout& L = *p.layout;
+    bool same = L.ndim == ndim && L.npts == p.npts && L.pad_mode == pad_mode;
+    for (int d = 0; same && d < 3; ++d)
+      same = L.grid_size[d] == G[i].n[d] && L.ntaps[d] == G[i].ntap[d] &&
+             L.width[d] == (d >= 3 - ndim ? width[d - (3 - ndim)] : 1.0f);
+    if (!same)
+      return fail(TNL_EINVAL, "batch problem: plan built for another size, grid, width or pad_mode");
+  }
+  return TNL_OK;
+}
+
+extern "C" int tnl_batch_layout_query(const tnl_batch_problem* probs, int nprob,
+                                      int ndim, int kernel, int norm,
+                                      int pad_mode, int is_complex,
+                                      tnl_batch_layout* layout) {
+  if (!probs || !layout || nprob < 1)
+    return fail(TNL_EINVAL, "batch needs at least one problem");
+  std::vector<InterpGeom> G((size_t)nprob);
+  if (int rc = batch_geoms(probs, nprob, ndim, kernel, norm, pad_mode, is_complex, G.data()))
+    return rc;
+  std::memset(layout, 0, sizeof(*layout));
+  layout->nprob = nprob;
+  layout->ndim = ndim;
+  layout->kernel = kernel;
+  layout->norm = norm;
+  layout->pad_mode = pad_mode;
+  layout->is_complex = is_complex ? 1 : 0;
+  return batch_prepare(G.data(), probs, nprob, is_complex, layout);
+}
+
+extern "C" int tnl_batch_build(const tnl_batch_problem* probs,
+                               const tnl_batch_layout* layout, void* ws,
+                               void* stream) {
+  if (!probs || !layout || !ws) return fail(TNL_EINVAL, "null pointer");
+  std::vector<InterpGeom> G((size_t)layout->nprob);
+  if (int rc = batch_geoms(probs, layout->nprob, layout->ndim, layout->kernel,
+                           layout->norm, layout->pad_mode, layout->is_complex, G.data()))
+    return rc;
+  return batch_build(G.data(), probs, *layout, ws, (hipStream_t)stream);
+}
+
+extern "C" int tnl_ungrid_batched(const float* grid_vals, float* pts_out,
+                                  const tnl_batch_layout* layout,
+                                  const void* ws, void* stream) {
+  if (!grid_vals || !pts_out || !layout || !ws) return fail(TNL_EINVAL, "null pointer");
+  return batch_run(false, *layout, ws, grid_vals, pts_out, (hipStream_t)stream);
+}
+
+extern "C" int tnl_grid_batched(const float* pts_vals, float* grid_out,
+                                const tnl_batch_layout* layout, const void* ws,
+                                void* stream) {
+  if (!pts_vals || !grid_out || !layout || !ws) return fail(TNL_EINVAL, "null pointer");
+  return batch_run(true, *layout, ws, pts_vals, grid_out, (hipStream_t)stream);
 }

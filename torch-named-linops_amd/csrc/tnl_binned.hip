@@ -16,6 +16,8 @@
 //   a region zeroed by a small pre-pass.
 // Workgroup ids are remapped so each XCD walks a contiguous range of tiles
 // (neighbouring halos then share that XCD's L2).
+#include <cstdlib>
+
 #include "tnl_common.hpp"
 #include "tnl_internal.hpp"
 
@@ -59,21 +61,18 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nwg) {
   return (bid % NUM_XCD) * per + bid / NUM_XCD;
 }
 
+// One subproblem (<= max_sub points of one bin) of one problem, channels
+// [c0, c0 + CC).  Called by all threads of the workgroup.
 template <int NDIM, bool CPLX, int CC, int NT>
-__global__ __launch_bounds__(BIN_THREADS) void ungrid_binned_kernel(
+__device__ __forceinline__ void ungrid_sub(
     const float* __restrict__ vals, float* __restrict__ out, int64_t nbatch,
-    InterpGeom G, BinGeom B, const int* __restrict__ perm,
-    const float* __restrict__ slocs, const int4* __restrict__ sub,
-    const int* __restrict__ nsub_ptr) {
+    const InterpGeom& G, const BinGeom& B, const int* __restrict__ perm,
+    const float* __restrict__ slocs, int4 sp, int64_t c0) {
   using C = Ch<CPLX>;
   using T = typename C::T;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* lds = reinterpret_cast<T*>(lds_raw);
-  const int64_t s = xcd_remap(blockIdx.x, gridDim.x);
-  if (s >= *nsub_ptr) return;
-  const int4 sp = sub[s];
   const int bin = sp.x, start = sp.y, count = sp.z;
-  const int64_t c0 = (int64_t)blockIdx.y * CC;
   int nc = (int)(nbatch - c0);
   nc = nc < CC ? nc : CC;
   if (bin == B.nbins) {  // points without taps: zero output
@@ -211,8 +210,50 @@ __global__ __launch_bounds__(BIN_THREADS) void ungrid_binned_kernel(
   }
 }
 
+template <int NDIM, bool CPLX, int CC, int NT>
+__global__ __launch_bounds__(BIN_THREADS) void ungrid_binned_kernel(
+    const float* __restrict__ vals, float* __restrict__ out, int64_t nbatch,
+    InterpGeom G, BinGeom B, const int* __restrict__ perm,
+    const float* __restrict__ slocs, const int4* __restrict__ sub,
+    const int* __restrict__ nsub_ptr) {
+  const int64_t s = xcd_remap(blockIdx.x, gridDim.x);
+  if (s >= *nsub_ptr) return;
+  ungrid_sub<NDIM, CPLX, CC, NT>(vals, out, nbatch, G, B, perm, slocs, sub[s],
+                                 (int64_t)blockIdx.y * CC);
+}
+
 // Pre-pass for gridding: zero the output region of every split (shared)
-// tile, once per tile (by its first item).
+// tile, once per tile (by its first item).  Called by one workgroup per item.
+template <bool CPLX>
+__device__ __forceinline__ void zero_shared_item(float* __restrict__ out,
+                                                 int64_t nbatch,
+                                                 const InterpGeom& G,
+                                                 const BinGeom& B, int4 item,
+                                                 const int* __restrict__ gstarts) {
+  if (!item.w || item.y != gstarts[item.x]) return;
+  int origin[3];
+  int r = item.x;
+  for (int d = 2; d >= 0; --d) {
+    origin[d] = (r % B.ntiles[d]) * B.tile[d];
+    r /= B.ntiles[d];
+  }
+  const int comp = CPLX ? 2 : 1;
+  const int t12 = B.tile[1] * B.tile[2];
+  const int64_t total = (int64_t)nbatch * B.tcells * comp;
+  for (int64_t idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    const int64_t c = idx / (B.tcells * comp);
+    int e = (int)(idx - c * B.tcells * comp);
+    const int part = e % comp;
+    const int cell = e / comp;
+    const int lx = cell / t12, rem = cell - lx * t12;
+    const int ly = rem / B.tile[2], lz = rem - ly * B.tile[2];
+    const int64_t gx = origin[0] + lx, gy = origin[1] + ly,
+                  gz = origin[2] + lz;
+    if (gx >= G.n[0] || gy >= G.n[1] || gz >= G.n[2]) continue;
+    out[(c * G.ncells + (gx * G.n[1] + gy) * G.n[2] + gz) * comp + part] = 0.0f;
+  }
+}
+
 template <bool CPLX>
 __global__ void grid_zero_shared_kernel(float* __restrict__ out,
                                         int64_t nbatch, InterpGeom G,
@@ -221,32 +262,8 @@ __global__ void grid_zero_shared_kernel(float* __restrict__ out,
                                         const int* __restrict__ nitems_ptr,
                                         const int* __restrict__ gstarts) {
   const int nitems = *nitems_ptr;
-  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const int4 item = items[it];
-    if (!item.w || item.y != gstarts[item.x]) continue;
-    int origin[3];
-    int r = item.x;
-    for (int d = 2; d >= 0; --d) {
-      origin[d] = (r % B.ntiles[d]) * B.tile[d];
-      r /= B.ntiles[d];
-    }
-    const int comp = CPLX ? 2 : 1;
-    const int t12 = B.tile[1] * B.tile[2];
-    const int64_t total = (int64_t)nbatch * B.tcells * comp;
-    for (int64_t idx = threadIdx.x; idx < total; idx += blockDim.x) {
-      const int64_t c = idx / (B.tcells * comp);
-      int e = (int)(idx - c * B.tcells * comp);
-      const int part = e % comp;
-      const int cell = e / comp;
-      const int lx = cell / t12, rem = cell - lx * t12;
-      const int ly = rem / B.tile[2], lz = rem - ly * B.tile[2];
-      const int64_t gx = origin[0] + lx, gy = origin[1] + ly,
-                    gz = origin[2] + lz;
-      if (gx >= G.n[0] || gy >= G.n[1] || gz >= G.n[2]) continue;
-      out[(c * G.ncells + (gx * G.n[1] + gy) * G.n[2] + gz) * comp + part] =
-          0.0f;
-    }
-  }
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x)
+    zero_shared_item<CPLX>(out, nbatch, G, B, items[it], gstarts);
 }
 
 // Gridding, "lanes = taps".  A workgroup of NW waves owns one tile
@@ -311,12 +328,42 @@ template <> struct VecF<1> { using T = float; };
 template <> struct VecF<2> { using T = float2; };
 template <> struct VecF<4> { using T = float4; };
 
+// This lane's tap in the main block (3-D: 4x4x4, 2-D: 8x8) and, in 3-D,
+// its tap of the second pass over the 61 taps with an index 4.
+struct GridLanes {
+  int mk[3], sk[3];
+  bool shell_ok;
+};
+
+template <int NDIM>
+__device__ __forceinline__ GridLanes make_grid_lanes(int lane) {
+  GridLanes g;
+  g.shell_ok = false;
+  g.sk[0] = g.sk[1] = g.sk[2] = 0;
+  if (NDIM >= 3) {
+    g.mk[0] = lane >> 4; g.mk[1] = (lane >> 2) & 3; g.mk[2] = lane & 3;
+    int seen = 0;
+    for (int a = 0; a < 5; ++a)
+      for (int b2 = 0; b2 < 5; ++b2)
+        for (int c2 = 0; c2 < 5; ++c2)
+          if (a == 4 || b2 == 4 || c2 == 4) {
+            if (seen == lane) { g.sk[0] = a; g.sk[1] = b2; g.sk[2] = c2; g.shell_ok = true; }
+            ++seen;
+          }
+  } else {
+    g.mk[0] = 0; g.mk[1] = lane >> 3; g.mk[2] = lane & 7;
+  }
+  return g;
+}
+
+// One work item (a tile and the touches listed for it) of one problem,
+// every channel chunk.  Called by all threads of the workgroup.
 template <int NDIM, bool CPLX, int CC, int NT, int NW>
-__global__ __launch_bounds__(GW * NW) void grid_binned_kernel(
-    const float* __restrict__ vals, float* __restrict__ out, int64_t nbatch,
-    InterpGeom G, BinGeom B, const int* __restrict__ perm,
-    const float* __restrict__ slocs, const int* __restrict__ tlist,
-    const int4* __restrict__ items, const int* __restrict__ nitems_ptr) {
+__device__ __forceinline__ void grid_item(
+    const GridLanes& ln, const float* __restrict__ vals, float* __restrict__ out,
+    int64_t nbatch, const InterpGeom& G, const BinGeom& B,
+    const int* __restrict__ perm, const float* __restrict__ slocs,
+    const int* __restrict__ tlist, int4 item) {
   using C = Ch<CPLX>;
   using T = typename C::T;
   constexpr int COMP = CPLX ? 2 : 1;
@@ -343,317 +390,309 @@ __global__ __launch_bounds__(GW * NW) void grid_binned_kernel(
   const int tid = threadIdx.x;
   const int lane = tid & (GW - 1);
   const int wv = __builtin_amdgcn_readfirstlane(tid / GW);
-  const int nitems = *nitems_ptr;
   const T* __restrict__ v = reinterpret_cast<const T*>(vals);
   const int dstride[3] = {B.gxs, B.gys, 1};
-
-  // This lane's tap in the main block and its cell offset from a point's
-  // tap (0,0,0); the 3-D shell tap (an index 4) of the second pass.
-  int mk[3], sk[3];
-  bool shell_ok = false;
-  if (NDIM >= 3) {
-    mk[0] = lane >> 4; mk[1] = (lane >> 2) & 3; mk[2] = lane & 3;
-    sk[0] = sk[1] = sk[2] = 0;
-    int seen = 0;
-    for (int a = 0; a < 5; ++a)
-      for (int b2 = 0; b2 < 5; ++b2)
-        for (int c2 = 0; c2 < 5; ++c2)
-          if (a == 4 || b2 == 4 || c2 == 4) {
-            if (seen == lane) { sk[0] = a; sk[1] = b2; sk[2] = c2; shell_ok = true; }
-            ++seen;
-          }
-  } else {
-    mk[0] = 0; mk[1] = lane >> 3; mk[2] = lane & 7;
-    sk[0] = sk[1] = sk[2] = 0;
-  }
+  const int mk[3] = {ln.mk[0], ln.mk[1], ln.mk[2]};
+  const int sk[3] = {ln.sk[0], ln.sk[1], ln.sk[2]};
+  const bool shell_ok = ln.shell_ok;
   const int mkoff = mk[0] * dstride[0] + mk[1] * dstride[1] + mk[2];
   float* __restrict__ accw = acc + wv * PPW * pcells;  // this wave's planes
   const float* __restrict__ w64l = s_w64 + lane * W64S;
 
-  for (int64_t it = xcd_remap(blockIdx.x, gridDim.x); it < nitems;
-       it += gridDim.x) {
-    const int4 item = items[it];
-    const int tile = item.x, start = item.y, count = item.z;
-    const bool shared = item.w != 0;
-    int origin[3];
-    {
-      int r = tile;
-      for (int d = 2; d >= 0; --d) {
-        origin[d] = (r % B.ntiles[d]) * B.tile[d];
-        r /= B.ntiles[d];
-      }
+  const int tile = item.x, start = item.y, count = item.z;
+  const bool shared = item.w != 0;
+  int origin[3];
+  {
+    int r = tile;
+    for (int d = 2; d >= 0; --d) {
+      origin[d] = (r % B.ntiles[d]) * B.tile[d];
+      r /= B.ntiles[d];
     }
-    for (int64_t c0 = 0; c0 < nbatch; c0 += CC) {
-      const int nc = (int)(nbatch - c0 < CC ? nbatch - c0 : CC);
-      // raw inputs of one batch, per role: a coordinate or the values
-      float pt[NR];
-      T pv[CC];
-      auto load = [&](int b0, int nb) {
-        if (lane < nb) {
-          const int q = tlist[start + b0 + lane];
+  }
+  for (int64_t c0 = 0; c0 < nbatch; c0 += CC) {
+    const int nc = (int)(nbatch - c0 < CC ? nbatch - c0 : CC);
+    // raw inputs of one batch, per role: a coordinate or the values
+    float pt[NR];
+    T pv[CC];
+    auto load = [&](int b0, int nb) {
+      if (lane < nb) {
+        const int q = tlist[start + b0 + lane];
 #pragma unroll
-          for (int sr = 0; sr < NR; ++sr) {
-            const int role = wv + sr * NW;
-            if (role < 3) {
-              pt[sr] = role >= 3 - NDIM ? slocs[(int64_t)q * NDIM + role - (3 - NDIM)] : 0.f;
-            } else {
-              const int p = perm[q];
+        for (int sr = 0; sr < NR; ++sr) {
+          const int role = wv + sr * NW;
+          if (role < 3) {
+            pt[sr] = role >= 3 - NDIM ? slocs[(int64_t)q * NDIM + role - (3 - NDIM)] : 0.f;
+          } else {
+            const int p = perm[q];
 #pragma unroll
-              for (int c = 0; c < CC; ++c)
-                pv[c] = c < nc ? v[(c0 + c) * B.npts + p] : C::zero();
-            }
+            for (int c = 0; c < CC; ++c)
+              pv[c] = c < nc ? v[(c0 + c) * B.npts + p] : C::zero();
           }
         }
-      };
-      {
-        float4* a4 = reinterpret_cast<float4*>(acc);
-        const int n4 = NP * pcells / 4;
-        for (int i = tid; i < n4; i += NTH) a4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int i = n4 * 4 + tid; i < NP * pcells; i += NTH) acc[i] = 0.f;
       }
-      load(0, count < PB ? count : PB);
-      for (int b0 = 0; b0 < count; b0 += PB) {
-        const int nb = count - b0 < PB ? count - b0 : PB;
-        __syncthreads();
-        // ---- phase 1, lanes = points: per-axis records and values
-        if (lane < nb) {
+    };
+    {
+      float4* a4 = reinterpret_cast<float4*>(acc);
+      const int n4 = NP * pcells / 4;
+      for (int i = tid; i < n4; i += NTH) a4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int i = n4 * 4 + tid; i < NP * pcells; i += NTH) acc[i] = 0.f;
+    }
+    load(0, count < PB ? count : PB);
+    for (int b0 = 0; b0 < count; b0 += PB) {
+      const int nb = count - b0 < PB ? count - b0 : PB;
+      __syncthreads();
+      // ---- phase 1, lanes = points: per-axis records and values
+      if (lane < nb) {
 #pragma unroll
-          for (int sr = 0; sr < NR; ++sr) {
-            const int role = wv + sr * NW;
-            if (role < 3) {
-              const int d = role;
-              int lo = 0, cnt = 1;
-              float t = 0.f;
-              int64_t g0 = 0;
-              if (d >= 3 - NDIM) {
-                t = pt[sr];
-                const TapRange r = tap_range(t, G.half[d], G.ntap[d], G.n[d], G.pad);
-                const int64_t gw =
-                    G.pad == TNL_PAD_CIRCULAR ? mod_pos(r.g0, G.n[d]) : r.g0;
-                lo = (int)(gw - origin[d]);
-                cnt = r.count;
-                g0 = r.g0;
-              }
-              // in-tile mask over taps k < cnt, and whether a tap wraps
-              // past the end of the grid into this tile
-              int m = 0, wrap = 0;
+        for (int sr = 0; sr < NR; ++sr) {
+          const int role = wv + sr * NW;
+          if (role < 3) {
+            const int d = role;
+            int lo = 0, cnt = 1;
+            float t = 0.f;
+            int64_t g0 = 0;
+            if (d >= 3 - NDIM) {
+              t = pt[sr];
+              const TapRange r = tap_range(t, G.half[d], G.ntap[d], G.n[d], G.pad);
+              const int64_t gw =
+                  G.pad == TNL_PAD_CIRCULAR ? mod_pos(r.g0, G.n[d]) : r.g0;
+              lo = (int)(gw - origin[d]);
+              cnt = r.count;
+              g0 = r.g0;
+            }
+            // in-tile mask over taps k < cnt, and whether a tap wraps
+            // past the end of the grid into this tile
+            int m = 0, wrap = 0;
 #pragma unroll
-              for (int k = 0; k < NT; ++k) {
-                if (k < cnt) {
-                  int l = lo + k;
-                  if (G.pad == TNL_PAD_CIRCULAR && origin[d] + l >= G.n[d]) {
-                    l -= (int)G.n[d];
-                    wrap = 1;
-                  }
-                  if (l >= 0 && l < B.tile[d]) m |= 1 << k;
+            for (int k = 0; k < NT; ++k) {
+              if (k < cnt) {
+                int l = lo + k;
+                if (G.pad == TNL_PAD_CIRCULAR && origin[d] + l >= G.n[d]) {
+                  l -= (int)G.n[d];
+                  wrap = 1;
                 }
+                if (l >= 0 && l < B.tile[d]) m |= 1 << k;
               }
-              s_dim[d * PB + lane] =
-                  make_int2(lo * dstride[d], m | (wrap << 8) | (int)((unsigned)lo << 16));
+            }
+            s_dim[d * PB + lane] =
+                make_int2(lo * dstride[d], m | (wrap << 8) | (int)((unsigned)lo << 16));
 #pragma unroll
-              for (int k = 0; k < NT; ++k) {
-                float w = (d < 3 - NDIM && k == 0) ? 1.0f : 0.0f;
-                if (d >= 3 - NDIM && k < cnt)
-                  w = window(G.kernel,
-                             fminf(fabsf((float)(g0 + k) - t) * G.inv_half[d], 1.0f),
-                             G.beta);
-                s_w1[lane * w1s + d * NT + k] = w;
+            for (int k = 0; k < NT; ++k) {
+              float w = (d < 3 - NDIM && k == 0) ? 1.0f : 0.0f;
+              if (d >= 3 - NDIM && k < cnt)
+                w = window(G.kernel,
+                           fminf(fabsf((float)(g0 + k) - t) * G.inv_half[d], 1.0f),
+                           G.beta);
+              s_w1[lane * w1s + d * NT + k] = w;
+            }
+          } else {
+            V* dst = reinterpret_cast<V*>(s_v + lane * NP);
+            float tmp[NP];
+#pragma unroll
+            for (int c = 0; c < CC; ++c) {
+              if constexpr (CPLX) {
+                tmp[2 * c] = pv[c].x;
+                tmp[2 * c + 1] = pv[c].y;
+              } else {
+                tmp[c] = pv[c];
+              }
+            }
+#pragma unroll
+            for (int i = 0; i < NP / VW; ++i) {
+              V x;
+              if constexpr (VW == 4) x = make_float4(tmp[4 * i], tmp[4 * i + 1], tmp[4 * i + 2], tmp[4 * i + 3]);
+              else if constexpr (VW == 2) x = make_float2(tmp[2 * i], tmp[2 * i + 1]);
+              else x = tmp[i];
+              dst[i] = x;
+            }
+          }
+        }
+      }
+      if (b0 + PB < count) load(b0 + PB, count - b0 - PB < PB ? count - b0 - PB : PB);
+      __syncthreads();
+      // ---- phase 2: tap weights (tap-major) and point records
+      {
+        const int j = tid % PB;
+        if (j < nb) {
+          const float* w1 = s_w1 + j * w1s;
+          // 8 groups of 8 taps: 3-D group g = (k0 = g / 2, k1 in 2(g % 2) + {0, 1},
+          // k2 in 0..3); 2-D group g = (k1 = g, k2 in 0..7)
+          for (int g = tid / PB; g < 8; g += NTH / PB) {
+            float* dst = s_w64 + (g * 8) * W64S + j;
+            if (NDIM >= 3) {
+              const float wx = w1[g >> 1];
+              const float wxy0 = wx * w1[NT + 2 * (g & 1)];
+              const float wxy1 = wx * w1[NT + 2 * (g & 1) + 1];
+#pragma unroll
+              for (int k2 = 0; k2 < 4; ++k2) {
+                const float wz = w1[2 * NT + k2];
+                dst[k2 * W64S] = wxy0 * wz;
+                dst[(4 + k2) * W64S] = wxy1 * wz;
               }
             } else {
-              V* dst = reinterpret_cast<V*>(s_v + lane * NP);
-              float tmp[NP];
+              const float wy = w1[NT + g];
 #pragma unroll
-              for (int c = 0; c < CC; ++c) {
-                if constexpr (CPLX) {
-                  tmp[2 * c] = pv[c].x;
-                  tmp[2 * c + 1] = pv[c].y;
-                } else {
-                  tmp[c] = pv[c];
-                }
-              }
-#pragma unroll
-              for (int i = 0; i < NP / VW; ++i) {
-                V x;
-                if constexpr (VW == 4) x = make_float4(tmp[4 * i], tmp[4 * i + 1], tmp[4 * i + 2], tmp[4 * i + 3]);
-                else if constexpr (VW == 2) x = make_float2(tmp[2 * i], tmp[2 * i + 1]);
-                else x = tmp[i];
-                dst[i] = x;
-              }
+              for (int k2 = 0; k2 < 8; ++k2)
+                dst[k2 * W64S] = k2 < NT ? wy * w1[2 * NT + k2] : 0.0f;
             }
           }
-        }
-        if (b0 + PB < count) load(b0 + PB, count - b0 - PB < PB ? count - b0 - PB : PB);
-        __syncthreads();
-        // ---- phase 2: tap weights (tap-major) and point records
-        {
-          const int j = tid % PB;
-          if (j < nb) {
-            const float* w1 = s_w1 + j * w1s;
-            // 8 groups of 8 taps: 3-D group g = (k0 = g / 2, k1 in 2(g % 2) + {0, 1},
-            // k2 in 0..3); 2-D group g = (k1 = g, k2 in 0..7)
-            for (int g = tid / PB; g < 8; g += NTH / PB) {
-              float* dst = s_w64 + (g * 8) * W64S + j;
-              if (NDIM >= 3) {
-                const float wx = w1[g >> 1];
-                const float wxy0 = wx * w1[NT + 2 * (g & 1)];
-                const float wxy1 = wx * w1[NT + 2 * (g & 1) + 1];
+          if (tid < PB) {
+            const int2 a0 = s_dim[j], a1 = s_dim[PB + j], a2 = s_dim[2 * PB + j];
+            const unsigned m0 = a0.y & 255, m1 = a1.y & 255, m2 = a2.y & 255;
+            uint64_t m64 = 0;
+            if (NDIM >= 3) {
+              unsigned m12 = 0;
 #pragma unroll
-                for (int k2 = 0; k2 < 4; ++k2) {
-                  const float wz = w1[2 * NT + k2];
-                  dst[k2 * W64S] = wxy0 * wz;
-                  dst[(4 + k2) * W64S] = wxy1 * wz;
-                }
-              } else {
-                const float wy = w1[NT + g];
+              for (int k1 = 0; k1 < 4; ++k1)
+                if ((m1 >> k1) & 1) m12 |= (m2 & 15u) << (4 * k1);
 #pragma unroll
-                for (int k2 = 0; k2 < 8; ++k2)
-                  dst[k2 * W64S] = k2 < NT ? wy * w1[2 * NT + k2] : 0.0f;
-              }
+              for (int k0 = 0; k0 < 4; ++k0)
+                if ((m0 >> k0) & 1) m64 |= (uint64_t)m12 << (16 * k0);
+            } else {
+#pragma unroll
+              for (int k1 = 0; k1 < 8; ++k1)
+                if ((m1 >> k1) & 1) m64 |= (uint64_t)m2 << (8 * k1);
             }
-            if (tid < PB) {
-              const int2 a0 = s_dim[j], a1 = s_dim[PB + j], a2 = s_dim[2 * PB + j];
-              const unsigned m0 = a0.y & 255, m1 = a1.y & 255, m2 = a2.y & 255;
-              uint64_t m64 = 0;
-              if (NDIM >= 3) {
-                unsigned m12 = 0;
-#pragma unroll
-                for (int k1 = 0; k1 < 4; ++k1)
-                  if ((m1 >> k1) & 1) m12 |= (m2 & 15u) << (4 * k1);
-#pragma unroll
-                for (int k0 = 0; k0 < 4; ++k0)
-                  if ((m0 >> k0) & 1) m64 |= (uint64_t)m12 << (16 * k0);
-              } else {
-#pragma unroll
-                for (int k1 = 0; k1 < 8; ++k1)
-                  if ((m1 >> k1) & 1) m64 |= (uint64_t)m2 << (8 * k1);
-              }
-              const int mall = a0.y | a1.y | a2.y;
-              const int flags = (NDIM >= 3 && (mall & 16) ? GF_SHELL : 0) |
-                                (mall & 256 ? GF_WRAP : 0);
-              s_meta[j] = make_int4(a0.x + a1.x + a2.x, (int)(unsigned)m64,
-                                    (int)(unsigned)(m64 >> 32), flags);
-            }
+            const int mall = a0.y | a1.y | a2.y;
+            const int flags = (NDIM >= 3 && (mall & 16) ? GF_SHELL : 0) |
+                              (mall & 256 ? GF_WRAP : 0);
+            s_meta[j] = make_int4(a0.x + a1.x + a2.x, (int)(unsigned)m64,
+                                  (int)(unsigned)(m64 >> 32), flags);
           }
         }
-        __syncthreads();
-        // ---- phase 3, lanes = taps: this wave's planes, point by point.
-        // The next point's record, values and weight are loaded during this
-        // point's read-modify-write (one wave's LDS operations complete in
-        // order, so its reads see its earlier writes).
-        const int nbu = __builtin_amdgcn_readfirstlane(nb);
-        int4 n_meta = s_meta[0];
-        float n_w = w64l[0];
-        V n_v[PPW / VW];
-        {
-          const V* src = reinterpret_cast<const V*>(s_v + wv * PPW);
+      }
+      __syncthreads();
+      // ---- phase 3, lanes = taps: this wave's planes, point by point.
+      // The next point's record, values and weight are loaded during this
+      // point's read-modify-write (one wave's LDS operations complete in
+      // order, so its reads see its earlier writes).
+      const int nbu = __builtin_amdgcn_readfirstlane(nb);
+      int4 n_meta = s_meta[0];
+      float n_w = w64l[0];
+      V n_v[PPW / VW];
+      {
+        const V* src = reinterpret_cast<const V*>(s_v + wv * PPW);
+#pragma unroll
+        for (int i = 0; i < PPW / VW; ++i) n_v[i] = src[i];
+      }
+      for (int j = 0; j < nbu; ++j) {
+        const int4 meta = n_meta;
+        const float w = n_w;
+        float vv[PPW];
+#pragma unroll
+        for (int i = 0; i < PPW / VW; ++i) {
+          const V x = n_v[i];
+          if constexpr (VW == 4) {
+            vv[4 * i] = x.x; vv[4 * i + 1] = x.y; vv[4 * i + 2] = x.z; vv[4 * i + 3] = x.w;
+          } else if constexpr (VW == 2) {
+            vv[2 * i] = x.x; vv[2 * i + 1] = x.y;
+          } else {
+            vv[i] = x;
+          }
+        }
+        if (j + 1 < nbu) {
+          n_meta = s_meta[j + 1];
+          n_w = w64l[j + 1];
+          const V* src = reinterpret_cast<const V*>(s_v + (j + 1) * NP + wv * PPW);
 #pragma unroll
           for (int i = 0; i < PPW / VW; ++i) n_v[i] = src[i];
         }
-        for (int j = 0; j < nbu; ++j) {
-          const int4 meta = n_meta;
-          const float w = n_w;
-          float vv[PPW];
+        const unsigned mlo = __builtin_amdgcn_readfirstlane(meta.y);
+        const unsigned mhi = __builtin_amdgcn_readfirstlane(meta.z);
+        const int flags = __builtin_amdgcn_readfirstlane(meta.w);
+        const uint64_t m64 = ((uint64_t)mhi << 32) | mlo;
+        if ((m64 >> lane) & 1) {
+          int cell = meta.x + mkoff;
+          if (flags & GF_WRAP) {  // footprint wraps around the grid in this tile
+            cell = 0;
 #pragma unroll
-          for (int i = 0; i < PPW / VW; ++i) {
-            const V x = n_v[i];
-            if constexpr (VW == 4) {
-              vv[4 * i] = x.x; vv[4 * i + 1] = x.y; vv[4 * i + 2] = x.z; vv[4 * i + 3] = x.w;
-            } else if constexpr (VW == 2) {
-              vv[2 * i] = x.x; vv[2 * i + 1] = x.y;
-            } else {
-              vv[i] = x;
+            for (int d = 0; d < 3; ++d) {
+              int l = (s_dim[d * PB + j].y >> 16) + mk[d];
+              if (origin[d] + l >= G.n[d]) l -= (int)G.n[d];
+              cell += l * dstride[d];
             }
           }
-          if (j + 1 < nbu) {
-            n_meta = s_meta[j + 1];
-            n_w = w64l[j + 1];
-            const V* src = reinterpret_cast<const V*>(s_v + (j + 1) * NP + wv * PPW);
+          float* cp = accw + cell;
+          float old[PPW];
 #pragma unroll
-            for (int i = 0; i < PPW / VW; ++i) n_v[i] = src[i];
-          }
-          const unsigned mlo = __builtin_amdgcn_readfirstlane(meta.y);
-          const unsigned mhi = __builtin_amdgcn_readfirstlane(meta.z);
-          const int flags = __builtin_amdgcn_readfirstlane(meta.w);
-          const uint64_t m64 = ((uint64_t)mhi << 32) | mlo;
-          if ((m64 >> lane) & 1) {
-            int cell = meta.x + mkoff;
-            if (flags & GF_WRAP) {  // footprint wraps around the grid in this tile
-              cell = 0;
+          for (int c = 0; c < PPW; ++c) old[c] = cp[c * pcells];
 #pragma unroll
-              for (int d = 0; d < 3; ++d) {
-                int l = (s_dim[d * PB + j].y >> 16) + mk[d];
-                if (origin[d] + l >= G.n[d]) l -= (int)G.n[d];
-                cell += l * dstride[d];
-              }
+          for (int c = 0; c < PPW; ++c) cp[c * pcells] = fmaf(w, vv[c], old[c]);
+        }
+        if (NDIM >= 3 && (flags & GF_SHELL)) {
+          // second pass: taps with some index 4 (points on integer coordinates)
+          const int2 a0 = s_dim[j], a1 = s_dim[PB + j], a2 = s_dim[2 * PB + j];
+          const bool ok = shell_ok && ((a0.y >> sk[0]) & (a1.y >> sk[1]) & (a2.y >> sk[2]) & 1);
+          if (ok) {
+            const int lo3[3] = {a0.y >> 16, a1.y >> 16, a2.y >> 16};
+            int cell = 0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              int l = lo3[d] + sk[d];
+              if (G.pad == TNL_PAD_CIRCULAR && origin[d] + l >= G.n[d]) l -= (int)G.n[d];
+              cell += l * dstride[d];
             }
+            const float* w1 = s_w1 + __builtin_amdgcn_readfirstlane(j * w1s);
+            const float ws3 = w1[sk[0]] * w1[NT + sk[1]] * w1[2 * NT + sk[2]];
             float* cp = accw + cell;
             float old[PPW];
 #pragma unroll
             for (int c = 0; c < PPW; ++c) old[c] = cp[c * pcells];
 #pragma unroll
-            for (int c = 0; c < PPW; ++c) cp[c * pcells] = fmaf(w, vv[c], old[c]);
-          }
-          if (NDIM >= 3 && (flags & GF_SHELL)) {
-            // second pass: taps with some index 4 (points on integer coordinates)
-            const int2 a0 = s_dim[j], a1 = s_dim[PB + j], a2 = s_dim[2 * PB + j];
-            const bool ok = shell_ok && ((a0.y >> sk[0]) & (a1.y >> sk[1]) & (a2.y >> sk[2]) & 1);
-            if (ok) {
-              const int lo3[3] = {a0.y >> 16, a1.y >> 16, a2.y >> 16};
-              int cell = 0;
-#pragma unroll
-              for (int d = 0; d < 3; ++d) {
-                int l = lo3[d] + sk[d];
-                if (G.pad == TNL_PAD_CIRCULAR && origin[d] + l >= G.n[d]) l -= (int)G.n[d];
-                cell += l * dstride[d];
-              }
-              const float* w1 = s_w1 + __builtin_amdgcn_readfirstlane(j * w1s);
-              const float ws3 = w1[sk[0]] * w1[NT + sk[1]] * w1[2 * NT + sk[2]];
-              float* cp = accw + cell;
-              float old[PPW];
-#pragma unroll
-              for (int c = 0; c < PPW; ++c) old[c] = cp[c * pcells];
-#pragma unroll
-              for (int c = 0; c < PPW; ++c) cp[c * pcells] = fmaf(ws3, vv[c], old[c]);
-            }
+            for (int c = 0; c < PPW; ++c) cp[c * pcells] = fmaf(ws3, vv[c], old[c]);
           }
         }
       }
-      __syncthreads();
-      // ---- write the tile: coalesced along z, one channel at a time
-      T* __restrict__ o = reinterpret_cast<T*>(out);
-      const int t12 = B.tile[1] * B.tile[2];
-      for (int cell = tid; cell < B.tcells; cell += NTH) {
-        const int lx = cell / t12, rem = cell - lx * t12;
-        const int ly = rem / B.tile[2], lz = rem - ly * B.tile[2];
-        const int64_t gx = origin[0] + lx, gy = origin[1] + ly,
-                      gz = origin[2] + lz;
-        if (gx >= G.n[0] || gy >= G.n[1] || gz >= G.n[2]) continue;
-        const int64_t g0 = (gx * G.n[1] + gy) * G.n[2] + gz;
-        const int pc = lx * dstride[0] + ly * dstride[1] + lz;
-#pragma unroll
-        for (int c = 0; c < CC; ++c) {
-          if (c >= nc) break;
-          const int64_t gi = (c0 + c) * G.ncells + g0;
-          const float a0 = acc[(c * COMP) * pcells + pc];
-          const float a1 = CPLX ? acc[(c * COMP + 1) * pcells + pc] : 0.0f;
-          if (!shared) {
-            if constexpr (CPLX)
-              o[gi] = make_float2(a0, a1);
-            else
-              o[gi] = a0;
-          } else {
-            if constexpr (CPLX) {
-              unsafeAtomicAdd(out + 2 * gi, a0);
-              unsafeAtomicAdd(out + 2 * gi + 1, a1);
-            } else {
-              unsafeAtomicAdd(out + gi, a0);
-            }
-          }
-        }
-      }
-      __syncthreads();
     }
+    __syncthreads();
+    // ---- write the tile: coalesced along z, one channel at a time
+    T* __restrict__ o = reinterpret_cast<T*>(out);
+    const int t12 = B.tile[1] * B.tile[2];
+    for (int cell = tid; cell < B.tcells; cell += NTH) {
+      const int lx = cell / t12, rem = cell - lx * t12;
+      const int ly = rem / B.tile[2], lz = rem - ly * B.tile[2];
+      const int64_t gx = origin[0] + lx, gy = origin[1] + ly,
+                    gz = origin[2] + lz;
+      if (gx >= G.n[0] || gy >= G.n[1] || gz >= G.n[2]) continue;
+      const int64_t g0 = (gx * G.n[1] + gy) * G.n[2] + gz;
+      const int pc = lx * dstride[0] + ly * dstride[1] + lz;
+#pragma unroll
+      for (int c = 0; c < CC; ++c) {
+        if (c >= nc) break;
+        const int64_t gi = (c0 + c) * G.ncells + g0;
+        const float a0 = acc[(c * COMP) * pcells + pc];
+        const float a1 = CPLX ? acc[(c * COMP + 1) * pcells + pc] : 0.0f;
+        if (!shared) {
+          if constexpr (CPLX)
+            o[gi] = make_float2(a0, a1);
+          else
+            o[gi] = a0;
+        } else {
+          if constexpr (CPLX) {
+            unsafeAtomicAdd(out + 2 * gi, a0);
+            unsafeAtomicAdd(out + 2 * gi + 1, a1);
+          } else {
+            unsafeAtomicAdd(out + gi, a0);
+          }
+        }
+      }
+    }
+    __syncthreads();
   }
+}
+
+template <int NDIM, bool CPLX, int CC, int NT, int NW>
+__global__ __launch_bounds__(GW * NW) void grid_binned_kernel(
+    const float* __restrict__ vals, float* __restrict__ out, int64_t nbatch,
+    InterpGeom G, BinGeom B, const int* __restrict__ perm,
+    const float* __restrict__ slocs, const int* __restrict__ tlist,
+    const int4* __restrict__ items, const int* __restrict__ nitems_ptr) {
+  const GridLanes ln = make_grid_lanes<NDIM>(threadIdx.x & (GW - 1));
+  const int nitems = *nitems_ptr;
+  for (int64_t it = xcd_remap(blockIdx.x, gridDim.x); it < nitems;
+       it += gridDim.x)
+    grid_item<NDIM, CPLX, CC, NT, NW>(ln, vals, out, nbatch, G, B, perm, slocs,
+                                      tlist, items[it]);
 }
 
 // Bank cost of one gridding RMW instruction: lane l touches the cell at
@@ -865,6 +904,263 @@ int interp_binned(bool adjoint, const float* vals, float* out, int64_t nbatch,
                       : dispatch_nt<3, false>(adjoint, vals, out, nbatch, G, L, ws, s);
   return is_complex ? dispatch_nt<2, true>(adjoint, vals, out, nbatch, G, L, ws, s)
                     : dispatch_nt<2, false>(adjoint, vals, out, nbatch, G, L, ws, s);
+}
+
+// ---------------------------------------------------------------- batches
+// Device table entry of one problem of a batch.  Global work indices are
+// the concatenation of the problems' subproblem (ungrid) and work-item
+// (grid) capacities; unused capacity is skipped.
+struct BatchDesc {
+  InterpGeom G;
+  BinGeom Bu, Bg;
+  int64_t nbatch;
+  int64_t grid_off, pts_off;  // float offsets into the base buffers
+  int64_t sub_begin, item_begin;
+  const int* perm;
+  const float* slocs;
+  const int4* sub;
+  const int* nsub;
+  const int* tlist;
+  const int4* items;
+  const int* nitems;
+  const int* gstarts;
+};
+
+// Last problem whose range starts at or before g (g is wave-uniform).
+__device__ __forceinline__ int find_problem(const BatchDesc* __restrict__ D,
+                                            int nprob, int64_t g, bool items) {
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int64_t b = items ? D[mid].item_begin : D[mid].sub_begin;
+    if (b <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int NDIM, bool CPLX, int CC, int NT>
+__global__ __launch_bounds__(BIN_THREADS) void ungrid_batched_kernel(
+    const BatchDesc* __restrict__ D, int nprob, int64_t total,
+    const float* __restrict__ gbase, float* __restrict__ pbase) {
+  const int64_t s = xcd_remap(blockIdx.x, gridDim.x);
+  if (s >= total) return;
+  const BatchDesc& d = D[find_problem(D, nprob, s, false)];
+  const int64_t sl = s - d.sub_begin;
+  const int64_t c0 = (int64_t)blockIdx.y * CC;
+  if (sl >= *d.nsub || c0 >= d.nbatch) return;
+  ungrid_sub<NDIM, CPLX, CC, NT>(gbase + d.grid_off, pbase + d.pts_off, d.nbatch,
+                                 d.G, d.Bu, d.perm, d.slocs, d.sub[sl], c0);
+}
+
+template <bool CPLX>
+__global__ void grid_zero_shared_batched_kernel(const BatchDesc* __restrict__ D,
+                                                int nprob, int64_t total,
+                                                float* __restrict__ gbase) {
+  for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {
+    const BatchDesc& d = D[find_problem(D, nprob, g, true)];
+    const int64_t it = g - d.item_begin;
+    if (it >= *d.nitems) continue;
+    zero_shared_item<CPLX>(gbase + d.grid_off, d.nbatch, d.G, d.Bg, d.items[it],
+                           d.gstarts);
+  }
+}
+
+template <int NDIM, bool CPLX, int CC, int NT, int NW>
+__global__ __launch_bounds__(GW * NW) void grid_batched_kernel(
+    const BatchDesc* __restrict__ D, int nprob, int64_t total,
+    const float* __restrict__ pbase, float* __restrict__ gbase) {
+  const GridLanes ln = make_grid_lanes<NDIM>(threadIdx.x & (GW - 1));
+  for (int64_t g = xcd_remap(blockIdx.x, gridDim.x); g < total; g += gridDim.x) {
+    const BatchDesc& d = D[find_problem(D, nprob, g, true)];
+    const int64_t it = g - d.item_begin;
+    if (it >= *d.nitems) continue;
+    grid_item<NDIM, CPLX, CC, NT, NW>(ln, pbase + d.pts_off, gbase + d.grid_off,
+                                      d.nbatch, d.G, d.Bg, d.perm, d.slocs,
+                                      d.tlist, d.items[it]);
+  }
+}
+
+
+int batch_prepare(const InterpGeom* G, const tnl_batch_problem* P, int nprob,
+                  int is_complex, tnl_batch_layout* L) {
+  const int comp_bytes = is_complex ? 8 : 4;
+  const int comp = is_complex ? 2 : 1;
+  int nt = 1;
+  int64_t budget = INT64_MAX;
+  L->total_sub = L->total_items = 0;
+  for (int i = 0; i < nprob; ++i) {
+    const tnl_plan_layout& pl = *P[i].layout;
+    // tiles may differ per problem (plans clamp them to each grid); the
+    // launches take the largest LDS need
+    if (pl.ndim != P[0].layout->ndim || pl.pad_mode != P[0].layout->pad_mode)
+      return fail(TNL_EINVAL, "every plan of a batch must have the same ndim and pad_mode");
+    if (!binned_supported(G[i], pl, false) || !binned_supported(G[i], pl, true))
+      return fail(TNL_EINVAL,
+                  "batched problems must fit the plan-driven kernels (2-D/3-D, norm 1, "
+                  "taps per axis within the limits)");
+    for (int d = 0; d < 3; ++d) nt = G[i].ntap[d] > nt ? G[i].ntap[d] : nt;
+    budget = pl.lds_budget < budget ? pl.lds_budget : budget;
+    L->total_sub += pl.sub_capacity;
+    L->total_items += pl.item_capacity;
+  }
+  L->nt = nt <= 5 ? 5 : 8;
+  if (L->ndim == 3 && L->nt > 5)
+    return fail(TNL_EINVAL, "3-D gridding covers at most 5 taps per axis");
+  // channels per workgroup: the largest power of two (<= 8) whose LDS fits
+  // every problem under the smallest budget
+  auto max_u = [&](int cc) {
+    int64_t m = 0;
+    for (int i = 0; i < nprob; ++i) {
+      const int64_t b = (int64_t)ungrid_lds_bytes(make_bin_geom(*P[i].layout, false), cc, comp_bytes);
+      m = b > m ? b : m;
+    }
+    return m;
+  };
+  auto max_g = [&](int cc) {
+    int64_t m = 0;
+    for (int i = 0; i < nprob; ++i) {
+      const int64_t b = (int64_t)grid_lds_bytes(make_bin_geom(*P[i].layout, true), cc * comp, L->nt);
+      m = b > m ? b : m;
+    }
+    return m;
+  };
+  int64_t maxc = 1;
+  for (int i = 0; i < nprob; ++i) maxc = P[i].nbatch > maxc ? P[i].nbatch : maxc;
+  int ucc = 1, gcc = 1;
+  while (ucc < 8 && ucc < maxc && max_u(2 * ucc) <= budget) ucc *= 2;
+  while (gcc < 8 && gcc < maxc && max_g(2 * gcc) <= budget) gcc *= 2;
+  // one channel per workgroup is always taken (as for a single problem);
+  // the hard limit is the CU's LDS
+  if (max_u(ucc) > 160 * 1024 || max_g(gcc) > 160 * 1024)
+    return fail(TNL_EINVAL, "batch needs more LDS per workgroup than a CU has");
+  L->ungrid_cc = ucc;
+  L->grid_cc = gcc;
+  L->ungrid_lds = max_u(ucc);
+  L->grid_lds = max_g(gcc);
+  L->ungrid_chunks = (int32_t)((maxc + ucc - 1) / ucc);
+  L->bytes = (int64_t)nprob * (int64_t)sizeof(BatchDesc);
+  (void)comp;
+  return TNL_OK;
+}
+
+int batch_build(const InterpGeom* G, const tnl_batch_problem* P,
+                const tnl_batch_layout& L, void* ws, hipStream_t s) {
+  const int comp = L.is_complex ? 2 : 1;
+  BatchDesc* D = (BatchDesc*)std::calloc((size_t)L.nprob, sizeof(BatchDesc));
+  if (!D) return fail(TNL_EINVAL, "out of host memory");
+  int64_t sub_begin = 0, item_begin = 0;
+  for (int i = 0; i < L.nprob; ++i) {
+    const tnl_plan_layout& pl = *P[i].layout;
+    const char* w = (const char*)P[i].plan;
+    BatchDesc& d = D[i];
+    d.G = G[i];
+    d.Bu = make_bin_geom(pl, false);
+    d.Bg = make_bin_geom(pl, true);
+    d.nbatch = P[i].nbatch;
+    d.grid_off = P[i].grid_offset * comp;
+    d.pts_off = P[i].pts_offset * comp;
+    d.sub_begin = sub_begin;
+    d.item_begin = item_begin;
+    sub_begin += pl.sub_capacity;
+    item_begin += pl.item_capacity;
+    d.perm = (const int*)(w + pl.off_perm);
+    d.slocs = (const float*)(w + pl.off_slocs);
+    d.sub = (const int4*)(w + pl.off_sub);
+    d.nsub = (const int*)(w + pl.off_suboff) + pl.u_nbins + 1;
+    d.tlist = (const int*)(w + pl.off_tlist);
+    d.items = (const int4*)(w + pl.off_items);
+    d.nitems = (const int*)(w + pl.off_itemoff) + pl.g_ntiles_total;
+    d.gstarts = (const int*)(w + pl.off_gstarts);
+  }
+  int rc = check_hip(hipMemcpyAsync(ws, D, (size_t)L.bytes, hipMemcpyHostToDevice, s),
+                     "batch table copy");
+  if (!rc) rc = check_hip(hipStreamSynchronize(s), "batch table copy");
+  std::free(D);
+  return rc;
+}
+
+static int resident_workgroups(const void* kern, int threads, size_t lds) {
+  int per_cu = 0, dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
+  }
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) !=
+          hipSuccess || per_cu < 1)
+    per_cu = 1;
+  return per_cu * ncu;
+}
+
+template <int NDIM, bool CPLX, int CC, int NT>
+static int launch_ungrid_batched(const tnl_batch_layout& L, const BatchDesc* D,
+                                 const float* in, float* out, hipStream_t s) {
+  const auto kern = ungrid_batched_kernel<NDIM, CPLX, CC, NT>;
+  if (int rc = allow_lds((const void*)kern, (size_t)L.ungrid_lds)) return rc;
+  const int64_t nwg = (L.total_sub + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+  kern<<<dim3((unsigned)nwg, (unsigned)L.ungrid_chunks), BIN_THREADS, (size_t)L.ungrid_lds,
+         s>>>(D, L.nprob, L.total_sub, in, out);
+  TNL_LAUNCH_CHECK("ungrid_batched");
+  return TNL_OK;
+}
+
+template <int NDIM, bool CPLX, int CC, int NT>
+static int launch_grid_batched(const tnl_batch_layout& L, const BatchDesc* D,
+                               const float* in, float* out, hipStream_t s) {
+  constexpr int NP = CC * (CPLX ? 2 : 1);
+  constexpr int NW = NP >= 4 ? 4 : NP;
+  if constexpr (NDIM == 3 && NT > 5) {
+    return fail(TNL_EINVAL, "3-D gridding covers at most 5 taps per axis");
+  } else {
+    grid_zero_shared_batched_kernel<CPLX><<<1024, 256, 0, s>>>(D, L.nprob, L.total_items,
+                                                                out);
+    TNL_LAUNCH_CHECK("grid_zero_shared_batched");
+    const auto kern = grid_batched_kernel<NDIM, CPLX, CC, NT, NW>;
+    if (int rc = allow_lds((const void*)kern, (size_t)L.grid_lds)) return rc;
+    int64_t nwg = resident_workgroups((const void*)kern, GW * NW, (size_t)L.grid_lds);
+    nwg = (nwg + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+    const int64_t need = (L.total_items + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+    if (nwg > need) nwg = need;
+    kern<<<(unsigned)nwg, GW * NW, (size_t)L.grid_lds, s>>>(D, L.nprob, L.total_items, in,
+                                                            out);
+    TNL_LAUNCH_CHECK("grid_batched");
+    return TNL_OK;
+  }
+}
+
+template <int NDIM, bool CPLX, int NT>
+static int batch_dispatch_cc(bool adjoint, const tnl_batch_layout& L,
+                             const BatchDesc* D, const float* in, float* out,
+                             hipStream_t s) {
+  const int cc = adjoint ? L.grid_cc : L.ungrid_cc;
+#define TNL_BATCH_CASE(C)                                                     \
+  case C:                                                                     \
+    return adjoint ? launch_grid_batched<NDIM, CPLX, C, NT>(L, D, in, out, s) \
+                   : launch_ungrid_batched<NDIM, CPLX, C, NT>(L, D, in, out, s);
+  switch (cc) {
+    TNL_BATCH_CASE(8)
+    TNL_BATCH_CASE(4)
+    TNL_BATCH_CASE(2)
+    TNL_BATCH_CASE(1)
+    default:
+      return fail(TNL_EINVAL, "bad channel chunk in batch layout");
+  }
+#undef TNL_BATCH_CASE
+}
+
+int batch_run(bool adjoint, const tnl_batch_layout& L, const void* ws,
+              const float* in, float* out, hipStream_t s) {
+  const BatchDesc* D = (const BatchDesc*)ws;
+  const bool cx = L.is_complex != 0;
+  if (L.ndim == 3) {
+    if (L.nt != 5) return fail(TNL_EINVAL, "3-D batches cover at most 5 taps per axis");
+    return cx ? batch_dispatch_cc<3, true, 5>(adjoint, L, D, in, out, s)
+              : batch_dispatch_cc<3, false, 5>(adjoint, L, D, in, out, s);
+  }
+  if (L.nt == 5)
+    return cx ? batch_dispatch_cc<2, true, 5>(adjoint, L, D, in, out, s)
+              : batch_dispatch_cc<2, false, 5>(adjoint, L, D, in, out, s);
+  return cx ? batch_dispatch_cc<2, true, 8>(adjoint, L, D, in, out, s)
+            : batch_dispatch_cc<2, false, 8>(adjoint, L, D, in, out, s);
 }
 
 }  // namespace tnl

@@ -24,6 +24,25 @@ int interp_grad_locs(const float* grid_vals, const float* pt_vals,
                      int64_t npts, const InterpGeom& G, int is_complex,
                      hipStream_t s);
 
+int batch_prepare(const InterpGeom* G, const tnl_batch_problem* P, int nprob,
+                  int is_complex, tnl_batch_layout* L);
+int batch_build(const InterpGeom* G, const tnl_batch_problem* P,
+                const tnl_batch_layout& L, void* ws, hipStream_t s);
+int batch_run(bool adjoint, const tnl_batch_layout& L, const void* ws,
+              const float* in, float* out, hipStream_t s);
+
+int seg_layout(const int64_t* seg_len, int nseg, int64_t block_floats,
+               tnl_seg_layout* L);
+int seg_build(const int64_t* seg_len, const tnl_seg_layout& L, void* ws,
+              hipStream_t s);
+int seg_dot(const float* a, const float* b, double* out, const tnl_seg_layout& L,
+            void* ws, hipStream_t s);
+int seg_cg_update(float* x, float* r, const float* p, const float* Ap,
+                  const double* rs, const double* pAp, double* rs_new,
+                  const tnl_seg_layout& L, void* ws, hipStream_t s);
+int seg_cg_direction(float* p, const float* r, double* rs, const double* rs_new,
+                     const tnl_seg_layout& L, void* ws, hipStream_t s);
+
 int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
                 const float* width, int pad_mode, const tnl_tuning* tune,
                 tnl_plan_layout* L);

@@ -104,6 +104,67 @@ class PlanLayout(ctypes.Structure):
 _layout_p = ctypes.POINTER(PlanLayout)
 _tuning_p = ctypes.POINTER(Tuning)
 
+
+class BatchProblem(ctypes.Structure):
+    """Mirror of tnl_batch_problem (include/tnl.h)."""
+
+    _fields_ = [
+        ("nbatch", ctypes.c_int64),
+        ("npts", ctypes.c_int64),
+        ("grid_size", ctypes.c_int64 * 3),
+        ("width", ctypes.c_float * 3),
+        ("beta", ctypes.c_float),
+        ("grid_offset", ctypes.c_int64),
+        ("pts_offset", ctypes.c_int64),
+        ("layout", _layout_p),
+        ("plan", ctypes.c_void_p),
+    ]
+
+
+class BatchLayout(ctypes.Structure):
+    """Mirror of tnl_batch_layout (include/tnl.h)."""
+
+    _fields_ = [
+        ("nprob", ctypes.c_int32),
+        ("ndim", ctypes.c_int32),
+        ("kernel", ctypes.c_int32),
+        ("norm", ctypes.c_int32),
+        ("pad_mode", ctypes.c_int32),
+        ("is_complex", ctypes.c_int32),
+        ("nt", ctypes.c_int32),
+        ("ungrid_cc", ctypes.c_int32),
+        ("grid_cc", ctypes.c_int32),
+        ("ungrid_chunks", ctypes.c_int32),
+        ("total_sub", ctypes.c_int64),
+        ("total_items", ctypes.c_int64),
+        ("ungrid_lds", ctypes.c_int64),
+        ("grid_lds", ctypes.c_int64),
+        ("bytes", ctypes.c_int64),
+    ]
+
+
+_blayout_p = ctypes.POINTER(BatchLayout)
+
+
+class SegLayout(ctypes.Structure):
+    """Mirror of tnl_seg_layout (include/tnl.h)."""
+
+    _fields_ = [
+        ("nseg", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("numel", ctypes.c_int64),
+        ("block_floats", ctypes.c_int64),
+        ("nblocks", ctypes.c_int64),
+        ("off_blocks", ctypes.c_int64),
+        ("off_blkseg", ctypes.c_int64),
+        ("off_segblk", ctypes.c_int64),
+        ("off_partial", ctypes.c_int64),
+        ("bytes", ctypes.c_int64),
+    ]
+
+
+_slayout_p = ctypes.POINTER(SegLayout)
+
 # (name, restype, argtypes) for every symbol in include/tnl.h
 SIGNATURES = {
     "tnl_last_error": (ctypes.c_char_p, []),
@@ -123,6 +184,16 @@ SIGNATURES = {
         [_vp, _vp, _vp, _i64, _i64, _i32, _i64p, _f32p, _i32, _i32, _i32,
          _i32, _f32, _layout_p, _vp, _vp],
     ),
+    "tnl_batch_layout_query": (
+        _i32, [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _blayout_p]),
+    "tnl_batch_build": (_i32, [_vp, _blayout_p, _vp, _vp]),
+    "tnl_ungrid_batched": (_i32, [_vp, _vp, _blayout_p, _vp, _vp]),
+    "tnl_grid_batched": (_i32, [_vp, _vp, _blayout_p, _vp, _vp]),
+    "tnl_seg_layout_query": (_i32, [_i64p, _i32, _i64, _slayout_p]),
+    "tnl_seg_build": (_i32, [_i64p, _slayout_p, _vp, _vp]),
+    "tnl_seg_dot": (_i32, [_vp, _vp, _vp, _slayout_p, _vp, _vp]),
+    "tnl_seg_cg_update": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _slayout_p, _vp, _vp]),
+    "tnl_seg_cg_direction": (_i32, [_vp, _vp, _vp, _vp, _slayout_p, _vp, _vp]),
     "tnl_interp_grad_locs": (
         _i32,
         [_vp, _vp, _vp, _vp, _i64, _i64, _i32, _i64p, _f32p, _i32, _i32, _i32,

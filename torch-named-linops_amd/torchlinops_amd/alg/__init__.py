@@ -1,3 +1,5 @@
+from .batched import BatchedNUFFTNormal, SegmentedCG
 from .cg import CGRun, DeviceCG, conjugate_gradients, inner
 
-__all__ = ["conjugate_gradients", "CGRun", "DeviceCG", "inner"]
+__all__ = ["conjugate_gradients", "CGRun", "DeviceCG", "inner", "BatchedNUFFTNormal",
+           "SegmentedCG"]

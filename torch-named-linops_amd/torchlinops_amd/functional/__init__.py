@@ -1,5 +1,6 @@
 """Functional API of the gridding path (mirrors torchlinops.functional)."""
 
+from ._batch import BatchedInterp
 from ._interp import grid, interp_grad_locs, interpolate, interpolate_adjoint, ungrid
 from ._nufft import center_crop, center_pad, nufft, nufft_adjoint
 from ._plan import clear_plan_cache
@@ -13,6 +14,7 @@ from ._unfold import (
 )
 
 __all__ = [
+    "BatchedInterp",
     "grid",
     "ungrid",
     "interpolate",
