@@ -182,3 +182,53 @@ def test_segmented_cg_graph_replays_match_eager(dev):
         assert rel_l2(got.cpu().numpy(), eager.cpu().numpy()) < 1e-5, rep
         assert torch.allclose(cg.rs, rs_eager, rtol=1e-4), rep
     assert (cg.relative_residuals() < 1).all()
+
+
+def test_autotuner_picks_a_candidate_and_keeps_results(dev, tmp_path):
+    from torchlinops_amd.functional import autotune as AT
+
+    rng = np.random.default_rng(90)
+    P = _problems(rng, 2, n=4, grids=[64, 96], mrange=(2000, 8000), cmax=3, widths=[4.0, 6.0])
+    locs = [to_dev(p["locs"], dev) for p in P]
+    args = ([p["grid"] for p in P], [p["w"] for p in P], [p["C"] for p in P])
+    kp = [{"beta": p["beta"]} for p in P]
+    best, timings = AT.autotune_batch(locs, *args, kernel_params=kp, reps=2)
+    assert best in [t[0] for t in timings] and len(timings) >= 3
+    assert all(ms > 0 for _, ms in timings)
+    again, t2 = AT.autotune_batch(locs, *args, kernel_params=kp)
+    assert again == best and t2 == []  # memoised
+    AT.save_cache(tmp_path / "tune.json")
+    AT.load_cache(tmp_path / "tune.json")
+    B0 = F.BatchedInterp(locs, *args, kernel_params=kp)
+    B1 = F.BatchedInterp(locs, *args, kernel_params=kp, tuning=best)
+    g = B0.new_grid().normal_()
+    assert rel_l2(B1.ungrid(g).cpu().numpy(), B0.ungrid(g).cpu().numpy()) < 1e-6
+    pts = B0.new_points().normal_()
+    assert rel_l2(B1.grid(pts).cpu().numpy(), B0.grid(pts).cpu().numpy()) < 1e-6
+    # single problem: applies the winner as the 2-D tuning
+    from torchlinops_amd.functional import _plan
+
+    saved = _plan.get_tuning(2)
+    try:
+        b1, t1 = AT.autotune_interp(locs[0], P[0]["grid"], P[0]["w"], P[0]["C"],
+                                    kernel_params=kp[0], reps=2)
+        assert b1 in [t[0] for t in t1]
+        assert all(_plan.get_tuning(2)[k] == v for k, v in b1.items())
+    finally:
+        with _plan._tuning_lock:
+            _plan._tuning[2] = saved
+        _plan.clear_plan_cache()
+
+
+def test_batched_equals_per_problem_launches(dev):
+    rng = np.random.default_rng(91)
+    P = _problems(rng, 2, n=5, grids=[48, 64], mrange=(1000, 6000), cmax=4, widths=[4.0, 6.0])
+    B = F.BatchedInterp([to_dev(p["locs"], dev) for p in P], [p["grid"] for p in P],
+                        [p["w"] for p in P], [p["C"] for p in P],
+                        kernel_params=[{"beta": p["beta"]} for p in P])
+    g = B.new_grid().normal_()
+    pts = B.new_points().normal_()
+    a, b = B.ungrid(g), B.ungrid_each(g, B.new_points())
+    assert rel_l2(a.cpu().numpy(), b.cpu().numpy()) < 1e-6
+    a, b = B.grid(pts), B.grid_each(pts, B.new_grid())
+    assert rel_l2(a.cpu().numpy(), b.cpu().numpy()) < 1e-6

@@ -26,6 +26,8 @@ from .._lib import (
     BatchLayout,
     BatchProblem,
     check,
+    f32_array,
+    i64_array,
     lib,
     require_device,
     stream_of,
@@ -79,7 +81,7 @@ class BatchedInterp:
         self.npts = [prod(l.shape[:-1]) for l in locs]
         self.kernel, self.norm, self.pad_mode = kernel, int(norm), pad_mode
         pad_code = PAD_MODES[pad_mode]
-        self.plans = []
+        self.plans, self.locs2d = [], []
         probs = (BatchProblem * n)()
         goff = poff = 0
         self.grid_offsets, self.pts_offsets = [], []
@@ -91,6 +93,7 @@ class BatchedInterp:
             l2 = l.reshape(self.npts[i], self.ndim).to(torch.float32).contiguous()
             plan = build_plan(l2, self.ndim, self.grid_sizes[i], w, pad_code, tuning)
             self.plans.append(plan)
+            self.locs2d.append(l2)
             p = probs[i]
             p.nbatch = self.nbatches[i]
             p.npts = self.npts[i]
@@ -161,4 +164,33 @@ class BatchedInterp:
         with torch.cuda.device(self.device):
             check(lib().tnl_grid_batched(self._raw(pts_flat), self._raw(out), self.layout,
                                          self.workspace.data_ptr(), stream_of(out)))
+        return out
+
+    # ---- the same work launched problem by problem (same plans)
+    def _each(self, adjoint: bool, src: torch.Tensor, dst: torch.Tensor) -> None:
+        fn = lib().tnl_grid if adjoint else lib().tnl_ungrid
+        sv = self.pts_views(src) if adjoint else self.grid_views(src)
+        dv = self.grid_views(dst) if adjoint else self.pts_views(dst)
+        with torch.cuda.device(self.device):
+            for i, plan in enumerate(self.plans):
+                p = self._probs[i]
+                check(fn(self._raw(sv[i]), self.locs2d[i].data_ptr(), self._raw(dv[i]),
+                         self.nbatches[i], self.npts[i],
+                         self.ndim, i64_array(self.grid_sizes[i]),
+                         f32_array(list(p.width)[:self.ndim]), KERNELS[self.kernel], self.norm,
+                         PAD_MODES[self.pad_mode], int(self.is_complex), float(p.beta),
+                         plan.layout, plan.ptr, stream_of(dst)))
+
+    def ungrid_each(self, grid_flat: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+        """``ungrid`` as one launch per problem (for comparison)."""
+        self._check(grid_flat, self.grid_numel, "ungrid input")
+        self._check(out, self.pts_numel, "ungrid output")
+        self._each(False, grid_flat, out)
+        return out
+
+    def grid_each(self, pts_flat: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+        """``grid`` as one launch per problem (for comparison)."""
+        self._check(pts_flat, self.pts_numel, "grid input")
+        self._check(out, self.grid_numel, "grid output")
+        self._each(True, pts_flat, out)
         return out
