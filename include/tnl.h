@@ -223,6 +223,30 @@ int tnl_seg_cg_direction(float* p, const float* r, double* rs,
                          const double* rs_new, const tnl_seg_layout* layout,
                          void* ws, void* stream);
 
+/* ---- 3x3x3 convolutions of the learned preconditioner -------------------
+ * Channel-last bf16 activations [nbatch, z, y, x, c_p] (c_p a multiple of
+ * 8, padding channels zero); weights bf16 [16][Kp], row = output channel,
+ * column tap * cin_p + ci, tap = (dz+1)*9 + (dy+1)*3 + (dx+1), Kp =
+ * 32 * ceil(27 cin_p / 32), rows >= cout and columns >= 27 cin_p zero.
+ * "Same" zero padding; fp32 accumulation on the matrix cores (BF16 MFMA).
+ *   tnl_conv3d_fwd: out = [relu](conv(in, W) + bias), written as bf16
+ *     and/or fp32 [nbatch, z, y, x, cout_p] (either pointer may be null;
+ *     channels >= cout are not written).  cin_p in {8, 16}, cout <= 16.
+ *     The data gradient is this call with flipped, transposed weights.
+ *   tnl_conv3d_wgrad: dw[co][ci][tap] (fp32 [cout][cin_p][27]) = sum over
+ *     voxels of dout[co](p) * in[ci](p + tap); ws holds per-tile partials
+ *     (tnl_conv3d_wgrad_workspace bytes), summed in a fixed order.
+ * No counterpart in the reference (the learned preconditioner is an
+ * extension beside its CG, src/torchlinops/alg/pcg.py).
+ */
+int tnl_conv3d_fwd(const void* in, const void* w, const float* bias,
+                   void* out_bf16, float* out_f32, int64_t nbatch, int z, int y,
+                   int x, int cin_p, int cout, int cout_p, int relu, void* stream);
+int64_t tnl_conv3d_wgrad_workspace(int64_t nbatch, int z, int y, int x);
+int tnl_conv3d_wgrad(const void* in, const void* dout, float* dw, float* ws,
+                     int64_t ws_bytes, int64_t nbatch, int z, int y, int x,
+                     int cin_p, int cout, int cout_p, void* stream);
+
 /* ---- trajectory gradient of interpolate / interpolate_adjoint ----------
  * grad_locs[p, d] = sum_b sum_{g in taps(p)}
  *                     Re(conj(pt_vals[b, p]) * grid_vals[b, g]) * dW(g, t_p)/dt_{p,d}

@@ -388,3 +388,48 @@ def conjugate_gradients(A, y, max_num_iters=20, x0=None):
         rs = np.vdot(r, r).real
         p = (rs / rs_old) * p + r
     return x
+
+
+# ------------------------------------------- learned-preconditioner convs
+# No reference counterpart (the learned preconditioner is an extension):
+# float64 restatement of a 3x3x3 "same" cross-correlation layer and its
+# backward pass, pinned by scipy.ndimage.correlate and finite differences.
+def conv3d(x, w, b=None, relu=False):
+    """x [B, Cin, Z, Y, X], w [Cout, Cin, 3, 3, 3] -> [B, Cout, Z, Y, X]:
+    out[o](p) = sum_{c, d} w[o, c, d + 1] x[c](p + d), zero outside."""
+    x = np.asarray(x, dtype=np.float64)
+    w = np.asarray(w, dtype=np.float64)
+    B, C, Z, Y, X = x.shape
+    xp = np.pad(x, ((0, 0), (0, 0), (1, 1), (1, 1), (1, 1)))
+    out = np.zeros((B, w.shape[0], Z, Y, X))
+    for dz in range(3):
+        for dy in range(3):
+            for dx in range(3):
+                out += np.einsum("oc,bczyx->bozyx", w[:, :, dz, dy, dx],
+                                 xp[:, :, dz:dz + Z, dy:dy + Y, dx:dx + X])
+    if b is not None:
+        out += np.asarray(b, dtype=np.float64)[None, :, None, None, None]
+    return np.maximum(out, 0.0) if relu else out
+
+
+def conv3d_backward(x, w, g):
+    """Gradients of sum(g * conv3d(x, w, b)) (no activation):
+    dx[c](p) = sum_{o, d} w[o, c, d + 1] g[o](p - d),
+    dw[o, c, d + 1] = sum_p g[o](p) x[c](p + d), db[o] = sum_p g[o](p)."""
+    x = np.asarray(x, dtype=np.float64)
+    w = np.asarray(w, dtype=np.float64)
+    g = np.asarray(g, dtype=np.float64)
+    B, C, Z, Y, X = x.shape
+    xp = np.pad(x, ((0, 0), (0, 0), (1, 1), (1, 1), (1, 1)))
+    gp = np.pad(g, ((0, 0), (0, 0), (1, 1), (1, 1), (1, 1)))
+    dx = np.zeros_like(x)
+    dw = np.zeros_like(w)
+    for dz in range(3):
+        for dy in range(3):
+            for ddx in range(3):
+                # g[o](p - d) with d = (dz - 1, ...) sits at gp index p - d + 1
+                gs = gp[:, :, 2 - dz:2 - dz + Z, 2 - dy:2 - dy + Y, 2 - ddx:2 - ddx + X]
+                dx += np.einsum("oc,bozyx->bczyx", w[:, :, dz, dy, ddx], gs)
+                dw[:, :, dz, dy, ddx] = np.einsum(
+                    "bozyx,bczyx->oc", g, xp[:, :, dz:dz + Z, dy:dy + Y, ddx:ddx + X])
+    return dx, dw, g.sum(axis=(0, 2, 3, 4))

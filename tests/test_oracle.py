@@ -391,3 +391,44 @@ def test_oracle_grad_locs_planted_sign_error_is_caught():
     fd = (f(tp) - f(tm)) / (2 * step)
     assert abs(g[0, 1] - fd) < 1e-5 * (1 + abs(fd))
     assert abs(-g[0, 1] - fd) > 1e-3 * (1 + abs(fd))  # a flipped sign would fail
+
+
+# --------------------------------------- learned-preconditioner convs
+def test_conv3d_oracle_matches_scipy_correlate():
+    from scipy.ndimage import correlate
+
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal((2, 3, 5, 6, 7))
+    w = rng.standard_normal((4, 3, 3, 3, 3))
+    b = rng.standard_normal(4)
+    out = O.conv3d(x, w, b)
+    for bi in range(2):
+        for o in range(4):
+            want = sum(correlate(x[bi, c], w[o, c], mode="constant", cval=0.0)
+                       for c in range(3)) + b[o]
+            assert np.allclose(out[bi, o], want, atol=1e-12)
+    assert np.array_equal(O.conv3d(x, w, b, relu=True), np.maximum(out, 0))
+
+
+def test_conv3d_oracle_backward_matches_fd():
+    rng = np.random.default_rng(22)
+    x = rng.standard_normal((1, 2, 4, 5, 3))
+    w = rng.standard_normal((3, 2, 3, 3, 3))
+    b = rng.standard_normal(3)
+    g = rng.standard_normal((1, 3, 4, 5, 3))
+    dx, dw, db = O.conv3d_backward(x, w, g)
+    L = lambda xx, ww, bb: float(np.sum(g * O.conv3d(xx, ww, bb)))  # noqa: E731
+    h = 1e-6
+    for idx in [(0, 0, 0, 0, 0), (0, 1, 3, 4, 2), (0, 1, 2, 2, 1)]:
+        xp, xm = x.copy(), x.copy()
+        xp[idx] += h
+        xm[idx] -= h
+        assert abs((L(xp, w, b) - L(xm, w, b)) / (2 * h) - dx[idx]) < 1e-6
+    for idx in [(0, 0, 0, 0, 0), (2, 1, 1, 2, 0), (1, 0, 2, 1, 2)]:
+        wp, wm = w.copy(), w.copy()
+        wp[idx] += h
+        wm[idx] -= h
+        assert abs((L(x, wp, b) - L(x, wm, b)) / (2 * h) - dw[idx]) < 1e-6
+    bp = b.copy()
+    bp[1] += h
+    assert abs((L(x, w, bp) - L(x, w, b)) / h - db[1]) < 1e-5

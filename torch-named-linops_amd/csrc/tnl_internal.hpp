@@ -43,6 +43,14 @@ int seg_cg_update(float* x, float* r, const float* p, const float* Ap,
 int seg_cg_direction(float* p, const float* r, double* rs, const double* rs_new,
                      const tnl_seg_layout& L, void* ws, hipStream_t s);
 
+int conv3d_fwd(const void* in, const void* w, const float* bias, void* out_b,
+               float* out_f, int64_t nbatch, int Z, int Y, int X, int cin_p,
+               int cout, int cout_p, int relu, hipStream_t s);
+int64_t conv3d_wgrad_ws_bytes(int64_t nbatch, int Z, int Y, int X);
+int conv3d_wgrad(const void* in, const void* dout, float* dw, float* ws,
+                 int64_t ws_bytes, int64_t nbatch, int Z, int Y, int X, int cin_p,
+                 int cout, int cout_p, hipStream_t s);
+
 int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
                 const float* width, int pad_mode, const tnl_tuning* tune,
                 tnl_plan_layout* L);

@@ -194,6 +194,11 @@ SIGNATURES = {
     "tnl_seg_dot": (_i32, [_vp, _vp, _vp, _slayout_p, _vp, _vp]),
     "tnl_seg_cg_update": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _slayout_p, _vp, _vp]),
     "tnl_seg_cg_direction": (_i32, [_vp, _vp, _vp, _vp, _slayout_p, _vp, _vp]),
+    "tnl_conv3d_fwd": (
+        _i32, [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp]),
+    "tnl_conv3d_wgrad_workspace": (_i64, [_i64, _i32, _i32, _i32]),
+    "tnl_conv3d_wgrad": (
+        _i32, [_vp, _vp, _vp, _vp, _i64, _i64, _i32, _i32, _i32, _i32, _i32, _i32, _vp]),
     "tnl_interp_grad_locs": (
         _i32,
         [_vp, _vp, _vp, _vp, _i64, _i64, _i32, _i64p, _f32p, _i32, _i32, _i32,

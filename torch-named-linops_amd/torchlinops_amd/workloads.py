@@ -9,7 +9,7 @@ from __future__ import annotations
 import numpy as np
 
 __all__ = ["radial_3d", "spiral_2d", "vd_spiral_2d", "ellipse_phantom",
-           "ellipsoid_phantom", "gaussian_coil_maps", "uneven_2d_problems"]
+           "ellipsoid_phantom", "smooth_field", "gaussian_coil_maps", "uneven_2d_problems"]
 
 
 def radial_3d(nspokes: int, nread: int, n: int, rng: np.random.Generator,
@@ -85,11 +85,23 @@ def ellipsoid_phantom(n: int, rng: np.random.Generator, nell: int = 8) -> np.nda
     return img
 
 
+def smooth_field(n: int, rng: np.random.Generator, ndim: int = 3, corr: float = 0.15) -> np.ndarray:
+    """1 + 0.5 x (complex Gaussian noise low-passed to a correlation length
+    of ``corr`` x field of view, unit RMS): smooth, nonzero everywhere."""
+    shape = (n,) * ndim
+    z = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    k = np.meshgrid(*([np.fft.fftfreq(n)] * ndim), indexing="ij")
+    k2 = sum(kk**2 for kk in k)
+    f = np.fft.ifftn(np.fft.fftn(z) * np.exp(-2 * (np.pi * corr * n) ** 2 * k2 / n**2))
+    f = f / np.sqrt(np.mean(np.abs(f) ** 2))
+    return (1.0 + 0.5 * f).astype(np.complex64)
+
+
 def gaussian_coil_maps(ncoil: int, n: int, rng: np.random.Generator, ndim: int = 3,
-                       width: float = 0.6) -> np.ndarray:
+                       width: float = 0.6, normalize: bool = True) -> np.ndarray:
     """Smooth coil sensitivities: Gaussian bumps centred on a sphere (circle)
-    around the field of view, with random phases, normalised so that
-    sum_c |S_c|^2 = 1 everywhere."""
+    around the field of view, with random phases; with ``normalize``,
+    scaled so that sum_c |S_c|^2 = 1 everywhere."""
     g = np.linspace(-1, 1, n, dtype=np.float32)
     grids = np.meshgrid(*([g] * ndim), indexing="ij")
     maps = np.empty((ncoil,) + (n,) * ndim, dtype=np.complex64)
@@ -98,7 +110,8 @@ def gaussian_coil_maps(ncoil: int, n: int, rng: np.random.Generator, ndim: int =
         d = 1.2 * d / np.linalg.norm(d)
         r2 = sum((gr - dc) ** 2 for gr, dc in zip(grids, d))
         maps[c] = np.exp(-r2 / (2 * width**2)) * np.exp(1j * rng.uniform(0, 2 * np.pi))
-    maps /= np.sqrt(np.sum(np.abs(maps) ** 2, axis=0, keepdims=True))
+    if normalize:
+        maps /= np.sqrt(np.sum(np.abs(maps) ** 2, axis=0, keepdims=True))
     return maps
 
 
