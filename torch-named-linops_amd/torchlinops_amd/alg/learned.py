@@ -282,7 +282,7 @@ def train_preconditioner(net: PrecondNet, data, steps: int = 300, lr: float = 3e
         opt.zero_grad()
         loss.backward()
         opt.step()
-        losses.append(float(loss))
+        losses.append(float(loss.detach()))
         if log and (it % 50 == 0 or it == steps - 1):
             log(f"train step {it}: loss {losses[-1]:.4f}")
     return losses
