@@ -165,7 +165,8 @@ typedef struct tnl_batch_layout {
   int32_t nt;             /* tap capacity per axis of the launches */
   int32_t ungrid_cc;      /* channels per ungrid workgroup */
   int32_t grid_cc;        /* channels per gridding accumulator */
-  int32_t ungrid_chunks;  /* max over problems of ceil(nbatch / ungrid_cc) */
+  int32_t ungrid_chunks;  /* channel chunks an ungrid workgroup walks:
+                             max over problems of ceil(nbatch / ungrid_cc) */
   int64_t total_sub;      /* sum of the plans' subproblem capacities */
   int64_t total_items;    /* sum of the plans' gridding work-item capacities */
   int64_t ungrid_lds;     /* LDS bytes per ungrid workgroup */

@@ -1,12 +1,13 @@
 // Plan-driven interpolation (ungrid) and gridding kernels, 2-D and 3-D,
 // norm 1 (separable weights).
 //
-// ungrid: one workgroup per (subproblem, channel chunk).  A subproblem is
-//   <= max_sub points of one bin.  The workgroup stages the bin's grid tile
-//   plus a halo of ntap-1 cells per axis (wrapped, or zero outside the grid
-//   under zero padding) for CC channels in LDS, then each lane gathers the
-//   taps of one point from LDS for all CC channels at once: the per-point
-//   weights are computed once and reused for every channel.
+// ungrid: one workgroup per subproblem.  A subproblem is <= max_sub points
+//   of one bin.  For each chunk of CC channels the workgroup stages the
+//   bin's grid tile plus a halo of ntap-1 cells per axis (wrapped, or zero
+//   outside the grid under zero padding) in LDS, then each lane gathers the
+//   taps of one point from LDS for all CC channels at once.  The per-point
+//   weights are computed once, on the first chunk, and kept in registers
+//   for the later chunks.
 // grid: "owner computes".  One workgroup per (work item, channel chunk); a
 //   work item is a gridding tile with the list of points touching it.  Taps
 //   inside the tile are accumulated with LDS float atomics, then the tile is
@@ -61,25 +62,28 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nwg) {
   return (bid % NUM_XCD) * per + bid / NUM_XCD;
 }
 
-// One subproblem (<= max_sub points of one bin) of one problem, channels
-// [c0, c0 + CC).  Called by all threads of the workgroup.
+// One subproblem (<= max_sub points of one bin) of one problem, every
+// channel, CC channels at a time.  Called by all threads of the workgroup.
+// Each thread keeps the first tap offset, tap counts and window values of
+// its first UPTS points in registers across the channel chunks, so the
+// windows are evaluated once per point.
+constexpr int UPTS = 2;  // cached points per thread (max_sub <= 512)
+
 template <int NDIM, bool CPLX, int CC, int NT>
 __device__ __forceinline__ void ungrid_sub(
     const float* __restrict__ vals, float* __restrict__ out, int64_t nbatch,
     const InterpGeom& G, const BinGeom& B, const int* __restrict__ perm,
-    const float* __restrict__ slocs, int4 sp, int64_t c0) {
+    const float* __restrict__ slocs, int4 sp) {
   using C = Ch<CPLX>;
   using T = typename C::T;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* lds = reinterpret_cast<T*>(lds_raw);
   const int bin = sp.x, start = sp.y, count = sp.z;
-  int nc = (int)(nbatch - c0);
-  nc = nc < CC ? nc : CC;
+  T* __restrict__ o = reinterpret_cast<T*>(out);
   if (bin == B.nbins) {  // points without taps: zero output
-    T* __restrict__ o = reinterpret_cast<T*>(out);
     for (int i = threadIdx.x; i < count; i += BIN_THREADS) {
       const int p = perm[start + i];
-      for (int c = 0; c < nc; ++c) o[(c0 + c) * B.npts + p] = C::zero();
+      for (int64_t c = 0; c < nbatch; ++c) o[c * B.npts + p] = C::zero();
     }
     return;
   }
@@ -94,119 +98,175 @@ __device__ __forceinline__ void ungrid_sub(
       origin[d] = ti * B.tile[d];
     }
   }
-  // ---- stage window: channel planes [c][cell] in LDS (lanes gathering
-  // different cells then fall in different banks), coalesced along z in
-  // HBM.  A table of row bases (one per (channel, x, y) row of the window)
-  // keeps the per-element index math to one division by the row length.
   const T* __restrict__ v = reinterpret_cast<const T*>(vals);
   const int w2 = B.win[2];
   const int nrows = CC * B.win[0] * B.win[1];
   int64_t* rowbase = reinterpret_cast<int64_t*>(
       lds_raw + ((size_t)CC * B.wcells * sizeof(T) + 7) / 8 * 8);
-  for (int r = threadIdx.x; r < nrows; r += BIN_THREADS) {
-    const int c = r / (B.win[0] * B.win[1]);
-    const int rem = r - c * (B.win[0] * B.win[1]);
-    const int wx = rem / B.win[1], wy = rem - wx * B.win[1];
-    int64_t gx = origin[0] + wx, gy = origin[1] + wy;
-    bool ok = c < nc;
-    if (G.pad == TNL_PAD_CIRCULAR) {
-      gx = mod_pos(gx, G.n[0]);
-      gy = mod_pos(gy, G.n[1]);
-    } else {
-      ok = ok && gx < G.n[0] && gy < G.n[1];
-    }
-    rowbase[r] = ok ? (c0 + c) * G.ncells + (gx * G.n[1] + gy) * G.n[2] : -1;
-  }
-  __syncthreads();
   const float inv_w2 = 1.0f / (float)w2;
   const int total = nrows * w2;
-  for (int idx = threadIdx.x; idx < total; idx += BIN_THREADS) {
-    int r = (int)(((float)idx + 0.5f) * inv_w2);
-    if (r * w2 > idx) --r;              // correct the float quotient
-    else if ((r + 1) * w2 <= idx) ++r;
-    const int wz = idx - r * w2;
-    int64_t gz = origin[2] + wz;
-    const int64_t base = rowbase[r];
-    bool ok = base >= 0;
-    if (G.pad == TNL_PAD_CIRCULAR) {
-      if (gz >= G.n[2]) gz = mod_pos(gz, G.n[2]);
-    } else {
-      ok = ok && gz < G.n[2];
-    }
-    T val = C::zero();
-    if (ok) val = v[base + gz];
-    // channel c of row r lives in plane c; rows of one channel are
-    // contiguous, so the LDS index is simply idx
-    lds[idx] = val;
-  }
-  __syncthreads();
 
-  // ---- gather: one point per lane.  The x taps are a runtime loop (their
-  // window values computed on the fly), y and z taps are unrolled to NT:
-  // the loop body stays small enough for the instruction cache.
-  for (int i = threadIdx.x; i < count; i += BIN_THREADS) {
-    const int q = start + i;
-    const int p = perm[q];
-    float t[3] = {0.f, 0.f, 0.f};
-    TapRange r[3];
-    int lo[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      r[d].g0 = 0;
-      r[d].count = 1;
-      lo[d] = 0;
-      if (d >= 3 - NDIM) {
-        t[d] = slocs[(int64_t)q * NDIM + d - (3 - NDIM)];
-        r[d] = tap_range(t[d], G.half[d], G.ntap[d], G.n[d], G.pad);
-        const int64_t gw =
-            G.pad == TNL_PAD_CIRCULAR ? mod_pos(r[d].g0, G.n[d]) : r[d].g0;
-        lo[d] = (int)(gw - origin[d]);
+  // per-point data cached across channel chunks
+  float cw[UPTS][3 * NT];
+  int crow[UPTS], ccnt[UPTS], cpnt[UPTS];
+
+  for (int64_t c0 = 0; c0 < nbatch; c0 += CC) {
+    const int nc = (int)(nbatch - c0 < CC ? nbatch - c0 : CC);
+    if (c0 > 0) __syncthreads();  // previous chunk's gathers are done
+    // ---- stage window: channel planes [c][cell] in LDS (lanes gathering
+    // different cells then fall in different banks), coalesced along z in
+    // HBM.  A table of row bases (one per (channel, x, y) row of the
+    // window) keeps the per-element index math to one division.
+    for (int r = threadIdx.x; r < nrows; r += BIN_THREADS) {
+      const int c = r / (B.win[0] * B.win[1]);
+      const int rem = r - c * (B.win[0] * B.win[1]);
+      const int wx = rem / B.win[1], wy = rem - wx * B.win[1];
+      int64_t gx = origin[0] + wx, gy = origin[1] + wy;
+      bool ok = c < nc;
+      if (G.pad == TNL_PAD_CIRCULAR) {
+        gx = mod_pos(gx, G.n[0]);
+        gy = mod_pos(gy, G.n[1]);
+      } else {
+        ok = ok && gx < G.n[0] && gy < G.n[1];
       }
+      rowbase[r] = ok ? (c0 + c) * G.ncells + (gx * G.n[1] + gy) * G.n[2] : -1;
     }
-    float wy[NT], wz[NT];
-#pragma unroll
-    for (int k = 0; k < NT; ++k) {
-      wy[k] = 0.0f;
-      wz[k] = 0.0f;
-      if (NDIM >= 2 && k < r[1].count)
-        wy[k] = window(G.kernel,
-                       fminf(fabsf((float)(r[1].g0 + k) - t[1]) * G.inv_half[1], 1.0f),
-                       G.beta);
-      if (k < r[2].count)
-        wz[k] = window(G.kernel,
-                       fminf(fabsf((float)(r[2].g0 + k) - t[2]) * G.inv_half[2], 1.0f),
-                       G.beta);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < total; idx += BIN_THREADS) {
+      int r = (int)(((float)idx + 0.5f) * inv_w2);
+      if (r * w2 > idx) --r;              // correct the float quotient
+      else if ((r + 1) * w2 <= idx) ++r;
+      const int wz = idx - r * w2;
+      int64_t gz = origin[2] + wz;
+      const int64_t base = rowbase[r];
+      bool ok = base >= 0;
+      if (G.pad == TNL_PAD_CIRCULAR) {
+        if (gz >= G.n[2]) gz = mod_pos(gz, G.n[2]);
+      } else {
+        ok = ok && gz < G.n[2];
+      }
+      T val = C::zero();
+      if (ok) val = v[base + gz];
+      // channel c of row r lives in plane c; rows of one channel are
+      // contiguous, so the LDS index is simply idx
+      lds[idx] = val;
     }
-    if (NDIM < 2) wy[0] = 1.0f;
-    T acc[CC];
+    __syncthreads();
+
+    // ---- gather: one point per lane
+    int slot = 0;
+    for (int i = threadIdx.x; i < count; i += BIN_THREADS, ++slot) {
+      float w[3 * NT];
+      int rowoff, cnt0, cnt1, cnt2, p;
+      bool cached = false;
 #pragma unroll
-    for (int c = 0; c < CC; ++c) acc[c] = C::zero();
+      for (int k = 0; k < UPTS; ++k)
+        if (c0 > 0 && slot == k) {
+#pragma unroll
+          for (int e = 0; e < 3 * NT; ++e) w[e] = cw[k][e];
+          rowoff = crow[k];
+          cnt0 = ccnt[k] & 255; cnt1 = (ccnt[k] >> 8) & 255; cnt2 = (ccnt[k] >> 16) & 255;
+          p = cpnt[k];
+          cached = true;
+        }
+      if (!cached) {
+        const int q = start + i;
+        p = perm[q];
+        float t[3] = {0.f, 0.f, 0.f};
+        TapRange r[3];
+        int lo[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          r[d].g0 = 0;
+          r[d].count = 1;
+          lo[d] = 0;
+          if (d >= 3 - NDIM) {
+            t[d] = slocs[(int64_t)q * NDIM + d - (3 - NDIM)];
+            r[d] = tap_range(t[d], G.half[d], G.ntap[d], G.n[d], G.pad);
+            const int64_t gw =
+                G.pad == TNL_PAD_CIRCULAR ? mod_pos(r[d].g0, G.n[d]) : r[d].g0;
+            lo[d] = (int)(gw - origin[d]);
+          }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+          for (int k = 0; k < NT; ++k) {
+            float x = (d < 3 - NDIM && k == 0) ? 1.0f : 0.0f;
+            if (d >= 3 - NDIM && k < r[d].count)
+              x = window(G.kernel,
+                         fminf(fabsf((float)(r[d].g0 + k) - t[d]) * G.inv_half[d], 1.0f),
+                         G.beta);
+            w[d * NT + k] = x;
+          }
+        rowoff = (lo[0] * B.win[1] + lo[1]) * B.win[2] + lo[2];
+        cnt0 = r[0].count; cnt1 = r[1].count; cnt2 = r[2].count;
+#pragma unroll
+        for (int k = 0; k < UPTS; ++k)
+          if (slot == k) {
+#pragma unroll
+            for (int e = 0; e < 3 * NT; ++e) cw[k][e] = w[e];
+            crow[k] = rowoff;
+            ccnt[k] = cnt0 | (cnt1 << 8) | (cnt2 << 16);
+            cpnt[k] = p;
+          }
+      }
+      T acc[CC];
+#pragma unroll
+      for (int c = 0; c < CC; ++c) acc[c] = C::zero();
+      // common case: four taps on every axis (all but points on integer
+      // coordinates for widths in (3, 4]).  Branch-free, so each x row's
+      // 16 x CC LDS reads issue together and their latency overlaps.
+      const bool four = NT >= 4 && cnt1 == 4 && cnt2 == 4 && (NDIM < 3 || cnt0 == 4);
+      if (NDIM >= 2 && four) {
+#pragma unroll
+        for (int kx = 0; kx < (NDIM >= 3 ? 4 : 1); ++kx) {
+          const float wx = w[kx];
+          const T* rp = lds + rowoff + kx * B.win[1] * B.win[2];
+          T tv[4][4][CC];
+#pragma unroll
+          for (int ky = 0; ky < 4; ++ky)
+#pragma unroll
+            for (int kz = 0; kz < 4; ++kz)
+#pragma unroll
+              for (int c = 0; c < CC; ++c) tv[ky][kz][c] = rp[ky * B.win[2] + c * B.wcells + kz];
+#pragma unroll
+          for (int ky = 0; ky < 4; ++ky) {
+            const float wxy = wx * w[NT + ky];
+#pragma unroll
+            for (int kz = 0; kz < 4; ++kz) {
+              const float wt = wxy * w[2 * NT + kz];
+#pragma unroll
+              for (int c = 0; c < CC; ++c) C::fma(acc[c], wt, tv[ky][kz][c]);
+            }
+          }
+        }
+      } else {
 #pragma unroll 1
-    for (int kx = 0; kx < r[0].count; ++kx) {
-      float wx = 1.0f;
-      if (NDIM >= 3)
-        wx = window(G.kernel,
-                    fminf(fabsf((float)(r[0].g0 + kx) - t[0]) * G.inv_half[0], 1.0f),
-                    G.beta);
-      const int xrow = (lo[0] + kx) * B.win[1];
+        for (int kx = 0; kx < cnt0; ++kx) {
+          float wx = 1.0f;
 #pragma unroll
-      for (int ky = 0; ky < NT; ++ky) {
-        if (ky >= r[1].count) break;
-        const float wxy = wx * wy[ky];
-        const T* rowp = lds + (xrow + lo[1] + ky) * B.win[2] + lo[2];
+          for (int k = 0; k < NT; ++k)
+            if (k == kx) wx = w[k];
 #pragma unroll
-        for (int kz = 0; kz < NT; ++kz) {
-          if (kz >= r[2].count) break;
-          const float wt = wxy * wz[kz];
+          for (int ky = 0; ky < NT; ++ky) {
+            if (ky >= cnt1) break;
+            const float wxy = wx * w[NT + ky];
+            const T* rowp = lds + rowoff + (kx * B.win[1] + ky) * B.win[2];
 #pragma unroll
-          for (int c = 0; c < CC; ++c) C::fma(acc[c], wt, rowp[c * B.wcells + kz]);
+            for (int kz = 0; kz < NT; ++kz) {
+              if (kz >= cnt2) break;
+              const float wt = wxy * w[2 * NT + kz];
+#pragma unroll
+              for (int c = 0; c < CC; ++c) C::fma(acc[c], wt, rowp[c * B.wcells + kz]);
+            }
+          }
         }
       }
-    }
-    T* __restrict__ o = reinterpret_cast<T*>(out);
 #pragma unroll
-    for (int c = 0; c < CC; ++c)
-      if (c < nc) o[(c0 + c) * B.npts + p] = acc[c];
+      for (int c = 0; c < CC; ++c)
+        if (c < nc) o[(c0 + c) * B.npts + p] = acc[c];
+    }
   }
 }
 
@@ -216,10 +276,15 @@ __global__ __launch_bounds__(BIN_THREADS) void ungrid_binned_kernel(
     InterpGeom G, BinGeom B, const int* __restrict__ perm,
     const float* __restrict__ slocs, const int4* __restrict__ sub,
     const int* __restrict__ nsub_ptr) {
-  const int64_t s = xcd_remap(blockIdx.x, gridDim.x);
-  if (s >= *nsub_ptr) return;
-  ungrid_sub<NDIM, CPLX, CC, NT>(vals, out, nbatch, G, B, perm, slocs, sub[s],
-                                 (int64_t)blockIdx.y * CC);
+  // The grid is sized for the plan's subproblem capacity; remap over the
+  // live count only, so that every XCD gets its share of the real work
+  // (hardware hands workgroup b to XCD b % NUM_XCD).
+  const int64_t nsub = *nsub_ptr;
+  const int64_t n8 = (nsub + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+  if ((int64_t)blockIdx.x >= n8) return;
+  const int64_t s = xcd_remap(blockIdx.x, n8);
+  if (s >= nsub) return;
+  ungrid_sub<NDIM, CPLX, CC, NT>(vals, out, nbatch, G, B, perm, slocs, sub[s]);
 }
 
 // Pre-pass for gridding: zero the output region of every split (shared)
@@ -811,7 +876,6 @@ static int launch_binned(bool adjoint, const float* vals, float* out,
                          hipStream_t s) {
   const BinGeom B = make_bin_geom(L, adjoint);
   const int comp_bytes = CPLX ? 8 : 4;
-  const unsigned nchunk = (unsigned)((nbatch + CC - 1) / CC);
   const int* perm = (const int*)(ws + L.off_perm);
   const float* slocs = (const float*)(ws + L.off_slocs);
   if (!adjoint) {
@@ -820,7 +884,7 @@ static int launch_binned(bool adjoint, const float* vals, float* out,
     if (int rc = allow_lds((const void*)ungrid_binned_kernel<NDIM, CPLX, CC, NT>, lds))
       return rc;
     ungrid_binned_kernel<NDIM, CPLX, CC, NT>
-        <<<dim3((unsigned)nwg, nchunk), BIN_THREADS, lds, s>>>(
+        <<<dim3((unsigned)nwg, 1), BIN_THREADS, lds, s>>>(
             vals, out, nbatch, G, B, perm, slocs,
             (const int4*)(ws + L.off_sub),
             (const int*)(ws + L.off_suboff) + L.u_nbins + 1);
@@ -946,10 +1010,9 @@ __global__ __launch_bounds__(BIN_THREADS) void ungrid_batched_kernel(
   if (s >= total) return;
   const BatchDesc& d = D[find_problem(D, nprob, s, false)];
   const int64_t sl = s - d.sub_begin;
-  const int64_t c0 = (int64_t)blockIdx.y * CC;
-  if (sl >= *d.nsub || c0 >= d.nbatch) return;
+  if (sl >= *d.nsub) return;
   ungrid_sub<NDIM, CPLX, CC, NT>(gbase + d.grid_off, pbase + d.pts_off, d.nbatch,
-                                 d.G, d.Bu, d.perm, d.slocs, d.sub[sl], c0);
+                                 d.G, d.Bu, d.perm, d.slocs, d.sub[sl]);
 }
 
 template <bool CPLX>
@@ -1097,8 +1160,8 @@ static int launch_ungrid_batched(const tnl_batch_layout& L, const BatchDesc* D,
   const auto kern = ungrid_batched_kernel<NDIM, CPLX, CC, NT>;
   if (int rc = allow_lds((const void*)kern, (size_t)L.ungrid_lds)) return rc;
   const int64_t nwg = (L.total_sub + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
-  kern<<<dim3((unsigned)nwg, (unsigned)L.ungrid_chunks), BIN_THREADS, (size_t)L.ungrid_lds,
-         s>>>(D, L.nprob, L.total_sub, in, out);
+  kern<<<dim3((unsigned)nwg, 1), BIN_THREADS, (size_t)L.ungrid_lds, s>>>(D, L.nprob,
+                                                                        L.total_sub, in, out);
   TNL_LAUNCH_CHECK("ungrid_batched");
   return TNL_OK;
 }

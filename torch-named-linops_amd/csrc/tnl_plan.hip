@@ -102,7 +102,7 @@ int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
   }
   L->u_nbins = (int64_t)L->u_ntiles[0] * L->u_ntiles[1] * L->u_ntiles[2];
   L->g_ntiles_total = (int64_t)L->g_ntiles[0] * L->g_ntiles[1] * L->g_ntiles[2];
-  L->max_sub = (tune && tune->max_sub > 0) ? tune->max_sub : 2048;
+  L->max_sub = (tune && tune->max_sub > 0) ? tune->max_sub : 512;
   L->sub_capacity = L->u_nbins + (npts + L->max_sub - 1) / L->max_sub + 1;
   L->g_cap = (tune && tune->g_cap > 0) ? tune->g_cap : 2048;
   L->lds_budget = (tune && tune->lds_budget > 0) ? tune->lds_budget : 53248;
