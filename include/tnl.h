@@ -61,8 +61,15 @@ typedef struct tnl_tuning {
   int32_t max_sub;    /* points per interpolation subproblem                */
   int32_t g_cap;      /* touches per gridding work item                     */
   int32_t lds_budget; /* LDS bytes per workgroup for channel chunking       */
-  int32_t reserved;
+  int32_t grid_accum; /* TNL_GRID_ACCUM_*: 3-D gridding accumulator        */
 } tnl_tuning;
+
+/* Gridding accumulator.  AUTO takes the matrix-core (MFMA) accumulator for
+ * 3-D plans with 8x8x8 gridding tiles and the LDS accumulator otherwise;
+ * MFMA on a plan that does not qualify is an error. */
+#define TNL_GRID_ACCUM_AUTO 0
+#define TNL_GRID_ACCUM_LDS 1
+#define TNL_GRID_ACCUM_MFMA 2
 
 typedef struct tnl_plan_layout {
   int32_t ndim;
@@ -85,6 +92,7 @@ typedef struct tnl_plan_layout {
   int64_t touch_capacity;/* upper bound on (tile, point) touches             */
   int64_t item_capacity; /* upper bound on gridding work items               */
   int64_t lds_budget;    /* LDS bytes per workgroup                          */
+  int64_t grid_accum;    /* TNL_GRID_ACCUM_* (resolved: never AUTO)          */
   /* byte offsets inside the workspace                                      */
   int64_t off_counts, off_starts, off_cursor, off_perm, off_slocs;
   int64_t off_suboff, off_sub;

@@ -52,6 +52,17 @@ def test_plan_layout_is_host_only_and_consistent():
     assert tuple(L3.u_tile) == (4, 8, 16) and L3.g_cap == 512 and L3.lds_budget == 131072
     with pytest.raises(ValueError, match="160 KiB"):
         plan_layout(10, 2, (8, 8), (4.0, 4.0), 0, tuning={"lds_budget": 1 << 20})
+    # gridding accumulator: matrix cores for 3-D 8x8x8 tiles, LDS otherwise
+    assert L.grid_accum == 2 and L2.grid_accum == 1
+    L4 = plan_layout(1000, 3, (64, 64, 64), (4.0,) * 3, 0, tuning={"grid_accum": "lds"})
+    assert L4.grid_accum == 1
+    L5 = plan_layout(1000, 3, (64, 64, 64), (4.0,) * 3, 0, tuning={"g_tile": (8, 8, 16)})
+    assert L5.grid_accum == 1
+    with pytest.raises(ValueError, match="8x8x8"):
+        plan_layout(1000, 3, (64, 64, 64), (4.0,) * 3, 0,
+                    tuning={"g_tile": (8, 8, 16), "grid_accum": "mfma"})
+    with pytest.raises(ValueError, match="grid_accum"):
+        plan_layout(1000, 3, (64, 64, 64), (4.0,) * 3, 0, tuning={"grid_accum": "fast"})
 
 
 def test_library_validation_maps_to_exceptions():

@@ -760,6 +760,268 @@ __global__ __launch_bounds__(GW * NW) void grid_binned_kernel(
                                       tlist, items[it]);
 }
 
+// ---------------------------------------------------------------------------
+// Gridding with matrix-core accumulation (3-D, 8x8x8 tiles).
+//
+// For one tile, out[cell][plane] = sum_p A[cell][p] V[p][plane] with
+// A[cell][p] = wx_p(x) wy_p(y) wz_p(z), the point's separable weight at the
+// cell (zero off its footprint), and V the point's channel values (planes:
+// re/im of up to 8 coils, or up to 16 real channels).  That is a
+// (512 x P) by (P x 16) product.  v_mfma_f32_16x16x4f32 takes a block of 16
+// cells (one x plane, a 4x4 (y, z) patch) by 16 planes by 4 points; lane l
+// holds A[l % 16][l / 16], B[l / 16][l % 16] and D[4 (l / 16) + i][l % 16].
+// A block that none of the 4 points touches is skipped (a wave-uniform
+// mask), so the work follows the footprints rather than the tile volume.
+//
+// Workgroup: 4 waves on one tile.  Waves 0, 1 hold x planes 0-3 (16 blocks,
+// 64 accumulator registers per lane), waves 2, 3 x planes 4-7; the two waves
+// of a pair take alternate groups of 4 points.  Touches are taken in
+// batches of MB points (one per thread):
+//   1. lanes = points: first taps, the 8-cell tile-local window of every
+//      axis (zero off the footprint; wrapped taps land where they wrap to,
+//      and taps that land on one cell add), the channel values, and the
+//      mask of touched blocks OR-ed over each group of 4 points.  The next
+//      batch's raw inputs are loaded right after.
+//   2. waves: per group of 4 points, A = products of the windows, B = the
+//      values, one MFMA per live block.
+// At the end of the tile the two partial sums of each x half meet in LDS
+// (one wave stores, the other adds) and the tile goes to HBM once,
+// coalesced along z.
+typedef float mf32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int MF_THREADS = 256;
+constexpr int MB = 256;          // points per batch (one per thread)
+constexpr int MF_PSTR = 516;     // staging plane stride (floats): 512 + 4
+constexpr int MF_NPL = 16;       // accumulator planes
+constexpr size_t MF_LDS_BATCH = (size_t)MB * (8 + 8 + 8 + MF_NPL) * 4 + MB;
+constexpr size_t MF_LDS_STAGE = (size_t)MF_NPL * MF_PSTR * 4;
+constexpr size_t MF_LDS = MF_LDS_BATCH > MF_LDS_STAGE ? MF_LDS_BATCH : MF_LDS_STAGE;
+
+// 8-cell tile-local window of one axis for the point at coordinate t; bit
+// l of *pres is set when some tap lands on cell l.  Order of the output:
+// plain (W[l] at l) when PERM is false, [W0, W4, W1, W5, W2, W6, W3, W7]
+// when true (so a lane reads the pair {W[j], W[j + 4]} with one b64 load).
+template <int NT, bool PERM>
+__device__ __forceinline__ void mf_axis_window(float t, int d, int origin,
+                                               const InterpGeom& G, float* out,
+                                               unsigned* pres) {
+  float W[8];
+#pragma unroll
+  for (int l = 0; l < 8; ++l) W[l] = 0.f;
+  unsigned pm = 0;
+  const TapRange r = tap_range(t, G.half[d], G.ntap[d], G.n[d], G.pad);
+  const int64_t gw = G.pad == TNL_PAD_CIRCULAR ? mod_pos(r.g0, G.n[d]) : r.g0;
+  const int lo = (int)(gw - origin);
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    if (k < r.count) {
+      int l = lo + k;
+      if (G.pad == TNL_PAD_CIRCULAR && origin + l >= G.n[d]) l -= (int)G.n[d];
+      const float w = window(G.kernel,
+                             fminf(fabsf((float)(r.g0 + k) - t) * G.inv_half[d], 1.0f),
+                             G.beta);
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        if (l == c) {
+          W[c] += w;
+          pm |= 1u << c;
+        }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) out[c] = PERM ? W[(c >> 1) + 4 * (c & 1)] : W[c];
+  *pres = pm;
+}
+
+template <bool CPLX, int NT>
+__global__ __launch_bounds__(MF_THREADS) void grid_mfma_kernel(
+    const float* __restrict__ vals, float* __restrict__ out, int64_t nbatch,
+    InterpGeom G, BinGeom B, const int* __restrict__ perm,
+    const float* __restrict__ slocs, const int* __restrict__ tlist,
+    const int4* __restrict__ items, const int* __restrict__ nitems_ptr) {
+  using C = Ch<CPLX>;
+  using T = typename C::T;
+  constexpr int CH = CPLX ? MF_NPL / 2 : MF_NPL;  // channels per chunk
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  float* s_wx = reinterpret_cast<float*>(lds_raw);  // [MB][8]
+  float* s_wy = s_wx + MB * 8;                      // [MB][8], permuted
+  float* s_wz = s_wy + MB * 8;                      // [MB][8], permuted
+  float* s_v = s_wz + MB * 8;                       // [MB][16]
+  unsigned* s_km = reinterpret_cast<unsigned*>(s_v + MB * MF_NPL);  // [MB / 4]
+  float* stage = reinterpret_cast<float*>(lds_raw);  // [16][MF_PSTR], after batches
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int half = wv >> 1, par = wv & 1;
+  const int n16 = lane & 15, k4 = lane >> 4;  // B/D column, A column / D row group
+  const int yy = n16 >> 2, zz = n16 & 3;      // A row m = 4 yy + zz
+  const T* __restrict__ v = reinterpret_cast<const T*>(vals);
+  const int nitems = *nitems_ptr;
+
+  for (int64_t it = xcd_remap(blockIdx.x, gridDim.x); it < nitems; it += gridDim.x) {
+    const int4 item = items[it];
+    const int start = item.y, count = item.z;
+    const bool shared = item.w != 0;
+    int origin[3];
+    {
+      int r = item.x;
+      for (int d = 2; d >= 0; --d) {
+        origin[d] = (r % B.ntiles[d]) * B.tile[d];
+        r /= B.ntiles[d];
+      }
+    }
+    for (int64_t c0 = 0; c0 < nbatch; c0 += CH) {
+      const int nc = (int)(nbatch - c0 < CH ? nbatch - c0 : CH);
+      mf32x4 acc[16];
+#pragma unroll
+      for (int b = 0; b < 16; ++b) acc[b] = (mf32x4){0.f, 0.f, 0.f, 0.f};
+      // raw inputs of this thread's point of a batch
+      float pt[3] = {0.f, 0.f, 0.f};
+      T pv[CH];
+      auto load = [&](int b0, int nb) {
+        if (tid < nb) {
+          const int q = tlist[start + b0 + tid];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) pt[d] = slocs[(int64_t)q * 3 + d];
+          const int p = perm[q];
+#pragma unroll
+          for (int c = 0; c < CH; ++c) pv[c] = c < nc ? v[(c0 + c) * B.npts + p] : C::zero();
+        }
+      };
+      load(0, count < MB ? count : MB);
+      for (int b0 = 0; b0 < count; b0 += MB) {
+        const int nb = count - b0 < MB ? count - b0 : MB;
+        __syncthreads();  // previous batch's MFMA reads (or the write-out) are done
+        // ---- phase 1, lanes = points
+        {
+          const bool live = tid < nb;
+          float wx[8], wy[8], wz[8];
+          unsigned px = 0, py = 0, pz = 0;
+          if (live) {
+            mf_axis_window<NT, false>(pt[0], 0, origin[0], G, wx, &px);
+            mf_axis_window<NT, true>(pt[1], 1, origin[1], G, wy, &py);
+            mf_axis_window<NT, true>(pt[2], 2, origin[2], G, wz, &pz);
+          } else {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) wx[c] = wy[c] = wz[c] = 0.f;
+          }
+          float4* dx = reinterpret_cast<float4*>(s_wx + tid * 8);
+          float4* dy = reinterpret_cast<float4*>(s_wy + tid * 8);
+          float4* dz = reinterpret_cast<float4*>(s_wz + tid * 8);
+          dx[0] = make_float4(wx[0], wx[1], wx[2], wx[3]);
+          dx[1] = make_float4(wx[4], wx[5], wx[6], wx[7]);
+          dy[0] = make_float4(wy[0], wy[1], wy[2], wy[3]);
+          dy[1] = make_float4(wy[4], wy[5], wy[6], wy[7]);
+          dz[0] = make_float4(wz[0], wz[1], wz[2], wz[3]);
+          dz[1] = make_float4(wz[4], wz[5], wz[6], wz[7]);
+          float tv[MF_NPL];
+#pragma unroll
+          for (int c = 0; c < CH; ++c) {
+            if constexpr (CPLX) {
+              tv[2 * c] = live ? pv[c].x : 0.f;
+              tv[2 * c + 1] = live ? pv[c].y : 0.f;
+            } else {
+              tv[c] = live ? pv[c] : 0.f;
+            }
+          }
+          float4* dv = reinterpret_cast<float4*>(s_v + tid * MF_NPL);
+#pragma unroll
+          for (int i = 0; i < MF_NPL / 4; ++i)
+            dv[i] = make_float4(tv[4 * i], tv[4 * i + 1], tv[4 * i + 2], tv[4 * i + 3]);
+          // blocks touched: bit 4 x + (2 py + pz) for x plane x, (y, z) patch
+          const unsigned yh = ((py & 15u) ? 1u : 0u) | ((py >> 4) ? 2u : 0u);
+          const unsigned zh = ((pz & 15u) ? 1u : 0u) | ((pz >> 4) ? 2u : 0u);
+          const unsigned pm = ((yh & 1u) && (zh & 1u) ? 1u : 0u) |
+                              ((yh & 1u) && (zh & 2u) ? 2u : 0u) |
+                              ((yh & 2u) && (zh & 1u) ? 4u : 0u) |
+                              ((yh & 2u) && (zh & 2u) ? 8u : 0u);
+          unsigned m = 0;
+#pragma unroll
+          for (int x = 0; x < 8; ++x)
+            if ((px >> x) & 1u) m |= pm << (4 * x);
+          m |= (unsigned)__shfl_xor((int)m, 1);
+          m |= (unsigned)__shfl_xor((int)m, 2);
+          if ((tid & 3) == 0) s_km[tid >> 2] = m;
+        }
+        if (b0 + MB < count) load(b0 + MB, count - b0 - MB < MB ? count - b0 - MB : MB);
+        __syncthreads();
+        // ---- phase 2: this wave's x half, every other group of 4 points
+        const int nkg = (nb + 3) >> 2;
+        for (int kg = par; kg < nkg; kg += 2) {
+          const unsigned hm =
+              (__builtin_amdgcn_readfirstlane(s_km[kg]) >> (16 * half)) & 0xFFFFu;
+          if (hm == 0) continue;
+          const int pnt = kg * 4 + k4;
+          const float4 x4 = *reinterpret_cast<const float4*>(s_wx + pnt * 8 + 4 * half);
+          const float2 y2 = *reinterpret_cast<const float2*>(s_wy + pnt * 8 + 2 * yy);
+          const float2 z2 = *reinterpret_cast<const float2*>(s_wz + pnt * 8 + 2 * zz);
+          const float bv = s_v[pnt * MF_NPL + n16];
+          const float wyz[4] = {y2.x * z2.x, y2.x * z2.y, y2.y * z2.x, y2.y * z2.y};
+          const float wxs[4] = {x4.x, x4.y, x4.z, x4.w};
+#pragma unroll
+          for (int xl = 0; xl < 4; ++xl)
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+              if ((hm >> (4 * xl + p)) & 1u)
+                acc[4 * xl + p] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                    wxs[xl] * wyz[p], bv, acc[4 * xl + p], 0, 0, 0);
+        }
+      }
+      __syncthreads();  // batch buffers become the staging area
+      // ---- combine the pair's partial sums: lane holds cells
+      // (x, 4 py + k4, 4 pz + i), i < 4, of plane n16
+      auto stage_at = [&](int xl, int p) {
+        const int x = 4 * half + xl, y = 4 * (p >> 1) + k4, z0 = 4 * (p & 1);
+        return reinterpret_cast<float4*>(stage + n16 * MF_PSTR + x * 64 + y * 8 + z0);
+      };
+      if (par == 0) {
+#pragma unroll
+        for (int b = 0; b < 16; ++b)
+          *stage_at(b >> 2, b & 3) = make_float4(acc[b][0], acc[b][1], acc[b][2], acc[b][3]);
+      }
+      __syncthreads();
+      if (par == 1) {
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+          float4* sp = stage_at(b >> 2, b & 3);
+          float4 o4 = *sp;
+          o4.x += acc[b][0]; o4.y += acc[b][1]; o4.z += acc[b][2]; o4.w += acc[b][3];
+          *sp = o4;
+        }
+      }
+      __syncthreads();
+      // ---- write the tile: coalesced along z
+      T* __restrict__ o = reinterpret_cast<T*>(out);
+      for (int cell = tid; cell < 512; cell += MF_THREADS) {
+        const int lx = cell >> 6, ly = (cell >> 3) & 7, lz = cell & 7;
+        const int64_t gx = origin[0] + lx, gy = origin[1] + ly, gz = origin[2] + lz;
+        if (gx >= G.n[0] || gy >= G.n[1] || gz >= G.n[2]) continue;
+        const int64_t g0 = (gx * G.n[1] + gy) * G.n[2] + gz;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          if (c >= nc) break;
+          const int64_t gi = (c0 + c) * G.ncells + g0;
+          if constexpr (CPLX) {
+            const float a0 = stage[(2 * c) * MF_PSTR + cell];
+            const float a1 = stage[(2 * c + 1) * MF_PSTR + cell];
+            if (!shared) {
+              o[gi] = make_float2(a0, a1);
+            } else {
+              unsafeAtomicAdd(out + 2 * gi, a0);
+              unsafeAtomicAdd(out + 2 * gi + 1, a1);
+            }
+          } else {
+            const float a0 = stage[c * MF_PSTR + cell];
+            if (!shared) o[gi] = a0;
+            else unsafeAtomicAdd(out + gi, a0);
+          }
+        }
+      }
+    }
+  }
+}
+
 // Bank cost of one gridding RMW instruction: lane l touches the cell at
 // offset X*k0 + Y*k1 + k2 from the point's first tap (3-D lanes: 4x4x4 taps,
 // 2-D: 8x8).  Sum of the worst bank multiplicities, for LDS serving a whole
@@ -842,11 +1104,22 @@ static size_t grid_lds_bytes(const BinGeom& B, int planes, int nt) {
   return (size_t)grid_lds_layout(B, planes, nt).bytes;
 }
 
+// The plan resolves the accumulator (matrix cores for 3-D 8x8x8 tiles
+// unless asked otherwise).
+static bool use_grid_mfma(const InterpGeom& G, const tnl_plan_layout& L) {
+  return G.ndim == 3 && L.grid_accum == TNL_GRID_ACCUM_MFMA;
+}
+
 bool binned_supported(const InterpGeom& G, const tnl_plan_layout& L,
                       bool adjoint) {
   if (G.ndim < 2 || G.norm != 1) return false;
   for (int d = 0; d < 3; ++d)
     if (G.ntap[d] > 8 || G.ntap[d] != L.ntaps[d]) return false;
+  if (adjoint && use_grid_mfma(G, L)) {
+    for (int d = 0; d < 3; ++d)
+      if (G.n[d] < G.ntap[d]) return false;  // a footprint would self-wrap
+    return true;
+  }
   if (adjoint) {
     const BinGeom B = make_bin_geom(L, true);
     for (int d = 3 - G.ndim; d < 3; ++d) {
@@ -867,6 +1140,42 @@ static int allow_lds(const void* fn, size_t bytes) {
   return check_hip(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)bytes),
                    "hipFuncSetAttribute(MaxDynamicSharedMemorySize)");
+}
+
+// Workgroups of a persistent kernel that fit on the device at once.
+static int resident_workgroups(const void* kern, int threads, size_t lds) {
+  int per_cu = 0, dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
+  }
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) !=
+          hipSuccess || per_cu < 1)
+    per_cu = 1;
+  return per_cu * ncu;
+}
+
+template <bool CPLX, int NT>
+static int launch_grid_mfma(const float* vals, float* out, int64_t nbatch,
+                            const InterpGeom& G, const tnl_plan_layout& L,
+                            const char* ws, hipStream_t s) {
+  const BinGeom B = make_bin_geom(L, true);
+  const int* nitems = (const int*)(ws + L.off_itemoff) + L.g_ntiles_total;
+  const int4* items = (const int4*)(ws + L.off_items);
+  grid_zero_shared_kernel<CPLX><<<1024, 256, 0, s>>>(
+      out, nbatch, G, B, items, nitems, (const int*)(ws + L.off_gstarts));
+  TNL_LAUNCH_CHECK("grid_zero_shared");
+  const auto kern = grid_mfma_kernel<CPLX, NT>;
+  if (int rc = allow_lds((const void*)kern, MF_LDS)) return rc;
+  int64_t nwg = resident_workgroups((const void*)kern, MF_THREADS, MF_LDS);
+  nwg = (nwg + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+  const int64_t need = (L.item_capacity + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+  if (nwg > need) nwg = need;
+  kern<<<(unsigned)nwg, MF_THREADS, MF_LDS, s>>>(
+      vals, out, nbatch, G, B, (const int*)(ws + L.off_perm),
+      (const float*)(ws + L.off_slocs), (const int*)(ws + L.off_tlist), items, nitems);
+  TNL_LAUNCH_CHECK("grid_mfma");
+  return TNL_OK;
 }
 
 template <int NDIM, bool CPLX, int CC, int NT>
@@ -903,16 +1212,7 @@ static int launch_binned(bool adjoint, const float* vals, float* out,
     const size_t lds = grid_lds_bytes(B, NP, NT);
     if (int rc = allow_lds((const void*)kern, lds)) return rc;
     // persistent: as many workgroups as can be resident walk the items
-    int per_cu = 0, dev = 0, ncu = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-      hipDeviceProp_t prop;
-      if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
-    }
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kern, GW * NW,
-                                                     lds) != hipSuccess ||
-        per_cu < 1)
-      per_cu = 1;
-    int64_t nwg = (int64_t)per_cu * ncu;
+    int64_t nwg = resident_workgroups((const void*)kern, GW * NW, lds);
     nwg = (nwg + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
     const int64_t need = (L.item_capacity + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
     if (nwg > need) nwg = need;
@@ -954,7 +1254,11 @@ static int dispatch_nt(bool adjoint, const float* vals, float* out,
                        int64_t nbatch, const InterpGeom& G,
                        const tnl_plan_layout& L, const char* ws,
                        hipStream_t s) {
-  const int nt = G.ntap[1] > G.ntap[2] ? G.ntap[1] : G.ntap[2];
+  int nt = G.ntap[0];
+  for (int d = 1; d < 3; ++d) nt = G.ntap[d] > nt ? G.ntap[d] : nt;
+  if (NDIM == 3 && adjoint && use_grid_mfma(G, L))
+    return nt <= 5 ? launch_grid_mfma<CPLX, 5>(vals, out, nbatch, G, L, ws, s)
+                   : launch_grid_mfma<CPLX, 8>(vals, out, nbatch, G, L, ws, s);
   if (nt <= 5) return dispatch_cc<NDIM, CPLX, 5>(adjoint, vals, out, nbatch, G, L, ws, s);
   return dispatch_cc<NDIM, CPLX, 8>(adjoint, vals, out, nbatch, G, L, ws, s);
 }
@@ -1140,18 +1444,6 @@ int batch_build(const InterpGeom* G, const tnl_batch_problem* P,
   if (!rc) rc = check_hip(hipStreamSynchronize(s), "batch table copy");
   std::free(D);
   return rc;
-}
-
-static int resident_workgroups(const void* kern, int threads, size_t lds) {
-  int per_cu = 0, dev = 0, ncu = 256;
-  if (hipGetDevice(&dev) == hipSuccess) {
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
-  }
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) !=
-          hipSuccess || per_cu < 1)
-    per_cu = 1;
-  return per_cu * ncu;
 }
 
 template <int NDIM, bool CPLX, int CC, int NT>

@@ -108,6 +108,17 @@ int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
   L->lds_budget = (tune && tune->lds_budget > 0) ? tune->lds_budget : 53248;
   if (L->lds_budget > 160 * 1024)
     return fail(TNL_EINVAL, "lds_budget above the 160 KiB LDS of a CU");
+  {
+    const int ga = tune ? tune->grid_accum : TNL_GRID_ACCUM_AUTO;
+    if (ga < TNL_GRID_ACCUM_AUTO || ga > TNL_GRID_ACCUM_MFMA)
+      return fail(TNL_EINVAL, "unknown grid_accum");
+    const bool tiles8 = ndim == 3 && L->g_tile[0] == 8 && L->g_tile[1] == 8 && L->g_tile[2] == 8;
+    if (ga == TNL_GRID_ACCUM_MFMA && !tiles8)
+      return fail(TNL_EINVAL, "the MFMA gridding accumulator needs a 3-D plan with 8x8x8 tiles");
+    L->grid_accum = ga == TNL_GRID_ACCUM_AUTO
+                        ? (tiles8 ? TNL_GRID_ACCUM_MFMA : TNL_GRID_ACCUM_LDS)
+                        : ga;
+  }
   L->touch_capacity = npts * touch_per_pt;
   L->item_capacity =
       L->g_ntiles_total + (L->touch_capacity + L->g_cap - 1) / L->g_cap + 1;

@@ -58,7 +58,7 @@ class Tuning(ctypes.Structure):
         ("max_sub", ctypes.c_int32),
         ("g_cap", ctypes.c_int32),
         ("lds_budget", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("grid_accum", ctypes.c_int32),
     ]
 
 
@@ -84,6 +84,7 @@ class PlanLayout(ctypes.Structure):
         ("touch_capacity", ctypes.c_int64),
         ("item_capacity", ctypes.c_int64),
         ("lds_budget", ctypes.c_int64),
+        ("grid_accum", ctypes.c_int64),
         ("off_counts", ctypes.c_int64),
         ("off_starts", ctypes.c_int64),
         ("off_cursor", ctypes.c_int64),

@@ -27,6 +27,9 @@ __all__ = ["Plan", "get_plan", "build_plan", "plan_layout", "clear_plan_cache",
 _MAX_PLANS = 4
 _AUTO_MIN_POINTS = 1 << 16
 
+# gridding accumulators (TNL_GRID_ACCUM_* in tnl.h)
+GRID_ACCUM = {"auto": 0, "lds": 1, "mfma": 2}
+
 # per-ndim tuning overrides (see autotune.py); {} = library defaults
 _tuning: dict = {}
 _tuning_lock = threading.Lock()
@@ -34,9 +37,10 @@ _tuning_lock = threading.Lock()
 
 def set_tuning(ndim: int, **knobs) -> None:
     """Set tiling knobs for ndim-D plans: u_tile, g_tile (3-tuples over the
-    left-padded axes), max_sub, g_cap, lds_budget.  Cached plans are
+    left-padded axes), max_sub, g_cap, lds_budget, grid_accum ("auto",
+    "lds" or "mfma": the 3-D gridding accumulator).  Cached plans are
     dropped so the next call builds with the new tiling."""
-    allowed = {"u_tile", "g_tile", "max_sub", "g_cap", "lds_budget"}
+    allowed = {"u_tile", "g_tile", "max_sub", "g_cap", "lds_budget", "grid_accum"}
     bad = set(knobs) - allowed
     if bad:
         raise ValueError(f"unknown tuning knobs {bad}")
@@ -66,6 +70,11 @@ def _tuning_struct(ndim: int, overrides: Optional[dict] = None) -> Tuning:
     for k in ("max_sub", "g_cap", "lds_budget"):
         if k in knobs:
             setattr(t, k, int(knobs[k]))
+    if "grid_accum" in knobs:
+        mode = knobs["grid_accum"]
+        if mode not in GRID_ACCUM:
+            raise ValueError(f"grid_accum must be one of {sorted(GRID_ACCUM)}, got {mode!r}")
+        t.grid_accum = GRID_ACCUM[mode]
     return t
 
 
