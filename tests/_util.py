@@ -7,8 +7,8 @@ import torch
 
 
 def rel_l2(a, b) -> float:
-    a = np.asarray(a)
-    b = np.asarray(b)
+    a = np.asarray(a).astype(np.complex128)
+    b = np.asarray(b).astype(np.complex128)
     nb = np.linalg.norm(b.ravel())
     return float(np.linalg.norm((a - b).ravel()) / (nb if nb > 0 else 1.0))
 

@@ -797,44 +797,40 @@ constexpr size_t MF_LDS_BATCH = (size_t)MB * (8 + 8 + 8 + MF_NPL) * 4 + MB;
 constexpr size_t MF_LDS_STAGE = (size_t)MF_NPL * MF_PSTR * 4;
 constexpr size_t MF_LDS = MF_LDS_BATCH > MF_LDS_STAGE ? MF_LDS_BATCH : MF_LDS_STAGE;
 
-// 8-cell tile-local window of one axis for the point at coordinate t; bit
-// l of *pres is set when some tap lands on cell l.  Order of the output:
-// plain (W[l] at l) when PERM is false, [W0, W4, W1, W5, W2, W6, W3, W7]
-// when true (so a lane reads the pair {W[j], W[j + 4]} with one b64 load).
+// 8-cell tile-local window of one axis for the point at coordinate t,
+// written to row[0..7]: each tap's weight lands in the slot of the cell it
+// falls on (taps wrap around the grid under circular padding; no two taps
+// share a cell, as the footprint never exceeds the grid), zero elsewhere.  Slot
+// order: plain (cell l at l) when PERM is false, [W0, W4, W1, W5, W2, W6,
+// W3, W7] when true, so a lane reads the pair {W[j], W[j + 4]} with one
+// b64 load.  Returns the mask of cells some tap falls on.
 template <int NT, bool PERM>
-__device__ __forceinline__ void mf_axis_window(float t, int d, int origin,
-                                               const InterpGeom& G, float* out,
-                                               unsigned* pres) {
-  float W[8];
-#pragma unroll
-  for (int l = 0; l < 8; ++l) W[l] = 0.f;
+__device__ __forceinline__ unsigned mf_axis_place(float t, int d, int origin,
+                                                  const InterpGeom& G,
+                                                  float* __restrict__ row) {
+  float4* r4 = reinterpret_cast<float4*>(row);
+  r4[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+  r4[1] = make_float4(0.f, 0.f, 0.f, 0.f);
   unsigned pm = 0;
   const TapRange r = tap_range(t, G.half[d], G.ntap[d], G.n[d], G.pad);
   const int64_t gw = G.pad == TNL_PAD_CIRCULAR ? mod_pos(r.g0, G.n[d]) : r.g0;
   const int lo = (int)(gw - origin);
 #pragma unroll
   for (int k = 0; k < NT; ++k) {
-    if (k < r.count) {
-      int l = lo + k;
-      if (G.pad == TNL_PAD_CIRCULAR && origin + l >= G.n[d]) l -= (int)G.n[d];
-      const float w = window(G.kernel,
-                             fminf(fabsf((float)(r.g0 + k) - t) * G.inv_half[d], 1.0f),
-                             G.beta);
-#pragma unroll
-      for (int c = 0; c < 8; ++c)
-        if (l == c) {
-          W[c] += w;
-          pm |= 1u << c;
-        }
+    int l = lo + k;
+    if (G.pad == TNL_PAD_CIRCULAR && origin + l >= G.n[d]) l -= (int)G.n[d];
+    if (k < r.count && l >= 0 && l < 8) {
+      row[PERM ? 2 * (l & 3) + (l >> 2) : l] =
+          window(G.kernel, fminf(fabsf((float)(r.g0 + k) - t) * G.inv_half[d], 1.0f),
+                 G.beta);
+      pm |= 1u << l;
     }
   }
-#pragma unroll
-  for (int c = 0; c < 8; ++c) out[c] = PERM ? W[(c >> 1) + 4 * (c & 1)] : W[c];
-  *pres = pm;
+  return pm;
 }
 
 template <bool CPLX, int NT>
-__global__ __launch_bounds__(MF_THREADS) void grid_mfma_kernel(
+__global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4))) void grid_mfma_kernel(
     const float* __restrict__ vals, float* __restrict__ out, int64_t nbatch,
     InterpGeom G, BinGeom B, const int* __restrict__ perm,
     const float* __restrict__ slocs, const int* __restrict__ tlist,
@@ -842,6 +838,7 @@ __global__ __launch_bounds__(MF_THREADS) void grid_mfma_kernel(
   using C = Ch<CPLX>;
   using T = typename C::T;
   constexpr int CH = CPLX ? MF_NPL / 2 : MF_NPL;  // channels per chunk
+  static_assert(MB == MF_THREADS && MB / 8 <= 64, "one point per thread; a lane per group");
   extern __shared__ __align__(16) unsigned char lds_raw[];
   float* s_wx = reinterpret_cast<float*>(lds_raw);  // [MB][8]
   float* s_wy = s_wx + MB * 8;                      // [MB][8], permuted
@@ -896,25 +893,18 @@ __global__ __launch_bounds__(MF_THREADS) void grid_mfma_kernel(
         // ---- phase 1, lanes = points
         {
           const bool live = tid < nb;
-          float wx[8], wy[8], wz[8];
           unsigned px = 0, py = 0, pz = 0;
           if (live) {
-            mf_axis_window<NT, false>(pt[0], 0, origin[0], G, wx, &px);
-            mf_axis_window<NT, true>(pt[1], 1, origin[1], G, wy, &py);
-            mf_axis_window<NT, true>(pt[2], 2, origin[2], G, wz, &pz);
+            px = mf_axis_place<NT, false>(pt[0], 0, origin[0], G, s_wx + tid * 8);
+            py = mf_axis_place<NT, true>(pt[1], 1, origin[1], G, s_wy + tid * 8);
+            pz = mf_axis_place<NT, true>(pt[2], 2, origin[2], G, s_wz + tid * 8);
           } else {
-#pragma unroll
-            for (int c = 0; c < 8; ++c) wx[c] = wy[c] = wz[c] = 0.f;
+            const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+            float4* dx = reinterpret_cast<float4*>(s_wx + tid * 8);
+            float4* dy = reinterpret_cast<float4*>(s_wy + tid * 8);
+            float4* dz = reinterpret_cast<float4*>(s_wz + tid * 8);
+            dx[0] = dx[1] = dy[0] = dy[1] = dz[0] = dz[1] = z4;
           }
-          float4* dx = reinterpret_cast<float4*>(s_wx + tid * 8);
-          float4* dy = reinterpret_cast<float4*>(s_wy + tid * 8);
-          float4* dz = reinterpret_cast<float4*>(s_wz + tid * 8);
-          dx[0] = make_float4(wx[0], wx[1], wx[2], wx[3]);
-          dx[1] = make_float4(wx[4], wx[5], wx[6], wx[7]);
-          dy[0] = make_float4(wy[0], wy[1], wy[2], wy[3]);
-          dy[1] = make_float4(wy[4], wy[5], wy[6], wy[7]);
-          dz[0] = make_float4(wz[0], wz[1], wz[2], wz[3]);
-          dz[1] = make_float4(wz[4], wz[5], wz[6], wz[7]);
           float tv[MF_NPL];
 #pragma unroll
           for (int c = 0; c < CH; ++c) {
@@ -946,26 +936,60 @@ __global__ __launch_bounds__(MF_THREADS) void grid_mfma_kernel(
         }
         if (b0 + MB < count) load(b0 + MB, count - b0 - MB < MB ? count - b0 - MB : MB);
         __syncthreads();
-        // ---- phase 2: this wave's x half, every other group of 4 points
-        const int nkg = (nb + 3) >> 2;
-        for (int kg = par; kg < nkg; kg += 2) {
-          const unsigned hm =
-              (__builtin_amdgcn_readfirstlane(s_km[kg]) >> (16 * half)) & 0xFFFFu;
-          if (hm == 0) continue;
-          const int pnt = kg * 4 + k4;
-          const float4 x4 = *reinterpret_cast<const float4*>(s_wx + pnt * 8 + 4 * half);
-          const float2 y2 = *reinterpret_cast<const float2*>(s_wy + pnt * 8 + 2 * yy);
-          const float2 z2 = *reinterpret_cast<const float2*>(s_wz + pnt * 8 + 2 * zz);
-          const float bv = s_v[pnt * MF_NPL + n16];
-          const float wyz[4] = {y2.x * z2.x, y2.x * z2.y, y2.y * z2.x, y2.y * z2.y};
-          const float wxs[4] = {x4.x, x4.y, x4.z, x4.w};
+        // ---- phase 2: this wave's x half, every other group of 4 points.
+        // Lane i holds the block mask of group par + 2 i; the groups with
+        // live blocks are walked from a ballot, and the next group's
+        // operands are read from LDS while the current group's MFMAs run.
+        {
+          const int nkg = (nb + 3) >> 2;
+          const int kgl = par + 2 * lane;
+          const unsigned lm =
+              lane < MB / 8 && kgl < nkg ? (s_km[kgl] >> (16 * half)) & 0xFFFFu : 0u;
+          uint64_t live = __ballot(lm != 0);
+          struct Ops {
+            float4 x4;
+            float2 y2, z2;
+            float bv;
+          };
+          auto fetch = [&](int i) {
+            const int pnt = (par + 2 * i) * 4 + k4;
+            Ops o;
+            o.x4 = *reinterpret_cast<const float4*>(s_wx + pnt * 8 + 4 * half);
+            o.y2 = *reinterpret_cast<const float2*>(s_wy + pnt * 8 + 2 * yy);
+            o.z2 = *reinterpret_cast<const float2*>(s_wz + pnt * 8 + 2 * zz);
+            o.bv = s_v[pnt * MF_NPL + n16];
+            return o;
+          };
+          if (live) {
+            int i = __builtin_ctzll(live);
+            live &= live - 1;
+            Ops cur = fetch(i);
+            unsigned hm = (unsigned)__builtin_amdgcn_readlane((int)lm, i);
+            for (;;) {
+              const bool more = live != 0;
+              Ops nxt = cur;
+              unsigned hn = 0;
+              if (more) {
+                const int j = __builtin_ctzll(live);
+                live &= live - 1;
+                nxt = fetch(j);
+                hn = (unsigned)__builtin_amdgcn_readlane((int)lm, j);
+              }
+              const float wyz[4] = {cur.y2.x * cur.z2.x, cur.y2.x * cur.z2.y,
+                                    cur.y2.y * cur.z2.x, cur.y2.y * cur.z2.y};
+              const float wxs[4] = {cur.x4.x, cur.x4.y, cur.x4.z, cur.x4.w};
 #pragma unroll
-          for (int xl = 0; xl < 4; ++xl)
+              for (int xl = 0; xl < 4; ++xl)
 #pragma unroll
-            for (int p = 0; p < 4; ++p)
-              if ((hm >> (4 * xl + p)) & 1u)
-                acc[4 * xl + p] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                    wxs[xl] * wyz[p], bv, acc[4 * xl + p], 0, 0, 0);
+                for (int p = 0; p < 4; ++p)
+                  if ((hm >> (4 * xl + p)) & 1u)
+                    acc[4 * xl + p] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                        wxs[xl] * wyz[p], cur.bv, acc[4 * xl + p], 0, 0, 0);
+              if (!more) break;
+              cur = nxt;
+              hm = hn;
+            }
+          }
         }
       }
       __syncthreads();  // batch buffers become the staging area
