@@ -226,6 +226,7 @@ def c5_learned_cg(dev, seed: int = 5, train_n: int = 32, train_problems: int = 6
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
     log(f"c5: {train_problems * train_rhs} training solves in {gen_s:.1f} s")
+    torch.manual_seed(5)
     net = Lm.PrecondNet().to(dev)
     t0 = time.perf_counter()
     losses = Lm.train_preconditioner(net, data, steps=train_steps, lr=1e-2, unroll=12)
@@ -388,8 +389,9 @@ def main() -> int:
         # HBM bytes per dispatch from the FETCH_SIZE / WRITE_SIZE passes
         # recorded in profiles/ (rocprofv3 cannot run inside this process)
         tb = json.loads(tfile.read_text())["traffic_bytes"]
-        key = "grid_binned_kernel" if dom == "grid" else "ungrid_binned_kernel"
-        hits = [v for k, v in tb.items() if key in k and "true, 1," not in k]
+        keys = ("grid_mfma_kernel", "grid_binned_kernel") if dom == "grid" else ("ungrid_binned_kernel",)
+        hits = [v for key in keys for k, v in tb.items()
+                if k.startswith("tnl::" + key) and "true, 1," not in k]
         traffic = float(hits[0]) if hits else None
     out = {
         "metric": METRIC,

@@ -804,7 +804,7 @@ constexpr size_t MF_LDS = MF_LDS_BATCH > MF_LDS_STAGE ? MF_LDS_BATCH : MF_LDS_ST
 // order: plain (cell l at l) when PERM is false, [W0, W4, W1, W5, W2, W6,
 // W3, W7] when true, so a lane reads the pair {W[j], W[j + 4]} with one
 // b64 load.  Returns the mask of cells some tap falls on.
-template <int NT, bool PERM>
+template <bool PERM>
 __device__ __forceinline__ unsigned mf_axis_place(float t, int d, int origin,
                                                   const InterpGeom& G,
                                                   float* __restrict__ row) {
@@ -813,15 +813,29 @@ __device__ __forceinline__ unsigned mf_axis_place(float t, int d, int origin,
   r4[1] = make_float4(0.f, 0.f, 0.f, 0.f);
   unsigned pm = 0;
   const TapRange r = tap_range(t, G.half[d], G.ntap[d], G.n[d], G.pad);
-  const int64_t gw = G.pad == TNL_PAD_CIRCULAR ? mod_pos(r.g0, G.n[d]) : r.g0;
-  const int lo = (int)(gw - origin);
-#pragma unroll
-  for (int k = 0; k < NT; ++k) {
+  // The first kept tap wrapped into [0, n) under circular padding: one add
+  // or subtract for coordinates within a grid length of the grid, a 64-bit
+  // modulo only beyond.  Cell indices are 32-bit from here (grid axes are
+  // far below 2^31), and tap k sits at g0f + k, exactly (float)(g0 + k)
+  // while |g0| < 2^24.
+  const int n = (int)G.n[d];
+  int64_t a = r.g0;
+  if (G.pad == TNL_PAD_CIRCULAR) {
+    if (a < 0) a += n;
+    else if (a >= n) a -= n;
+    if (a < 0 || a >= n) a = mod_pos(r.g0, G.n[d]);
+  }
+  const int lo = (int)a - origin;
+  const float g0f = (float)r.g0;
+  // not unrolled: one copy of the window code per axis keeps the kernel
+  // small in the instruction cache
+#pragma unroll 1
+  for (int k = 0; k < r.count; ++k) {
     int l = lo + k;
-    if (G.pad == TNL_PAD_CIRCULAR && origin + l >= G.n[d]) l -= (int)G.n[d];
-    if (k < r.count && l >= 0 && l < 8) {
+    if (G.pad == TNL_PAD_CIRCULAR && origin + l >= n) l -= n;
+    if ((unsigned)l < 8u) {
       row[PERM ? 2 * (l & 3) + (l >> 2) : l] =
-          window(G.kernel, fminf(fabsf((float)(r.g0 + k) - t) * G.inv_half[d], 1.0f),
+          window(G.kernel, fminf(fabsf((g0f + (float)k) - t) * G.inv_half[d], 1.0f),
                  G.beta);
       pm |= 1u << l;
     }
@@ -829,7 +843,7 @@ __device__ __forceinline__ unsigned mf_axis_place(float t, int d, int origin,
   return pm;
 }
 
-template <bool CPLX, int NT>
+template <bool CPLX>
 __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4))) void grid_mfma_kernel(
     const float* __restrict__ vals, float* __restrict__ out, int64_t nbatch,
     InterpGeom G, BinGeom B, const int* __restrict__ perm,
@@ -895,9 +909,9 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
           const bool live = tid < nb;
           unsigned px = 0, py = 0, pz = 0;
           if (live) {
-            px = mf_axis_place<NT, false>(pt[0], 0, origin[0], G, s_wx + tid * 8);
-            py = mf_axis_place<NT, true>(pt[1], 1, origin[1], G, s_wy + tid * 8);
-            pz = mf_axis_place<NT, true>(pt[2], 2, origin[2], G, s_wz + tid * 8);
+            px = mf_axis_place<false>(pt[0], 0, origin[0], G, s_wx + tid * 8);
+            py = mf_axis_place<true>(pt[1], 1, origin[1], G, s_wy + tid * 8);
+            pz = mf_axis_place<true>(pt[2], 2, origin[2], G, s_wz + tid * 8);
           } else {
             const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
             float4* dx = reinterpret_cast<float4*>(s_wx + tid * 8);
@@ -1179,7 +1193,7 @@ static int resident_workgroups(const void* kern, int threads, size_t lds) {
   return per_cu * ncu;
 }
 
-template <bool CPLX, int NT>
+template <bool CPLX>
 static int launch_grid_mfma(const float* vals, float* out, int64_t nbatch,
                             const InterpGeom& G, const tnl_plan_layout& L,
                             const char* ws, hipStream_t s) {
@@ -1189,7 +1203,7 @@ static int launch_grid_mfma(const float* vals, float* out, int64_t nbatch,
   grid_zero_shared_kernel<CPLX><<<1024, 256, 0, s>>>(
       out, nbatch, G, B, items, nitems, (const int*)(ws + L.off_gstarts));
   TNL_LAUNCH_CHECK("grid_zero_shared");
-  const auto kern = grid_mfma_kernel<CPLX, NT>;
+  const auto kern = grid_mfma_kernel<CPLX>;
   if (int rc = allow_lds((const void*)kern, MF_LDS)) return rc;
   int64_t nwg = resident_workgroups((const void*)kern, MF_THREADS, MF_LDS);
   nwg = (nwg + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
@@ -1281,8 +1295,7 @@ static int dispatch_nt(bool adjoint, const float* vals, float* out,
   int nt = G.ntap[0];
   for (int d = 1; d < 3; ++d) nt = G.ntap[d] > nt ? G.ntap[d] : nt;
   if (NDIM == 3 && adjoint && use_grid_mfma(G, L))
-    return nt <= 5 ? launch_grid_mfma<CPLX, 5>(vals, out, nbatch, G, L, ws, s)
-                   : launch_grid_mfma<CPLX, 8>(vals, out, nbatch, G, L, ws, s);
+    return launch_grid_mfma<CPLX>(vals, out, nbatch, G, L, ws, s);
   if (nt <= 5) return dispatch_cc<NDIM, CPLX, 5>(adjoint, vals, out, nbatch, G, L, ws, s);
   return dispatch_cc<NDIM, CPLX, 8>(adjoint, vals, out, nbatch, G, L, ws, s);
 }

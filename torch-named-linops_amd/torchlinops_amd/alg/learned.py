@@ -266,14 +266,32 @@ def pcg_unrolled(A, z: torch.Tensor, d: torch.Tensor, iters: int) -> torch.Tenso
 
 
 def train_preconditioner(net: PrecondNet, data, steps: int = 300, lr: float = 3e-3,
-                         unroll: int = 12, log=None):
+                         unroll: int = 12, log=None, max_grad_norm: float = 1.0,
+                         keep_best: bool = True):
     """data: list of (features [1, F, *im], A, z [R, *im], x* [R, *im])
     with x* = A^-1 z from plain CG.  Minimises the relative error of
-    ``unroll`` preconditioned-CG iterations.  Returns the losses."""
+    ``unroll`` preconditioned-CG iterations (Adam, gradients clipped to
+    ``max_grad_norm``).  The loss through unrolled CG is rough, so with
+    ``keep_best`` the parameters at the start of the pass over ``data``
+    with the lowest mean loss are restored at the end.  Returns the
+    per-step losses."""
     opt = torch.optim.Adam(net.parameters(), lr=lr)
     losses = []
+    nd = len(data)
+    best, best_state, snap = float("inf"), None, None
+
+    def close_pass():
+        nonlocal best, best_state
+        if snap is not None and len(losses) >= nd:
+            pass_loss = sum(losses[-nd:]) / nd
+            if pass_loss < best:
+                best, best_state = pass_loss, snap
+
     for it in range(steps):
-        feats, A, z, x = data[it % len(data)]
+        feats, A, z, x = data[it % nd]
+        if keep_best and it % nd == 0:
+            close_pass()
+            snap = {k: v.detach().clone() for k, v in net.state_dict().items()}
         d = net(feats)[0]
         xk = pcg_unrolled(A, z, d, unroll)
         err = (xk - x).abs() ** 2
@@ -281,10 +299,17 @@ def train_preconditioner(net: PrecondNet, data, steps: int = 300, lr: float = 3e
         loss = (err.sum(dim=red) / (x.abs() ** 2).sum(dim=red)).mean()
         opt.zero_grad()
         loss.backward()
+        if max_grad_norm:
+            torch.nn.utils.clip_grad_norm_(net.parameters(), max_grad_norm)
         opt.step()
         losses.append(float(loss.detach()))
         if log and (it % 50 == 0 or it == steps - 1):
             log(f"train step {it}: loss {losses[-1]:.4f}")
+    if keep_best:
+        if steps % nd == 0:
+            close_pass()
+        if best_state is not None:
+            net.load_state_dict(best_state)
     return losses
 
 
