@@ -974,34 +974,40 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
             o.bv = s_v[pnt * MF_NPL + n16];
             return o;
           };
+          auto group = [&](const Ops& o, unsigned hm) {
+            const float wyz[4] = {o.y2.x * o.z2.x, o.y2.x * o.z2.y, o.y2.y * o.z2.x,
+                                  o.y2.y * o.z2.y};
+            const float wxs[4] = {o.x4.x, o.x4.y, o.x4.z, o.x4.w};
+#pragma unroll
+            for (int xl = 0; xl < 4; ++xl)
+#pragma unroll
+              for (int p = 0; p < 4; ++p)
+                if ((hm >> (4 * xl + p)) & 1u)
+                  acc[4 * xl + p] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                      wxs[xl] * wyz[p], o.bv, acc[4 * xl + p], 0, 0, 0);
+          };
+          // two operand sets in turn: while one group's MFMAs issue, the
+          // other set is being filled (no register copies between them)
+          auto next = [&](int prev, unsigned* hm) {
+            const bool more = live != 0;
+            const int j = more ? __builtin_ctzll(live) : prev;
+            if (more) live &= live - 1;
+            *hm = more ? (unsigned)__builtin_amdgcn_readlane((int)lm, j) : 0u;
+            return j;
+          };
           if (live) {
-            int i = __builtin_ctzll(live);
-            live &= live - 1;
-            Ops cur = fetch(i);
-            unsigned hm = (unsigned)__builtin_amdgcn_readlane((int)lm, i);
+            unsigned ha, hb;
+            int ia = next(0, &ha);
+            Ops oa = fetch(ia);
             for (;;) {
-              const bool more = live != 0;
-              Ops nxt = cur;
-              unsigned hn = 0;
-              if (more) {
-                const int j = __builtin_ctzll(live);
-                live &= live - 1;
-                nxt = fetch(j);
-                hn = (unsigned)__builtin_amdgcn_readlane((int)lm, j);
-              }
-              const float wyz[4] = {cur.y2.x * cur.z2.x, cur.y2.x * cur.z2.y,
-                                    cur.y2.y * cur.z2.x, cur.y2.y * cur.z2.y};
-              const float wxs[4] = {cur.x4.x, cur.x4.y, cur.x4.z, cur.x4.w};
-#pragma unroll
-              for (int xl = 0; xl < 4; ++xl)
-#pragma unroll
-                for (int p = 0; p < 4; ++p)
-                  if ((hm >> (4 * xl + p)) & 1u)
-                    acc[4 * xl + p] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                        wxs[xl] * wyz[p], cur.bv, acc[4 * xl + p], 0, 0, 0);
-              if (!more) break;
-              cur = nxt;
-              hm = hn;
+              const int ib = next(ia, &hb);
+              Ops ob = fetch(ib);
+              group(oa, ha);
+              if (!hb) break;
+              ia = next(ib, &ha);
+              oa = fetch(ia);
+              group(ob, hb);
+              if (!ha) break;
             }
           }
         }
