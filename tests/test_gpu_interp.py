@@ -221,3 +221,24 @@ def test_kb_beta_regimes(dev, ndim, beta):
     locs[:5] = np.round(locs[:5])
     width = 4.0 if ndim == 3 else 6.0
     _run(dev, (n,) * ndim, locs, 2, width, "kaiser_bessel", "circular", 1, beta, "always", True, 42)
+
+
+def test_grid_mfma_order_is_deterministic(dev):
+    """With matrix-core gridding the plan orders each work item's touches
+    by (block mask, point), so the summation order no longer depends on
+    the order of the plan's atomics: two plans built for the same
+    trajectory grid bit-identically (no split tiles here)."""
+    import torchlinops_amd.functional as F
+    from torchlinops_amd import config
+    from torchlinops_amd.functional import _plan
+
+    rng = np.random.default_rng(51)
+    locs = to_dev(rng.uniform(0, 32, (6000, 3)).astype(np.float32), dev)
+    y = to_dev(crandn(rng, (3, 6000)), dev)
+    outs = []
+    for _ in range(2):
+        _plan.clear_plan_cache()
+        with config.using(interp_plan="always"):
+            outs.append(F.interpolate_adjoint(y, locs, (32, 32, 32), 4.0).clone())
+    assert torch.equal(outs[0], outs[1])
+    _plan.clear_plan_cache()

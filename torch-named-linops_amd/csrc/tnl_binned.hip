@@ -773,9 +773,10 @@ __global__ __launch_bounds__(GW * NW) void grid_binned_kernel(
 // A block that none of the 4 points touches is skipped (a wave-uniform
 // mask), so the work follows the footprints rather than the tile volume.
 //
-// Workgroup: 4 waves on one tile.  Waves 0, 1 hold x planes 0-3 (16 blocks,
-// 64 accumulator registers per lane), waves 2, 3 x planes 4-7; the two waves
-// of a pair take alternate groups of 4 points.  Touches are taken in
+// Workgroup: 4 waves on one tile.  Waves 0, 1 hold the even x planes (16
+// blocks, 64 accumulator registers per lane), waves 2, 3 the odd ones, so a
+// footprint (4 or 5 consecutive planes) gives both pairs about the same
+// work; the two waves of a pair take alternate groups of 4 points.  Touches are taken in
 // batches of MB points (one per thread):
 //   1. lanes = points: first taps, the 8-cell tile-local window of every
 //      axis (zero off the footprint; wrapped taps land where they wrap to,
@@ -784,7 +785,7 @@ __global__ __launch_bounds__(GW * NW) void grid_binned_kernel(
 //      batch's raw inputs are loaded right after.
 //   2. waves: per group of 4 points, A = products of the windows, B = the
 //      values, one MFMA per live block.
-// At the end of the tile the two partial sums of each x half meet in LDS
+// At the end of the tile the two partial sums of each x plane meet in LDS
 // (one wave stores, the other adds) and the tile goes to HBM once,
 // coalesced along z.
 typedef float mf32x4 __attribute__((ext_vector_type(4)));
@@ -801,10 +802,11 @@ constexpr size_t MF_LDS = MF_LDS_BATCH > MF_LDS_STAGE ? MF_LDS_BATCH : MF_LDS_ST
 // written to row[0..7]: each tap's weight lands in the slot of the cell it
 // falls on (taps wrap around the grid under circular padding; no two taps
 // share a cell, as the footprint never exceeds the grid), zero elsewhere.  Slot
-// order: plain (cell l at l) when PERM is false, [W0, W4, W1, W5, W2, W6,
-// W3, W7] when true, so a lane reads the pair {W[j], W[j + 4]} with one
-// b64 load.  Returns the mask of cells some tap falls on.
-template <bool PERM>
+// order: [W0, W2, W4, W6, W1, W3, W5, W7] when PAIRS is false (x: a wave
+// reads its four planes of one parity with one b128 load), [W0, W4, W1, W5,
+// W2, W6, W3, W7] when true (y, z: a lane reads the pair {W[j], W[j + 4]}
+// with one b64 load).  Returns the mask of cells some tap falls on.
+template <bool PAIRS>
 __device__ __forceinline__ unsigned mf_axis_place(float t, int d, int origin,
                                                   const InterpGeom& G,
                                                   float* __restrict__ row) {
@@ -834,7 +836,7 @@ __device__ __forceinline__ unsigned mf_axis_place(float t, int d, int origin,
     int l = lo + k;
     if (G.pad == TNL_PAD_CIRCULAR && origin + l >= n) l -= n;
     if ((unsigned)l < 8u) {
-      row[PERM ? 2 * (l & 3) + (l >> 2) : l] =
+      row[PAIRS ? 2 * (l & 3) + (l >> 2) : (l & 1) * 4 + (l >> 1)] =
           window(G.kernel, fminf(fabsf((g0f + (float)k) - t) * G.inv_half[d], 1.0f),
                  G.beta);
       pm |= 1u << l;
@@ -864,7 +866,7 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int half = wv >> 1, par = wv & 1;
+  const int xpar = wv >> 1, par = wv & 1;  // x-plane parity, point-group parity
   const int n16 = lane & 15, k4 = lane >> 4;  // B/D column, A column / D row group
   const int yy = n16 >> 2, zz = n16 & 3;      // A row m = 4 yy + zz
   const T* __restrict__ v = reinterpret_cast<const T*>(vals);
@@ -934,31 +936,28 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
           for (int i = 0; i < MF_NPL / 4; ++i)
             dv[i] = make_float4(tv[4 * i], tv[4 * i + 1], tv[4 * i + 2], tv[4 * i + 3]);
           // blocks touched: bit 4 x + (2 py + pz) for x plane x, (y, z) patch
-          const unsigned yh = ((py & 15u) ? 1u : 0u) | ((py >> 4) ? 2u : 0u);
-          const unsigned zh = ((pz & 15u) ? 1u : 0u) | ((pz >> 4) ? 2u : 0u);
-          const unsigned pm = ((yh & 1u) && (zh & 1u) ? 1u : 0u) |
-                              ((yh & 1u) && (zh & 2u) ? 2u : 0u) |
-                              ((yh & 2u) && (zh & 1u) ? 4u : 0u) |
-                              ((yh & 2u) && (zh & 2u) ? 8u : 0u);
-          unsigned m = 0;
-#pragma unroll
-          for (int x = 0; x < 8; ++x)
-            if ((px >> x) & 1u) m |= pm << (4 * x);
+          unsigned m = tile_blocks32(px, py, pz);
           m |= (unsigned)__shfl_xor((int)m, 1);
           m |= (unsigned)__shfl_xor((int)m, 2);
           if ((tid & 3) == 0) s_km[tid >> 2] = m;
         }
         if (b0 + MB < count) load(b0 + MB, count - b0 - MB < MB ? count - b0 - MB : MB);
         __syncthreads();
-        // ---- phase 2: this wave's x half, every other group of 4 points.
+        // ---- phase 2: this wave's x planes (one parity), every other group of
+        // 4 points.
         // Lane i holds the block mask of group par + 2 i; the groups with
         // live blocks are walked from a ballot, and the next group's
         // operands are read from LDS while the current group's MFMAs run.
         {
           const int nkg = (nb + 3) >> 2;
           const int kgl = par + 2 * lane;
-          const unsigned lm =
-              lane < MB / 8 && kgl < nkg ? (s_km[kgl] >> (16 * half)) & 0xFFFFu : 0u;
+          // this wave's blocks of the group: x planes xpar, xpar + 2, ...
+          unsigned lm = 0;
+          if (lane < MB / 8 && kgl < nkg) {
+            const unsigned mm = s_km[kgl];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) lm |= ((mm >> (4 * (2 * j + xpar))) & 15u) << (4 * j);
+          }
           uint64_t live = __ballot(lm != 0);
           struct Ops {
             float4 x4;
@@ -968,7 +967,7 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
           auto fetch = [&](int i) {
             const int pnt = (par + 2 * i) * 4 + k4;
             Ops o;
-            o.x4 = *reinterpret_cast<const float4*>(s_wx + pnt * 8 + 4 * half);
+            o.x4 = *reinterpret_cast<const float4*>(s_wx + pnt * 8 + 4 * xpar);
             o.y2 = *reinterpret_cast<const float2*>(s_wy + pnt * 8 + 2 * yy);
             o.z2 = *reinterpret_cast<const float2*>(s_wz + pnt * 8 + 2 * zz);
             o.bv = s_v[pnt * MF_NPL + n16];
@@ -1016,7 +1015,7 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
       // ---- combine the pair's partial sums: lane holds cells
       // (x, 4 py + k4, 4 pz + i), i < 4, of plane n16
       auto stage_at = [&](int xl, int p) {
-        const int x = 4 * half + xl, y = 4 * (p >> 1) + k4, z0 = 4 * (p & 1);
+        const int x = 2 * xl + xpar, y = 4 * (p >> 1) + k4, z0 = 4 * (p & 1);
         return reinterpret_cast<float4*>(stage + n16 * MF_PSTR + x * 64 + y * 8 + z0);
       };
       if (par == 0) {

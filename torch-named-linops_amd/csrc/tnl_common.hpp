@@ -196,4 +196,41 @@ __device__ __forceinline__ TapRange tap_range(float t, float half, int ntap,
   return r;
 }
 
+// Cells [0, 8) of a tile starting at cell `origin` of an axis of n cells
+// that the taps of coordinate t fall on (bit l = cell origin + l; taps wrap
+// under circular padding).  The tile-local footprint of one axis.
+__device__ __forceinline__ unsigned tile_cells8(float t, float half, int ntap,
+                                                int64_t n, int pad, int origin) {
+  const TapRange r = tap_range(t, half, ntap, n, pad);
+  int64_t a = r.g0;
+  if (pad == TNL_PAD_CIRCULAR) {
+    if (a < 0) a += n;
+    else if (a >= n) a -= n;
+    if (a < 0 || a >= n) a = mod_pos(r.g0, n);
+  }
+  const int lo = (int)a - origin;
+  unsigned m = 0;
+  for (int k = 0; k < r.count; ++k) {
+    int l = lo + k;
+    if (pad == TNL_PAD_CIRCULAR && origin + l >= n) l -= (int)n;
+    if ((unsigned)l < 8u) m |= 1u << l;
+  }
+  return m;
+}
+
+// Blocks of an 8x8x8 tile that a footprint with tile-local cell masks
+// (mx, my, mz) touches: bit 4 x + 2 py + pz for x plane x and the 4x4
+// (y, z) patch (py, pz) (cells 4 py .. 4 py + 3 along y, likewise z).
+__device__ __forceinline__ unsigned tile_blocks32(unsigned mx, unsigned my, unsigned mz) {
+  const unsigned yh = ((my & 15u) ? 1u : 0u) | ((my >> 4) ? 2u : 0u);
+  const unsigned zh = ((mz & 15u) ? 1u : 0u) | ((mz >> 4) ? 2u : 0u);
+  const unsigned pm = ((yh & 1u) && (zh & 1u) ? 1u : 0u) | ((yh & 1u) && (zh & 2u) ? 2u : 0u) |
+                      ((yh & 2u) && (zh & 1u) ? 4u : 0u) | ((yh & 2u) && (zh & 2u) ? 8u : 0u);
+  unsigned m = 0;
+#pragma unroll
+  for (int x = 0; x < 8; ++x)
+    if ((mx >> x) & 1u) m |= pm << (4 * x);
+  return m;
+}
+
 }  // namespace tnl

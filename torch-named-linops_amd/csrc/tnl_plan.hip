@@ -365,6 +365,80 @@ __global__ void item_fill_kernel(const int* __restrict__ gcounts,
   }
 }
 
+// Matrix-core gridding (3-D, 8x8x8 tiles) accumulates blocks of 16 cells
+// for groups of 4 consecutive touches of a work item and skips the blocks
+// no touch of the group reaches.  Ordering each item's touches by their
+// block mask puts touches with equal masks side by side, so a group's
+// union of blocks is close to one touch's.  Ties go by the point's index in
+// the caller's order (not its sorted index, which the atomics of the point
+// scatter decide), so the summation order of an item is the same for every
+// plan of a trajectory.  One workgroup per item: bitonic sort of
+// (mask << 32 | original index) keys in LDS, each carrying its sorted
+// index; items above SORT_CAP touches (only with a g_cap tuned above it)
+// keep the scatter order.
+constexpr int SORT_CAP = 4096;
+
+__global__ __launch_bounds__(256) void item_sort_kernel(const int4* __restrict__ items,
+                                                        const int* __restrict__ nitems_ptr,
+                                                        const float* __restrict__ slocs,
+                                                        const int* __restrict__ perm,
+                                                        PlanDev P, int* __restrict__ tlist) {
+  __shared__ unsigned long long key[SORT_CAP];
+  __shared__ int val[SORT_CAP];
+  const int nitems = *nitems_ptr;
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const int4 item = items[it];
+    const int start = item.y, count = item.z;
+    if (count < 2 || count > SORT_CAP) continue;
+    int origin[3];
+    {
+      int r = item.x;
+      for (int d = 2; d >= 0; --d) {
+        origin[d] = (r % P.gntiles[d]) * P.gtile[d];
+        r /= P.gntiles[d];
+      }
+    }
+    int n2 = 2;
+    while (n2 < count) n2 <<= 1;
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+      unsigned long long k = ~0ull;
+      int q = 0;
+      if (i < count) {
+        q = tlist[start + i];
+        unsigned m[3];
+        for (int d = 0; d < 3; ++d)
+          m[d] = tile_cells8(slocs[(int64_t)q * 3 + d], P.half[d], P.ntap[d], P.n[d], P.pad,
+                             origin[d]);
+        k = ((unsigned long long)tile_blocks32(m[0], m[1], m[2]) << 32) | (unsigned)perm[q];
+      }
+      key[i] = k;
+      val[i] = q;
+    }
+    __syncthreads();
+    for (int k = 2; k <= n2; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+          const int l = i ^ j;
+          if (l > i) {
+            const unsigned long long a = key[i], b = key[l];
+            const bool up = (i & k) == 0;
+            if ((a > b) == up) {
+              key[i] = b;
+              key[l] = a;
+              const int t = val[i];
+              val[i] = val[l];
+              val[l] = t;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int i = threadIdx.x; i < count; i += blockDim.x) tlist[start + i] = val[i];
+    __syncthreads();
+  }
+}
+
 PlanDev make_plan_dev(const tnl_plan_layout& L) {
   PlanDev P;
   std::memset(&P, 0, sizeof(P));
@@ -479,5 +553,11 @@ extern "C" int tnl_plan_build(const float* locs, const tnl_plan_layout* layout,
   item_fill_kernel<<<grid_for(L.g_ntiles_total), PLAN_THREADS, 0, s>>>(
       gcounts, gstarts, itemoff, P, items);
   TNL_LAUNCH_CHECK("plan items");
+  if (npts > 0 && L.grid_accum == TNL_GRID_ACCUM_MFMA) {
+    const int64_t nwg = L.item_capacity < 8192 ? L.item_capacity : 8192;
+    item_sort_kernel<<<(unsigned)nwg, 256, 0, s>>>(items, itemoff + L.g_ntiles_total, slocs,
+                                                    perm, P, tlist);
+    TNL_LAUNCH_CHECK("plan item sort");
+  }
   return TNL_OK;
 }
