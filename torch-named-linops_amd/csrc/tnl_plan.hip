@@ -365,6 +365,110 @@ __global__ void item_fill_kernel(const int* __restrict__ gcounts,
   }
 }
 
+// Interpolation gathers with lane = point: in one LDS instruction every lane
+// reads its point's window row at the same tap offset, so two lanes whose
+// rows start in the same bank (complex cells: 8-byte words, 32 bank pairs)
+// serialise.  Each subproblem's points are reordered so that consecutive
+// points start in different banks: by rank within their bank class (rank
+// by original index), then by class.  The order depends only on the
+// trajectory, so the plan stays deterministic.  One workgroup per
+// subproblem; subproblems above ORDER_CAP points (a max_sub tuned above
+// it) keep the bin order.
+constexpr int ORDER_CAP = 2048;
+
+__global__ __launch_bounds__(256) void sub_bank_order_kernel(const int4* __restrict__ sub,
+                                                             const int* __restrict__ nsub_ptr,
+                                                             PlanDev P, int* __restrict__ perm,
+                                                             float* __restrict__ slocs) {
+  __shared__ unsigned long long key[ORDER_CAP];
+  __shared__ float s_loc[ORDER_CAP * 3];
+  __shared__ int s_perm[ORDER_CAP];
+  __shared__ int ccount[32], cstart[32];
+  const int nsub = *nsub_ptr;
+  int win[3];
+  for (int d = 0; d < 3; ++d) win[d] = P.utile[d] + P.ntap[d] - 1;
+  for (int si = blockIdx.x; si < nsub; si += gridDim.x) {
+    const int4 sp = sub[si];
+    const int bin = sp.x, start = sp.y, count = sp.z;
+    if (bin >= P.u_nbins || count < 2 || count > ORDER_CAP) continue;
+    int origin[3];
+    {
+      int r = bin;
+      for (int d = 2; d >= 0; --d) {
+        origin[d] = (r % P.untiles[d]) * P.utile[d];
+        r /= P.untiles[d];
+      }
+    }
+    if (threadIdx.x < 32) ccount[threadIdx.x] = 0;
+    __syncthreads();
+    int n2 = 2;
+    while (n2 < count) n2 <<= 1;
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+      unsigned long long k = ~0ull;
+      if (i < count) {
+        const int64_t q = start + i;
+        int lo[3] = {0, 0, 0};
+        for (int d = 3 - P.ndim; d < 3; ++d) {
+          const float t = slocs[q * P.ndim + d - (3 - P.ndim)];
+          s_loc[i * 3 + d] = t;
+          const TapRange r = tap_range(t, P.half[d], P.ntap[d], P.n[d], P.pad);
+          const int64_t gw = P.pad == TNL_PAD_CIRCULAR ? mod_pos(r.g0, P.n[d]) : r.g0;
+          lo[d] = (int)(gw - origin[d]);
+        }
+        const int pid = perm[q];
+        s_perm[i] = pid;
+        const unsigned cls = (unsigned)((lo[0] * win[1] + lo[1]) * win[2] + lo[2]) & 31u;
+        atomicAdd(&ccount[cls], 1);
+        k = ((unsigned long long)cls << 43) | ((unsigned long long)(unsigned)pid << 11) |
+            (unsigned)i;
+      }
+      key[i] = k;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int acc = 0;
+      for (int c = 0; c < 32; ++c) {
+        cstart[c] = acc;
+        acc += ccount[c];
+      }
+    }
+    // sort by (class, original index), then re-key by (rank, class)
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int k = 2; k <= n2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+            const int l = i ^ j;
+            if (l > i) {
+              const unsigned long long a = key[i], b = key[l];
+              if ((a > b) == ((i & k) == 0)) {
+                key[i] = b;
+                key[l] = a;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+      if (pass == 0) {
+        for (int i = threadIdx.x; i < count; i += blockDim.x) {
+          const unsigned long long k = key[i];
+          const int cls = (int)(k >> 43);
+          const unsigned long long rank = (unsigned long long)(i - cstart[cls]);
+          key[i] = (rank << 16) | ((unsigned long long)cls << 11) | (k & 2047ull);
+        }
+        __syncthreads();
+      }
+    }
+    for (int j = threadIdx.x; j < count; j += blockDim.x) {
+      const int i = (int)(key[j] & 2047ull);
+      const int64_t q = start + j;
+      perm[q] = s_perm[i];
+      for (int d = 3 - P.ndim; d < 3; ++d) slocs[q * P.ndim + d - (3 - P.ndim)] = s_loc[i * 3 + d];
+    }
+    __syncthreads();
+  }
+}
+
 // Matrix-core gridding (3-D, 8x8x8 tiles) accumulates blocks of 16 cells
 // for groups of 4 consecutive touches of a work item and skips the blocks
 // no touch of the group reaches.  Ordering each item's touches by their
@@ -542,6 +646,12 @@ extern "C" int tnl_plan_build(const float* locs, const tnl_plan_layout* layout,
   sub_fill_kernel<<<grid_for(nb1), PLAN_THREADS, 0, s>>>(
       counts, starts, suboff, P, sub);
   TNL_LAUNCH_CHECK("plan subproblems");
+  if (npts > 0) {
+    const int64_t nwg = L.sub_capacity < 8192 ? L.sub_capacity : 8192;
+    sub_bank_order_kernel<<<(unsigned)nwg, 256, 0, s>>>(sub, suboff + L.u_nbins + 1, P, perm,
+                                                         slocs);
+    TNL_LAUNCH_CHECK("plan subproblem order");
+  }
   scan_kernel<<<1, 1024, 0, s>>>(gcounts, gstarts, L.g_ntiles_total, 1, 0);
   if (npts > 0) {
     touch_kernel<true><<<grid_for(npts), PLAN_THREADS, 0, s>>>(
