@@ -140,6 +140,29 @@ int tnl_grid(const float* vals, const float* locs, float* out, int64_t nbatch,
              int is_complex, float beta, const tnl_plan_layout* layout,
              const void* plan, void* stream);
 
+/* Scratch bytes that tnl_grid_ws uses with this plan for nbatch channels:
+ * the channel values re-packed point-major, one 64-byte record (up to 8
+ * complex / 16 real channel values) per point and channel chunk, for the
+ * matrix-core gridding kernel (3-D plans with 8x8x8 tiles); 0 for other
+ * plans (tnl_grid_ws then equals tnl_grid). */
+int tnl_grid_scratch_bytes(const tnl_plan_layout* layout, int64_t nbatch,
+                           int is_complex, int64_t* bytes);
+
+/* tnl_grid with a caller-owned device scratch buffer of at least
+ * tnl_grid_scratch_bytes bytes (its contents on entry are ignored and
+ * overwritten).  With the records each touch of a gridding tile reads one
+ * contiguous record instead of its channel values scattered over the
+ * coil-major input.  No counterpart in the reference: its Triton kernel
+ * reads the channel values of each point where they lie
+ * (src/torchlinops/functional/_interp/grid.py:161-513).  Same results as
+ * tnl_grid. */
+int tnl_grid_ws(const float* vals, const float* locs, float* out, int64_t nbatch,
+                int64_t npts, int ndim, const int64_t* grid_size,
+                const float* width, int kernel, int norm, int pad_mode,
+                int is_complex, float beta, const tnl_plan_layout* layout,
+                const void* plan, void* scratch, int64_t scratch_bytes,
+                void* stream);
+
 /* ---- many problems in one persistent launch ----------------------------
  * A batch is a set of independent interpolation problems of one ndim,
  * kernel, norm, pad_mode and dtype, each with its own grid size, width,

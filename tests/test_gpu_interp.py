@@ -242,3 +242,48 @@ def test_grid_mfma_order_is_deterministic(dev):
             outs.append(F.interpolate_adjoint(y, locs, (32, 32, 32), 4.0).clone())
     assert torch.equal(outs[0], outs[1])
     _plan.clear_plan_cache()
+
+
+@pytest.mark.parametrize("cplx,nbatch", [(True, 3), (True, 11), (False, 17)])
+def test_grid_ws_records_match_direct_reads(dev, cplx, nbatch):
+    """tnl_grid_ws (values re-packed point-major into the caller's scratch)
+    gives bit-identical results to tnl_grid (values read from the
+    coil-major input) through the C ABI: same kernel, same summation
+    order (no split tiles here); a scratch below tnl_grid_scratch_bytes is
+    rejected."""
+    import ctypes
+    from torchlinops_amd import _lib
+    from torchlinops_amd.functional import _plan
+
+    rng = np.random.default_rng(61)
+    n = (24, 32, 40)
+    locs = to_dev(rng.uniform(-2, np.array(n) + 2, (5000, 3)).astype(np.float32), dev)
+    y = crandn(rng, (nbatch, 5000)) if cplx else rng.standard_normal((nbatch, 5000)).astype(np.float32)
+    yv = to_dev(y, dev)
+    yr = torch.view_as_real(yv) if cplx else yv
+    plan = _plan.build_plan(locs, 3, n, (4.0,) * 3, 0)
+    assert plan.layout.grid_accum == 2
+    h = _lib.lib()
+    nbytes = ctypes.c_int64(0)
+    _lib.check(h.tnl_grid_scratch_bytes(plan.layout, nbatch, int(cplx), ctypes.byref(nbytes)))
+    ch = 8 if cplx else 16
+    assert nbytes.value == -(-nbatch // ch) * 5000 * 64
+    outs = []
+    for scratch in (None, torch.empty(nbytes.value, dtype=torch.uint8, device=dev)):
+        out = torch.empty(nbatch, *n, dtype=yv.dtype, device=dev)
+        outr = torch.view_as_real(out) if cplx else out
+        args = [yr.data_ptr(), locs.data_ptr(), outr.data_ptr(), nbatch, 5000, 3,
+                _lib.i64_array(n), _lib.f32_array((4.0,) * 3), 0, 1, 0, int(cplx), 1.7,
+                plan.layout, plan.ptr]
+        if scratch is None:
+            _lib.check(h.tnl_grid(*args, _lib.stream_of(yv)))
+        else:
+            _lib.check(h.tnl_grid_ws(*args, scratch.data_ptr(), nbytes.value, _lib.stream_of(yv)))
+        torch.cuda.synchronize()
+        outs.append(out)
+    assert torch.equal(outs[0], outs[1])
+    ref = O.grid(y, locs.cpu().numpy(), n, 4.0, "kaiser_bessel", 1, "circular", 1.7, "f64")
+    assert rel_l2(outs[1].cpu().numpy(), ref) < TOL
+    small = torch.empty(nbytes.value - 16, dtype=torch.uint8, device=dev)
+    with pytest.raises(ValueError, match="scratch"):
+        _lib.check(h.tnl_grid_ws(*args, small.data_ptr(), small.numel(), _lib.stream_of(yv)))

@@ -79,7 +79,8 @@ static int interp_entry(bool adjoint, const float* vals, const float* locs,
                         const int64_t* grid_size, const float* width,
                         int kernel, int norm, int pad_mode, int is_complex,
                         float beta, const tnl_plan_layout* layout,
-                        const void* plan, void* stream) {
+                        const void* plan, void* scratch, int64_t scratch_bytes,
+                        void* stream) {
   InterpGeom G;
   if (int rc = make_geom(ndim, grid_size, width, kernel, norm, pad_mode,
                          is_complex, beta, G))
@@ -107,9 +108,14 @@ static int interp_entry(bool adjoint, const float* vals, const float* locs,
     if (!same)
       return fail(TNL_EINVAL,
                   "plan was built for another trajectory size, grid, width or pad_mode");
-    if (binned_supported(G, L, adjoint))
-      return interp_binned(adjoint, vals, out, nbatch, G, is_complex, L, plan,
-                           s);
+    if (binned_supported(G, L, adjoint)) {
+      if (scratch) {
+        if (scratch_bytes < grid_scratch_bytes(L, nbatch, is_complex))
+          return fail(TNL_EINVAL, "scratch smaller than tnl_grid_scratch_bytes");
+      }
+      return interp_binned(adjoint, vals, locs, out, nbatch, G, is_complex, L, plan,
+                           (float*)scratch, s);
+    }
   }
   if (adjoint) {
     if (int rc = fill_zero(out, nbatch * G.ncells * ncomp, s)) return rc;
@@ -134,7 +140,7 @@ extern "C" int tnl_ungrid(const float* vals, const float* locs, float* out,
                           const void* plan, void* stream) {
   return interp_entry(false, vals, locs, out, nbatch, npts, ndim, grid_size,
                       width, kernel, norm, pad_mode, is_complex, beta, layout,
-                      plan, stream);
+                      plan, nullptr, 0, stream);
 }
 
 extern "C" int tnl_grid(const float* vals, const float* locs, float* out,
@@ -145,7 +151,28 @@ extern "C" int tnl_grid(const float* vals, const float* locs, float* out,
                         const void* plan, void* stream) {
   return interp_entry(true, vals, locs, out, nbatch, npts, ndim, grid_size,
                       width, kernel, norm, pad_mode, is_complex, beta, layout,
-                      plan, stream);
+                      plan, nullptr, 0, stream);
+}
+
+extern "C" int tnl_grid_scratch_bytes(const tnl_plan_layout* layout, int64_t nbatch,
+                                      int is_complex, int64_t* bytes) {
+  if (!layout || !bytes) return fail(TNL_EINVAL, "null pointer");
+  if (nbatch < 0) return fail(TNL_EINVAL, "nbatch must be >= 0");
+  *bytes = grid_scratch_bytes(*layout, nbatch, is_complex);
+  return TNL_OK;
+}
+
+extern "C" int tnl_grid_ws(const float* vals, const float* locs, float* out,
+                           int64_t nbatch, int64_t npts, int ndim,
+                           const int64_t* grid_size, const float* width,
+                           int kernel, int norm, int pad_mode, int is_complex,
+                           float beta, const tnl_plan_layout* layout,
+                           const void* plan, void* scratch, int64_t scratch_bytes,
+                           void* stream) {
+  if (scratch_bytes < 0) return fail(TNL_EINVAL, "scratch_bytes must be >= 0");
+  return interp_entry(true, vals, locs, out, nbatch, npts, ndim, grid_size,
+                      width, kernel, norm, pad_mode, is_complex, beta, layout,
+                      plan, scratch_bytes > 0 ? scratch : nullptr, scratch_bytes, stream);
 }
 
 extern "C" int tnl_interp_grad_locs(const float* grid_vals, const float* pt_vals,

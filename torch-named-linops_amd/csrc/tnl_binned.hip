@@ -794,7 +794,8 @@ constexpr int MF_THREADS = 256;
 constexpr int MB = 256;          // points per batch (one per thread)
 constexpr int MF_PSTR = 516;     // staging plane stride (floats): 512 + 4
 constexpr int MF_NPL = 16;       // accumulator planes
-constexpr size_t MF_LDS_BATCH = (size_t)MB * (8 + 8 + 8 + MF_NPL) * 4 + MB;
+constexpr int MF_WROW = 8;       // floats per point row of a window
+constexpr size_t MF_LDS_BATCH = (size_t)MB * (3 * MF_WROW + MF_NPL) * 4 + MB;
 constexpr size_t MF_LDS_STAGE = (size_t)MF_NPL * MF_PSTR * 4;
 constexpr size_t MF_LDS = MF_LDS_BATCH > MF_LDS_STAGE ? MF_LDS_BATCH : MF_LDS_STAGE;
 
@@ -845,21 +846,60 @@ __device__ __forceinline__ unsigned mf_axis_place(float t, int d, int origin,
   return pm;
 }
 
+// Point-major record of one point and channel chunk for the matrix-core
+// gridding kernel: the chunk's 16 planes (re/im of 8 coils, or 16 real
+// channels; zero past nbatch), 64 bytes, in the caller's point order (a
+// plain transpose of the coil-major input: both sides coalesced).
+constexpr int MF_REC4 = 4;  // float4 per record
+
 template <bool CPLX>
+__global__ void pack_points_kernel(const float* __restrict__ vals, int64_t npts,
+                                   int64_t nbatch, float4* __restrict__ rec) {
+  constexpr int CH = CPLX ? MF_NPL / 2 : MF_NPL;
+  const int64_t nch = (nbatch + CH - 1) / CH;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npts;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t ch = 0; ch < nch; ++ch) {
+      float f[MF_NPL];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int64_t cg = ch * CH + c;
+        if constexpr (CPLX) {
+          const float2 x = cg < nbatch ? reinterpret_cast<const float2*>(vals)[cg * npts + p]
+                                       : make_float2(0.f, 0.f);
+          f[2 * c] = x.x;
+          f[2 * c + 1] = x.y;
+        } else {
+          f[c] = cg < nbatch ? vals[cg * npts + p] : 0.f;
+        }
+      }
+      float4* r = rec + (ch * npts + p) * MF_REC4;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        r[i] = make_float4(f[4 * i], f[4 * i + 1], f[4 * i + 2], f[4 * i + 3]);
+    }
+  }
+}
+
+// REC: the points' values come from the point-major records (rec, caller
+// order) and their coordinates from the caller's locs, instead of the
+// coil-major input and slocs.
+template <bool CPLX, bool REC>
 __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4))) void grid_mfma_kernel(
     const float* __restrict__ vals, float* __restrict__ out, int64_t nbatch,
     InterpGeom G, BinGeom B, const int* __restrict__ perm,
     const float* __restrict__ slocs, const int* __restrict__ tlist,
-    const int4* __restrict__ items, const int* __restrict__ nitems_ptr) {
+    const int4* __restrict__ items, const int* __restrict__ nitems_ptr,
+    const float* __restrict__ locs, const float4* __restrict__ rec) {
   using C = Ch<CPLX>;
   using T = typename C::T;
   constexpr int CH = CPLX ? MF_NPL / 2 : MF_NPL;  // channels per chunk
   static_assert(MB == MF_THREADS && MB / 8 <= 64, "one point per thread; a lane per group");
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  float* s_wx = reinterpret_cast<float*>(lds_raw);  // [MB][8]
-  float* s_wy = s_wx + MB * 8;                      // [MB][8], permuted
-  float* s_wz = s_wy + MB * 8;                      // [MB][8], permuted
-  float* s_v = s_wz + MB * 8;                       // [MB][16]
+  float* s_wx = reinterpret_cast<float*>(lds_raw);  // [MB][MF_WROW], x order
+  float* s_wy = s_wx + MB * MF_WROW;                // [MB][MF_WROW], pair order
+  float* s_wz = s_wy + MB * MF_WROW;                // [MB][MF_WROW], pair order
+  float* s_v = s_wz + MB * MF_WROW;                 // [MB][16]
   unsigned* s_km = reinterpret_cast<unsigned*>(s_v + MB * MF_NPL);  // [MB / 4]
   float* stage = reinterpret_cast<float*>(lds_raw);  // [16][MF_PSTR], after batches
 
@@ -872,10 +912,57 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
   const T* __restrict__ v = reinterpret_cast<const T*>(vals);
   const int nitems = *nitems_ptr;
 
-  for (int64_t it = xcd_remap(blockIdx.x, gridDim.x); it < nitems; it += gridDim.x) {
-    const int4 item = items[it];
+  // raw inputs of this thread's point of a batch, loaded one batch ahead
+  // (across channel chunks and work items too)
+  float pt[3] = {0.f, 0.f, 0.f};
+  T pv[CH];
+  auto load = [&](int start, int64_t c0, int b0, int nb) {
+    if (tid < nb) {
+      const int q = tlist[start + b0 + tid];
+      if constexpr (REC) {
+        const int64_t p = perm[q];
+        const float4* r = rec + ((c0 / CH) * B.npts + p) * MF_REC4;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) pt[d] = locs[p * 3 + d];
+        float f[MF_NPL];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float4 x = r[i];
+          f[4 * i] = x.x; f[4 * i + 1] = x.y; f[4 * i + 2] = x.z; f[4 * i + 3] = x.w;
+        }
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          if constexpr (CPLX) pv[c] = make_float2(f[2 * c], f[2 * c + 1]);
+          else pv[c] = f[c];
+        }
+      } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) pt[d] = slocs[(int64_t)q * 3 + d];
+        const int p = perm[q];
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+          pv[c] = c0 + c < nbatch ? v[(c0 + c) * B.npts + p] : C::zero();
+      }
+    }
+  };
+  int64_t it = xcd_remap(blockIdx.x, gridDim.x);
+  if (it >= nitems) return;
+  int4 item = items[it];
+  int64_t c0 = 0;
+  load(item.y, 0, 0, item.z < MB ? item.z : MB);
+  for (;;) {
     const int start = item.y, count = item.z;
     const bool shared = item.w != 0;
+    const int nc = (int)(nbatch - c0 < CH ? nbatch - c0 : CH);
+    // the next (item, channel chunk) of this workgroup
+    int64_t it_n = it, c0_n = c0 + CH;
+    int4 item_n = item;
+    if (c0_n >= nbatch) {
+      c0_n = 0;
+      it_n = it + gridDim.x;
+      if (it_n < nitems) item_n = items[it_n];
+    }
+    const bool has_next = it_n < nitems;
     int origin[3];
     {
       int r = item.x;
@@ -884,25 +971,11 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
         r /= B.ntiles[d];
       }
     }
-    for (int64_t c0 = 0; c0 < nbatch; c0 += CH) {
-      const int nc = (int)(nbatch - c0 < CH ? nbatch - c0 : CH);
+    {
       mf32x4 acc[16];
 #pragma unroll
       for (int b = 0; b < 16; ++b) acc[b] = (mf32x4){0.f, 0.f, 0.f, 0.f};
-      // raw inputs of this thread's point of a batch
-      float pt[3] = {0.f, 0.f, 0.f};
-      T pv[CH];
-      auto load = [&](int b0, int nb) {
-        if (tid < nb) {
-          const int q = tlist[start + b0 + tid];
-#pragma unroll
-          for (int d = 0; d < 3; ++d) pt[d] = slocs[(int64_t)q * 3 + d];
-          const int p = perm[q];
-#pragma unroll
-          for (int c = 0; c < CH; ++c) pv[c] = c < nc ? v[(c0 + c) * B.npts + p] : C::zero();
-        }
-      };
-      load(0, count < MB ? count : MB);
+      bool ahead = false;  // the next unit's first batch is loaded
       for (int b0 = 0; b0 < count; b0 += MB) {
         const int nb = count - b0 < MB ? count - b0 : MB;
         __syncthreads();  // previous batch's MFMA reads (or the write-out) are done
@@ -911,14 +984,14 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
           const bool live = tid < nb;
           unsigned px = 0, py = 0, pz = 0;
           if (live) {
-            px = mf_axis_place<false>(pt[0], 0, origin[0], G, s_wx + tid * 8);
-            py = mf_axis_place<true>(pt[1], 1, origin[1], G, s_wy + tid * 8);
-            pz = mf_axis_place<true>(pt[2], 2, origin[2], G, s_wz + tid * 8);
+            px = mf_axis_place<false>(pt[0], 0, origin[0], G, s_wx + tid * MF_WROW);
+            py = mf_axis_place<true>(pt[1], 1, origin[1], G, s_wy + tid * MF_WROW);
+            pz = mf_axis_place<true>(pt[2], 2, origin[2], G, s_wz + tid * MF_WROW);
           } else {
             const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-            float4* dx = reinterpret_cast<float4*>(s_wx + tid * 8);
-            float4* dy = reinterpret_cast<float4*>(s_wy + tid * 8);
-            float4* dz = reinterpret_cast<float4*>(s_wz + tid * 8);
+            float4* dx = reinterpret_cast<float4*>(s_wx + tid * MF_WROW);
+            float4* dy = reinterpret_cast<float4*>(s_wy + tid * MF_WROW);
+            float4* dz = reinterpret_cast<float4*>(s_wz + tid * MF_WROW);
             dx[0] = dx[1] = dy[0] = dy[1] = dz[0] = dz[1] = z4;
           }
           float tv[MF_NPL];
@@ -941,7 +1014,12 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
           m |= (unsigned)__shfl_xor((int)m, 2);
           if ((tid & 3) == 0) s_km[tid >> 2] = m;
         }
-        if (b0 + MB < count) load(b0 + MB, count - b0 - MB < MB ? count - b0 - MB : MB);
+        if (b0 + MB < count) {
+          load(start, c0, b0 + MB, count - b0 - MB < MB ? count - b0 - MB : MB);
+        } else if (has_next) {
+          load(item_n.y, c0_n, 0, item_n.z < MB ? item_n.z : MB);
+          ahead = true;
+        }
         __syncthreads();
         // ---- phase 2: this wave's x planes (one parity), every other group of
         // 4 points.
@@ -967,9 +1045,9 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
           auto fetch = [&](int i) {
             const int pnt = (par + 2 * i) * 4 + k4;
             Ops o;
-            o.x4 = *reinterpret_cast<const float4*>(s_wx + pnt * 8 + 4 * xpar);
-            o.y2 = *reinterpret_cast<const float2*>(s_wy + pnt * 8 + 2 * yy);
-            o.z2 = *reinterpret_cast<const float2*>(s_wz + pnt * 8 + 2 * zz);
+            o.x4 = *reinterpret_cast<const float4*>(s_wx + pnt * MF_WROW + 4 * xpar);
+            o.y2 = *reinterpret_cast<const float2*>(s_wy + pnt * MF_WROW + 2 * yy);
+            o.z2 = *reinterpret_cast<const float2*>(s_wz + pnt * MF_WROW + 2 * zz);
             o.bv = s_v[pnt * MF_NPL + n16];
             return o;
           };
@@ -1011,6 +1089,7 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
           }
         }
       }
+      if (!ahead && has_next) load(item_n.y, c0_n, 0, item_n.z < MB ? item_n.z : MB);
       __syncthreads();  // batch buffers become the staging area
       // ---- combine the pair's partial sums: lane holds cells
       // (x, 4 py + k4, 4 pz + i), i < 4, of plane n16
@@ -1062,6 +1141,10 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
         }
       }
     }
+    if (!has_next) break;
+    it = it_n;
+    item = item_n;
+    c0 = c0_n;
   }
 }
 
@@ -1199,16 +1282,23 @@ static int resident_workgroups(const void* kern, int threads, size_t lds) {
 }
 
 template <bool CPLX>
-static int launch_grid_mfma(const float* vals, float* out, int64_t nbatch,
+static int launch_grid_mfma(const float* vals, const float* locs, float* out, int64_t nbatch,
                             const InterpGeom& G, const tnl_plan_layout& L,
-                            const char* ws, hipStream_t s) {
+                            const char* ws, float* scratch, hipStream_t s) {
   const BinGeom B = make_bin_geom(L, true);
   const int* nitems = (const int*)(ws + L.off_itemoff) + L.g_ntiles_total;
   const int4* items = (const int4*)(ws + L.off_items);
   grid_zero_shared_kernel<CPLX><<<1024, 256, 0, s>>>(
       out, nbatch, G, B, items, nitems, (const int*)(ws + L.off_gstarts));
   TNL_LAUNCH_CHECK("grid_zero_shared");
-  const auto kern = grid_mfma_kernel<CPLX>;
+  float4* rec = reinterpret_cast<float4*>(scratch);
+  if (rec) {
+    const int64_t nb = (L.npts + 255) / 256;
+    pack_points_kernel<CPLX><<<(unsigned)(nb < 16384 ? nb : 16384), 256, 0, s>>>(
+        vals, L.npts, nbatch, rec);
+    TNL_LAUNCH_CHECK("pack_points");
+  }
+  const auto kern = rec ? grid_mfma_kernel<CPLX, true> : grid_mfma_kernel<CPLX, false>;
   if (int rc = allow_lds((const void*)kern, MF_LDS)) return rc;
   int64_t nwg = resident_workgroups((const void*)kern, MF_THREADS, MF_LDS);
   nwg = (nwg + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
@@ -1216,9 +1306,17 @@ static int launch_grid_mfma(const float* vals, float* out, int64_t nbatch,
   if (nwg > need) nwg = need;
   kern<<<(unsigned)nwg, MF_THREADS, MF_LDS, s>>>(
       vals, out, nbatch, G, B, (const int*)(ws + L.off_perm),
-      (const float*)(ws + L.off_slocs), (const int*)(ws + L.off_tlist), items, nitems);
+      (const float*)(ws + L.off_slocs), (const int*)(ws + L.off_tlist), items, nitems, locs,
+      rec);
   TNL_LAUNCH_CHECK("grid_mfma");
   return TNL_OK;
+}
+
+int64_t grid_scratch_bytes(const tnl_plan_layout& L, int64_t nbatch, int is_complex) {
+  if (L.ndim != 3 || L.grid_accum != TNL_GRID_ACCUM_MFMA || nbatch <= 0 || L.npts <= 0)
+    return 0;
+  const int64_t ch = is_complex ? MF_NPL / 2 : MF_NPL;
+  return (nbatch + ch - 1) / ch * L.npts * MF_REC4 * (int64_t)sizeof(float4);
 }
 
 template <int NDIM, bool CPLX, int CC, int NT>
@@ -1293,27 +1391,30 @@ static int dispatch_cc(bool adjoint, const float* vals, float* out,
 }
 
 template <int NDIM, bool CPLX>
-static int dispatch_nt(bool adjoint, const float* vals, float* out,
+static int dispatch_nt(bool adjoint, const float* vals, const float* locs, float* out,
                        int64_t nbatch, const InterpGeom& G,
-                       const tnl_plan_layout& L, const char* ws,
+                       const tnl_plan_layout& L, const char* ws, float* scratch,
                        hipStream_t s) {
   int nt = G.ntap[0];
   for (int d = 1; d < 3; ++d) nt = G.ntap[d] > nt ? G.ntap[d] : nt;
   if (NDIM == 3 && adjoint && use_grid_mfma(G, L))
-    return launch_grid_mfma<CPLX>(vals, out, nbatch, G, L, ws, s);
+    return launch_grid_mfma<CPLX>(vals, locs, out, nbatch, G, L, ws, scratch, s);
   if (nt <= 5) return dispatch_cc<NDIM, CPLX, 5>(adjoint, vals, out, nbatch, G, L, ws, s);
   return dispatch_cc<NDIM, CPLX, 8>(adjoint, vals, out, nbatch, G, L, ws, s);
 }
 
-int interp_binned(bool adjoint, const float* vals, float* out, int64_t nbatch,
-                  const InterpGeom& G, int is_complex, const tnl_plan_layout& L,
-                  const void* plan, hipStream_t s) {
+int interp_binned(bool adjoint, const float* vals, const float* locs, float* out,
+                  int64_t nbatch, const InterpGeom& G, int is_complex,
+                  const tnl_plan_layout& L, const void* plan, float* scratch,
+                  hipStream_t s) {
   const char* ws = (const char*)plan;
   if (G.ndim == 3)
-    return is_complex ? dispatch_nt<3, true>(adjoint, vals, out, nbatch, G, L, ws, s)
-                      : dispatch_nt<3, false>(adjoint, vals, out, nbatch, G, L, ws, s);
-  return is_complex ? dispatch_nt<2, true>(adjoint, vals, out, nbatch, G, L, ws, s)
-                    : dispatch_nt<2, false>(adjoint, vals, out, nbatch, G, L, ws, s);
+    return is_complex
+               ? dispatch_nt<3, true>(adjoint, vals, locs, out, nbatch, G, L, ws, scratch, s)
+               : dispatch_nt<3, false>(adjoint, vals, locs, out, nbatch, G, L, ws, scratch, s);
+  return is_complex
+             ? dispatch_nt<2, true>(adjoint, vals, locs, out, nbatch, G, L, ws, scratch, s)
+             : dispatch_nt<2, false>(adjoint, vals, locs, out, nbatch, G, L, ws, scratch, s);
 }
 
 // ---------------------------------------------------------------- batches

@@ -15,9 +15,13 @@ int interp_direct(bool adjoint, const float* vals, const float* locs,
 // problem is outside what the binned kernels cover (caller then falls back).
 bool binned_supported(const InterpGeom& G, const tnl_plan_layout& L,
                       bool adjoint);
-int interp_binned(bool adjoint, const float* vals, float* out, int64_t nbatch,
-                  const InterpGeom& G, int is_complex, const tnl_plan_layout& L,
-                  const void* plan, hipStream_t s);
+// scratch: NULL, or grid_scratch_bytes(L, nbatch, is_complex) bytes for the
+// point-major records of matrix-core gridding (ignored elsewhere).
+int interp_binned(bool adjoint, const float* vals, const float* locs, float* out,
+                  int64_t nbatch, const InterpGeom& G, int is_complex,
+                  const tnl_plan_layout& L, const void* plan, float* scratch,
+                  hipStream_t s);
+int64_t grid_scratch_bytes(const tnl_plan_layout& L, int64_t nbatch, int is_complex);
 
 int interp_grad_locs(const float* grid_vals, const float* pt_vals,
                      const float* locs, float* grad, int64_t nbatch,

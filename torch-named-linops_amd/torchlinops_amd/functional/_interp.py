@@ -19,6 +19,8 @@ gradient comes from the analytic window derivative in the
 
 from __future__ import annotations
 
+import ctypes
+
 from math import prod
 from typing import Optional
 
@@ -113,28 +115,39 @@ def _launch(adjoint: bool, vals_flat, locs, locs2d, out, nbatch, npts, ndim,
     beta = float(kernel_params.get("beta", 1.0)) if kernel == "kaiser_bessel" else 1.0
     pad_code = PAD_MODES[pad_mode]
     plan = get_plan(locs, locs2d, ndim, grid_size, width, norm, pad_code)
-    fn = lib().tnl_grid if adjoint else lib().tnl_ungrid
+    args = [
+        vals_flat.data_ptr(),
+        locs2d.data_ptr(),
+        out.data_ptr(),
+        nbatch,
+        npts,
+        ndim,
+        i64_array(grid_size),
+        f32_array(width),
+        KERNELS[kernel],
+        int(norm),
+        pad_code,
+        int(is_complex),
+        beta,
+        None if plan is None else plan.layout,
+        None if plan is None else plan.ptr,
+    ]
     with torch.cuda.device(vals_flat.device):
-        check(
-            fn(
-                vals_flat.data_ptr(),
-                locs2d.data_ptr(),
-                out.data_ptr(),
-                nbatch,
-                npts,
-                ndim,
-                i64_array(grid_size),
-                f32_array(width),
-                KERNELS[kernel],
-                int(norm),
-                pad_code,
-                int(is_complex),
-                beta,
-                None if plan is None else plan.layout,
-                None if plan is None else plan.ptr,
-                stream_of(vals_flat),
-            )
-        )
+        if not adjoint:
+            check(lib().tnl_ungrid(*args, stream_of(vals_flat)))
+            return
+        nbytes = ctypes.c_int64(0)
+        if plan is not None:
+            check(lib().tnl_grid_scratch_bytes(plan.layout, nbatch, int(is_complex),
+                                               ctypes.byref(nbytes)))
+        if nbytes.value > 0:
+            # point-major records for the matrix-core gridding kernel; from
+            # torch's caching allocator, so repeated calls do not hipMalloc
+            scratch = torch.empty(nbytes.value, dtype=torch.uint8, device=vals_flat.device)
+            check(lib().tnl_grid_ws(*args, scratch.data_ptr(), nbytes.value,
+                                    stream_of(vals_flat)))
+        else:
+            check(lib().tnl_grid(*args, stream_of(vals_flat)))
 
 
 def ungrid(
