@@ -101,10 +101,6 @@ __device__ __forceinline__ void ungrid_sub(
   const T* __restrict__ v = reinterpret_cast<const T*>(vals);
   const int w2 = B.win[2];
   const int nrows = CC * B.win[0] * B.win[1];
-  int64_t* rowbase = reinterpret_cast<int64_t*>(
-      lds_raw + ((size_t)CC * B.wcells * sizeof(T) + 7) / 8 * 8);
-  const float inv_w2 = 1.0f / (float)w2;
-  const int total = nrows * w2;
 
   // per-point data cached across channel chunks
   float cw[UPTS][3 * NT];
@@ -114,9 +110,11 @@ __device__ __forceinline__ void ungrid_sub(
     const int nc = (int)(nbatch - c0 < CC ? nbatch - c0 : CC);
     if (c0 > 0) __syncthreads();  // previous chunk's gathers are done
     // ---- stage window: channel planes [c][cell] in LDS (lanes gathering
-    // different cells then fall in different banks), coalesced along z in
-    // HBM.  A table of row bases (one per (channel, x, y) row of the
-    // window) keeps the per-element index math to one division.
+    // different cells then fall in different banks).  One thread per
+    // (channel, x, y) row of the window: the row's w2 cells are contiguous
+    // in HBM and in LDS (row r of the window starts at lds[r * w2]), so a
+    // row that neither wraps nor leaves the grid is copied with 16-byte
+    // loads and stores; the others go cell by cell.
     for (int r = threadIdx.x; r < nrows; r += BIN_THREADS) {
       const int c = r / (B.win[0] * B.win[1]);
       const int rem = r - c * (B.win[0] * B.win[1]);
@@ -129,27 +127,30 @@ __device__ __forceinline__ void ungrid_sub(
       } else {
         ok = ok && gx < G.n[0] && gy < G.n[1];
       }
-      rowbase[r] = ok ? (c0 + c) * G.ncells + (gx * G.n[1] + gy) * G.n[2] : -1;
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < total; idx += BIN_THREADS) {
-      int r = (int)(((float)idx + 0.5f) * inv_w2);
-      if (r * w2 > idx) --r;              // correct the float quotient
-      else if ((r + 1) * w2 <= idx) ++r;
-      const int wz = idx - r * w2;
-      int64_t gz = origin[2] + wz;
-      const int64_t base = rowbase[r];
-      bool ok = base >= 0;
-      if (G.pad == TNL_PAD_CIRCULAR) {
-        if (gz >= G.n[2]) gz = mod_pos(gz, G.n[2]);
-      } else {
-        ok = ok && gz < G.n[2];
+      T* dst = lds + (int64_t)r * w2;
+      if (!ok) {
+        for (int k = 0; k < w2; ++k) dst[k] = C::zero();
+        continue;
       }
-      T val = C::zero();
-      if (ok) val = v[base + gz];
-      // channel c of row r lives in plane c; rows of one channel are
-      // contiguous, so the LDS index is simply idx
-      lds[idx] = val;
+      const int64_t base = (c0 + c) * G.ncells + (gx * G.n[1] + gy) * G.n[2];
+      const int64_t z0 = origin[2];
+      constexpr int PER4 = 16 / (int)sizeof(T);  // cells per 16 bytes
+      if (z0 + w2 <= G.n[2] && w2 % PER4 == 0 && (base + z0) % PER4 == 0) {
+        const float4* src4 = reinterpret_cast<const float4*>(v + base + z0);
+        float4* dst4 = reinterpret_cast<float4*>(dst);
+        for (int k = 0; k < w2 / PER4; ++k) dst4[k] = src4[k];
+      } else {
+        for (int k = 0; k < w2; ++k) {
+          int64_t gz = z0 + k;
+          bool in = true;
+          if (G.pad == TNL_PAD_CIRCULAR) {
+            if (gz >= G.n[2]) gz = mod_pos(gz, G.n[2]);
+          } else {
+            in = gz < G.n[2];
+          }
+          dst[k] = in ? v[base + gz] : C::zero();
+        }
+      }
     }
     __syncthreads();
 
@@ -1222,8 +1223,7 @@ static BinGeom make_bin_geom(const tnl_plan_layout& L, bool adjoint) {
 }
 
 static size_t ungrid_lds_bytes(const BinGeom& B, int cc, int comp_bytes) {
-  return ((size_t)B.wcells * cc * comp_bytes + 7) / 8 * 8 +
-         (size_t)cc * B.win[0] * B.win[1] * sizeof(int64_t);
+  return (size_t)B.wcells * cc * comp_bytes;
 }
 
 static size_t grid_lds_bytes(const BinGeom& B, int planes, int nt) {
