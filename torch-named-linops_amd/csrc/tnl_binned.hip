@@ -771,21 +771,22 @@ __global__ __launch_bounds__(GW * NW) void grid_binned_kernel(
 // (512 x P) by (P x 16) product.  v_mfma_f32_16x16x4f32 takes a block of 16
 // cells (one x plane, a 4x4 (y, z) patch) by 16 planes by 4 points; lane l
 // holds A[l % 16][l / 16], B[l / 16][l % 16] and D[4 (l / 16) + i][l % 16].
-// A block that none of the 4 points touches is skipped (a wave-uniform
-// mask), so the work follows the footprints rather than the tile volume.
+// A block that none of the points of a pair of such groups touches is
+// skipped (a wave-uniform mask), so the work follows the footprints rather
+// than the tile volume.
 //
 // Workgroup: 4 waves on one tile.  Waves 0, 1 hold the even x planes (16
 // blocks, 64 accumulator registers per lane), waves 2, 3 the odd ones, so a
 // footprint (4 or 5 consecutive planes) gives both pairs about the same
-// work; the two waves of a pair take alternate groups of 4 points.  Touches are taken in
+// work; the two waves of a pair take alternate pairs of point groups.  Touches are taken in
 // batches of MB points (one per thread):
 //   1. lanes = points: first taps, the 8-cell tile-local window of every
 //      axis (zero off the footprint; wrapped taps land where they wrap to,
 //      and taps that land on one cell add), the channel values, and the
 //      mask of touched blocks OR-ed over each group of 4 points.  The next
 //      batch's raw inputs are loaded right after.
-//   2. waves: per group of 4 points, A = products of the windows, B = the
-//      values, one MFMA per live block.
+//   2. waves: per pair of groups of 4 points, A = products of the windows,
+//      B = the values, two MFMAs per live block.
 // At the end of the tile the two partial sums of each x plane meet in LDS
 // (one wave stores, the other adds) and the tile goes to HBM once,
 // coalesced along z.
@@ -895,7 +896,7 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
   using C = Ch<CPLX>;
   using T = typename C::T;
   constexpr int CH = CPLX ? MF_NPL / 2 : MF_NPL;  // channels per chunk
-  static_assert(MB == MF_THREADS && MB / 8 <= 64, "one point per thread; a lane per group");
+  static_assert(MB == MF_THREADS && MB / 16 <= 64, "one point per thread; a lane per pair");
   extern __shared__ __align__(16) unsigned char lds_raw[];
   float* s_wx = reinterpret_cast<float*>(lds_raw);  // [MB][MF_WROW], x order
   float* s_wy = s_wx + MB * MF_WROW;                // [MB][MF_WROW], pair order
@@ -1022,18 +1023,21 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
           ahead = true;
         }
         __syncthreads();
-        // ---- phase 2: this wave's x planes (one parity), every other group of
-        // 4 points.
-        // Lane i holds the block mask of group par + 2 i; the groups with
-        // live blocks are walked from a ballot, and the next group's
-        // operands are read from LDS while the current group's MFMAs run.
+        // ---- phase 2: this wave's x planes (one parity), every other pair
+        // of point groups (8 points; a pair shares one decision per block
+        // and gets two MFMAs per live block).  Lane i holds this wave's
+        // block mask of pair par + 2 i, OR-ed over its two groups; the pairs
+        // with live blocks are walked from a ballot, and the next pair's
+        // operands are read from LDS while the current pair's MFMAs run.
         {
           const int nkg = (nb + 3) >> 2;
-          const int kgl = par + 2 * lane;
-          // this wave's blocks of the group: x planes xpar, xpar + 2, ...
+          const int npg = (nkg + 1) >> 1;
+          const int pgl = par + 2 * lane;
           unsigned lm = 0;
-          if (lane < MB / 8 && kgl < nkg) {
-            const unsigned mm = s_km[kgl];
+          if (lane < MB / 16 && pgl < npg) {
+            unsigned mm = s_km[2 * pgl];
+            if (2 * pgl + 1 < nkg) mm |= s_km[2 * pgl + 1];
+            // this wave's blocks: x planes xpar, xpar + 2, ...
 #pragma unroll
             for (int j = 0; j < 4; ++j) lm |= ((mm >> (4 * (2 * j + xpar))) & 15u) << (4 * j);
           }
@@ -1043,8 +1047,9 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
             float2 y2, z2;
             float bv;
           };
-          auto fetch = [&](int i) {
-            const int pnt = (par + 2 * i) * 4 + k4;
+          // A (window) and B (value) operand parts of group g for this lane
+          auto fetch = [&](int g) {
+            const int pnt = g * 4 + k4;
             Ops o;
             o.x4 = *reinterpret_cast<const float4*>(s_wx + pnt * MF_WROW + 4 * xpar);
             o.y2 = *reinterpret_cast<const float2*>(s_wy + pnt * MF_WROW + 2 * yy);
@@ -1052,20 +1057,24 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
             o.bv = s_v[pnt * MF_NPL + n16];
             return o;
           };
-          auto group = [&](const Ops& o, unsigned hm) {
-            const float wyz[4] = {o.y2.x * o.z2.x, o.y2.x * o.z2.y, o.y2.y * o.z2.x,
-                                  o.y2.y * o.z2.y};
-            const float wxs[4] = {o.x4.x, o.x4.y, o.x4.z, o.x4.w};
+          auto pair = [&](const Ops& o0, const Ops& o1, unsigned hm) {
+            const float wyz0[4] = {o0.y2.x * o0.z2.x, o0.y2.x * o0.z2.y, o0.y2.y * o0.z2.x,
+                                   o0.y2.y * o0.z2.y};
+            const float wyz1[4] = {o1.y2.x * o1.z2.x, o1.y2.x * o1.z2.y, o1.y2.y * o1.z2.x,
+                                   o1.y2.y * o1.z2.y};
+            const float wxs0[4] = {o0.x4.x, o0.x4.y, o0.x4.z, o0.x4.w};
+            const float wxs1[4] = {o1.x4.x, o1.x4.y, o1.x4.z, o1.x4.w};
 #pragma unroll
             for (int xl = 0; xl < 4; ++xl)
 #pragma unroll
               for (int p = 0; p < 4; ++p)
-                if ((hm >> (4 * xl + p)) & 1u)
+                if ((hm >> (4 * xl + p)) & 1u) {
                   acc[4 * xl + p] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                      wxs[xl] * wyz[p], o.bv, acc[4 * xl + p], 0, 0, 0);
+                      wxs0[xl] * wyz0[p], o0.bv, acc[4 * xl + p], 0, 0, 0);
+                  acc[4 * xl + p] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                      wxs1[xl] * wyz1[p], o1.bv, acc[4 * xl + p], 0, 0, 0);
+                }
           };
-          // two operand sets in turn: while one group's MFMAs issue, the
-          // other set is being filled (no register copies between them)
           auto next = [&](int prev, unsigned* hm) {
             const bool more = live != 0;
             const int j = more ? __builtin_ctzll(live) : prev;
@@ -1073,18 +1082,21 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
             *hm = more ? (unsigned)__builtin_amdgcn_readlane((int)lm, j) : 0u;
             return j;
           };
+          // two operand sets in turn: while one pair's MFMAs issue, the
+          // other set is being filled (no register copies between them)
           if (live) {
             unsigned ha, hb;
             int ia = next(0, &ha);
-            Ops oa = fetch(ia);
+            Ops a0 = fetch(2 * (par + 2 * ia)), a1 = fetch(2 * (par + 2 * ia) + 1);
             for (;;) {
               const int ib = next(ia, &hb);
-              Ops ob = fetch(ib);
-              group(oa, ha);
+              const Ops b0 = fetch(2 * (par + 2 * ib)), b1 = fetch(2 * (par + 2 * ib) + 1);
+              pair(a0, a1, ha);
               if (!hb) break;
               ia = next(ib, &ha);
-              oa = fetch(ia);
-              group(ob, hb);
+              a0 = fetch(2 * (par + 2 * ia));
+              a1 = fetch(2 * (par + 2 * ia) + 1);
+              pair(b0, b1, hb);
               if (!ha) break;
             }
           }

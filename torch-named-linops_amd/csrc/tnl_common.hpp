@@ -51,16 +51,18 @@ __device__ __forceinline__ float i0_series(float u) {
 
 // Kaiser-Bessel window I0(beta * sqrt(1 - x^2)), unnormalised, |x| <= 1, as
 // in src/torchlinops/functional/_interp/kernels.py:50-89.  For beta <= 30
-// the power series of I0 in z^2 = beta^2 (1 - x^2) is summed directly: 24
-// terms leave a tail below 1e-7 of I0(beta) for beta <= 20, 32 terms for
-// beta <= 30 (the first omitted term bounds the tail, which then decays by
-// z^2 / (4 (k + 1)^2) per term).  That is branch-free and needs no sqrt or
-// exp; in float32 it stays within 1e-6 of I0, as close as the Abramowitz &
-// Stegun 9.8.1 / 9.8.2 polynomials (Handbook of Mathematical Functions,
-// p. 378) that the reference evaluates and that serve larger beta here.
+// the power series of I0 in z^2 = beta^2 (1 - x^2) is summed directly: 16
+// terms leave a tail below 1e-7 of I0(beta) for beta <= 10, 24 terms for
+// beta <= 20, 32 terms for beta <= 30 (the first omitted term bounds the
+// tail, which then decays by z^2 / (4 (k + 1)^2) per term).  That is
+// branch-free and needs no sqrt or exp; in float32 it stays within 1e-6 of
+// I0, as close as the Abramowitz & Stegun 9.8.1 / 9.8.2 polynomials
+// (Handbook of Mathematical Functions, p. 378) that the reference
+// evaluates and that serve larger beta here.
 // Callers guarantee |x| <= 1 (clamped).
 __device__ __forceinline__ float kb_window(float x, float beta) {
   const float v = fmaxf(1.0f - x * x, 0.0f);
+  if (beta <= 10.0f) return i0_series<16>(beta * beta * (1.0f / 64.0f) * v);
   if (beta <= 20.0f) return i0_series<24>(beta * beta * (1.0f / 64.0f) * v);
   if (beta <= 30.0f) return i0_series<32>(beta * beta * (1.0f / 64.0f) * v);
   const float z = beta * sqrtf(v);
