@@ -334,43 +334,35 @@ __global__ void grid_zero_shared_kernel(float* __restrict__ out,
 
 // Pre-pass for gridding of one problem: zero the output region of every
 // split tile (more than g_cap touches, so its items add with atomics).  One
-// wave checks 64 tiles at a time; the workgroup then zeroes the split ones
-// among them, a row of cells (contiguous along z) per thread group.
+// thread per (tile, cell), cells z fastest so that a wave's stores cover
+// whole rows; every channel.  Cells of unsplit tiles return at once.
+// Split tiles cluster (the dense centre of a radial trajectory), so work
+// is spread by cell rather than by tile.
 template <bool CPLX>
 __global__ __launch_bounds__(256) void grid_zero_split_kernel(
     float* __restrict__ out, int64_t nbatch, InterpGeom G, BinGeom B,
     const int* __restrict__ gcounts, int64_t ntiles, int g_cap) {
   using T = typename Ch<CPLX>::T;
-  __shared__ int list[64];
-  __shared__ int nlist;
   T* __restrict__ o = reinterpret_cast<T*>(out);
-  const int t12 = B.tile[1] * B.tile[2];
-  for (int64_t t0 = (int64_t)blockIdx.x * 64; t0 < ntiles; t0 += (int64_t)gridDim.x * 64) {
-    if (threadIdx.x == 0) nlist = 0;
-    __syncthreads();
-    if (threadIdx.x < 64 && t0 + threadIdx.x < ntiles && gcounts[t0 + threadIdx.x] > g_cap)
-      list[atomicAdd(&nlist, 1)] = (int)(t0 + threadIdx.x);
-    __syncthreads();
-    const int n = nlist;
-    for (int j = 0; j < n; ++j) {
-      int origin[3];
-      int r = list[j];
-      for (int d = 2; d >= 0; --d) {
-        origin[d] = (r % B.ntiles[d]) * B.tile[d];
-        r /= B.ntiles[d];
-      }
-      const int total = (int)nbatch * B.tcells;
-      for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int c = idx / B.tcells;
-        const int cell = idx - c * B.tcells;
-        const int lx = cell / t12, rem = cell - lx * t12;
-        const int ly = rem / B.tile[2], lz = rem - ly * B.tile[2];
-        const int64_t gx = origin[0] + lx, gy = origin[1] + ly, gz = origin[2] + lz;
-        if (gx < G.n[0] && gy < G.n[1] && gz < G.n[2])
-          o[c * G.ncells + (gx * G.n[1] + gy) * G.n[2] + gz] = Ch<CPLX>::zero();
-      }
+  const int tc = B.tcells, t12 = B.tile[1] * B.tile[2];
+  const int64_t total = ntiles * tc;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = idx / tc;
+    if (gcounts[t] <= g_cap) continue;
+    const int cell = (int)(idx - t * tc);
+    int origin[3];
+    int r = (int)t;
+    for (int d = 2; d >= 0; --d) {
+      origin[d] = (r % B.ntiles[d]) * B.tile[d];
+      r /= B.ntiles[d];
     }
-    __syncthreads();
+    const int lx = cell / t12, rem = cell - lx * t12;
+    const int ly = rem / B.tile[2], lz = rem - ly * B.tile[2];
+    const int64_t gx = origin[0] + lx, gy = origin[1] + ly, gz = origin[2] + lz;
+    if (gx >= G.n[0] || gy >= G.n[1] || gz >= G.n[2]) continue;
+    const int64_t g = (gx * G.n[1] + gy) * G.n[2] + gz;
+    for (int64_t c = 0; c < nbatch; ++c) o[c * G.ncells + g] = Ch<CPLX>::zero();
   }
 }
 
@@ -378,10 +370,8 @@ __global__ __launch_bounds__(256) void grid_zero_split_kernel(
 template <bool CPLX>
 static int zero_split_tiles(float* out, int64_t nbatch, const InterpGeom& G, const BinGeom& B,
                             const tnl_plan_layout& L, const char* ws, hipStream_t s) {
-  if ((int64_t)nbatch * B.tcells >= (int64_t)1 << 31)
-    return fail(TNL_EINVAL, "nbatch times gridding tile cells must stay below 2^31");
-  const int64_t ngroups = (L.g_ntiles_total + 63) / 64;
-  grid_zero_split_kernel<CPLX><<<(unsigned)(ngroups < 4096 ? ngroups : 4096), 256, 0, s>>>(
+  const int64_t nwg = (L.g_ntiles_total * B.tcells + 255) / 256;
+  grid_zero_split_kernel<CPLX><<<(unsigned)(nwg < 16384 ? nwg : 16384), 256, 0, s>>>(
       out, nbatch, G, B, (const int*)(ws + L.off_gcounts), L.g_ntiles_total, (int)L.g_cap);
   TNL_LAUNCH_CHECK("grid_zero_split");
   return TNL_OK;
