@@ -288,8 +288,9 @@ __global__ __launch_bounds__(BIN_THREADS) void ungrid_binned_kernel(
   ungrid_sub<NDIM, CPLX, CC, NT>(vals, out, nbatch, G, B, perm, slocs, sub[s]);
 }
 
-// Pre-pass for gridding: zero the output region of every split (shared)
-// tile, once per tile (by its first item).  Called by one workgroup per item.
+// Pre-pass for batched gridding: zero the output region of every split
+// (shared) tile, once per tile (by its first item).  Called by one
+// workgroup per item.
 template <bool CPLX>
 __device__ __forceinline__ void zero_shared_item(float* __restrict__ out,
                                                  int64_t nbatch,
@@ -318,18 +319,6 @@ __device__ __forceinline__ void zero_shared_item(float* __restrict__ out,
     if (gx >= G.n[0] || gy >= G.n[1] || gz >= G.n[2]) continue;
     out[(c * G.ncells + (gx * G.n[1] + gy) * G.n[2] + gz) * comp + part] = 0.0f;
   }
-}
-
-template <bool CPLX>
-__global__ void grid_zero_shared_kernel(float* __restrict__ out,
-                                        int64_t nbatch, InterpGeom G,
-                                        BinGeom B,
-                                        const int4* __restrict__ items,
-                                        const int* __restrict__ nitems_ptr,
-                                        const int* __restrict__ gstarts) {
-  const int nitems = *nitems_ptr;
-  for (int it = blockIdx.x; it < nitems; it += gridDim.x)
-    zero_shared_item<CPLX>(out, nbatch, G, B, items[it], gstarts);
 }
 
 // Pre-pass for gridding of one problem: zero the output region of every
@@ -823,13 +812,13 @@ __global__ __launch_bounds__(GW * NW) void grid_binned_kernel(
 // Workgroup: 4 waves on one tile.  Waves 0, 1 hold the even x planes (16
 // blocks, 64 accumulator registers per lane), waves 2, 3 the odd ones, so a
 // footprint (4 or 5 consecutive planes) gives both pairs about the same
-// work; the two waves of a pair take alternate pairs of point groups.  Touches are taken in
-// batches of MB points (one per thread):
+// work; the two waves of a pair take alternate pairs of point groups.
+// Touches are taken in batches of MB points (one per thread):
 //   1. lanes = points: first taps, the 8-cell tile-local window of every
-//      axis (zero off the footprint; wrapped taps land where they wrap to,
-//      and taps that land on one cell add), the channel values, and the
-//      mask of touched blocks OR-ed over each group of 4 points.  The next
-//      batch's raw inputs are loaded right after.
+//      axis (zero off the footprint; wrapped taps land where they wrap to),
+//      the channel values, and the mask of touched blocks OR-ed over each
+//      group of 4 points.  The next batch's raw inputs (of this tile, or of
+//      the workgroup's next tile or channel chunk) are loaded right after.
 //   2. waves: per pair of groups of 4 points, A = products of the windows,
 //      B = the values, two MFMAs per live block.
 // At the end of the tile the two partial sums of each x plane meet in LDS
