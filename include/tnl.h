@@ -61,12 +61,13 @@ typedef struct tnl_tuning {
   int32_t max_sub;    /* points per interpolation subproblem                */
   int32_t g_cap;      /* touches per gridding work item                     */
   int32_t lds_budget; /* LDS bytes per workgroup for channel chunking       */
-  int32_t grid_accum; /* TNL_GRID_ACCUM_*: 3-D gridding accumulator        */
+  int32_t grid_accum; /* TNL_GRID_ACCUM_*: gridding accumulator           */
 } tnl_tuning;
 
 /* Gridding accumulator.  AUTO takes the matrix-core (MFMA) accumulator for
- * 3-D plans with 8x8x8 gridding tiles and the LDS accumulator otherwise;
- * MFMA on a plan that does not qualify is an error. */
+ * 3-D plans with 8x8x8 and 2-D plans with 16x32 gridding tiles (the
+ * defaults for grids above 16 / 32 x 64) and the LDS accumulator
+ * otherwise; MFMA on a plan that does not qualify is an error. */
 #define TNL_GRID_ACCUM_AUTO 0
 #define TNL_GRID_ACCUM_LDS 1
 #define TNL_GRID_ACCUM_MFMA 2
@@ -143,8 +144,9 @@ int tnl_grid(const float* vals, const float* locs, float* out, int64_t nbatch,
 /* Scratch bytes that tnl_grid_ws uses with this plan for nbatch channels:
  * the channel values re-packed point-major, one 64-byte record (up to 8
  * complex / 16 real channel values) per point and channel chunk, for the
- * matrix-core gridding kernel (3-D plans with 8x8x8 tiles); 0 for other
- * plans (tnl_grid_ws then equals tnl_grid). */
+ * matrix-core gridding kernel (plans whose grid_accum is
+ * TNL_GRID_ACCUM_MFMA); 0 for other plans (tnl_grid_ws then equals
+ * tnl_grid). */
 int tnl_grid_scratch_bytes(const tnl_plan_layout* layout, int64_t nbatch,
                            int is_complex, int64_t* bytes);
 
@@ -198,6 +200,9 @@ typedef struct tnl_batch_layout {
   int32_t grid_cc;        /* channels per gridding accumulator */
   int32_t ungrid_chunks;  /* channel chunks an ungrid workgroup walks:
                              max over problems of ceil(nbatch / ungrid_cc) */
+  int32_t grid_mfma;      /* 1: every plan's grid_accum is TNL_GRID_ACCUM_MFMA,
+                             so the batch grids on the matrix cores */
+  int32_t reserved;
   int64_t total_sub;      /* sum of the plans' subproblem capacities */
   int64_t total_items;    /* sum of the plans' gridding work-item capacities */
   int64_t ungrid_lds;     /* LDS bytes per ungrid workgroup */

@@ -168,36 +168,42 @@ ACC_CASES = [
     ((32, 24, 40), 2000, 2, 6.0, "kaiser_bessel", "circular", True, 0),
     ((32, 24, 40), 2000, 1, (4.0, 3.0, 7.0), "kaiser_bessel", "zero", False, 128),
     ((17, 19, 23), 1500, 5, 2.5, "spline", "circular", True, 0),
+    # 2-D, 16x32 tiles
+    ((40, 70), 3000, 3, 4.0, "kaiser_bessel", "circular", True, 0),
+    ((64, 96), 4000, 11, 6.0, "kaiser_bessel", "zero", True, 64),
+    ((48, 80), 3000, 17, (3.0, 5.0), "spline", "circular", False, 0),
+    ((40, 70), 3000, 2, 5.0, "kaiser_bessel", "zero", False, 0),
 ]
 
 
 @pytest.mark.parametrize("case", ACC_CASES, ids=lambda c: f"{c[0]}-w{c[3]}-{c[5]}-nb{c[2]}")
 @pytest.mark.parametrize("accum", ["mfma", "lds"])
-def test_grid_accumulators_3d(dev, case, accum):
-    """3-D gridding on 8x8x8 tiles through either accumulator (matrix-core
-    blocks or the LDS read-modify-write) against the oracle: channel chunks
-    (nbatch above 8 complex / 16 real), widths past 5 taps, anisotropic
-    widths, both pads, points on integer coordinates and at the edges, and
-    split tiles (small g_cap)."""
+def test_grid_accumulators(dev, case, accum):
+    """Gridding on 8x8x8 (3-D) / 16x32 (2-D) tiles through either
+    accumulator (matrix-core blocks or the LDS read-modify-write) against
+    the oracle: channel chunks (nbatch above 8 complex / 16 real), widths
+    past 5 taps, anisotropic widths, both pads, points on integer
+    coordinates and at the edges, and split tiles (small g_cap)."""
     from torchlinops_amd.functional import _plan
 
     grid_size, npts, nbatch, width, kernel, pad, cplx, g_cap = case
+    ndim = len(grid_size)
     ntap = int(np.ceil(np.max(width))) + 1
-    if accum == "lds" and ntap > 5:
+    if accum == "lds" and ndim == 3 and ntap > 5:
         pytest.skip("the LDS accumulator covers at most 5 taps per axis in 3-D")
     rng = np.random.default_rng(31)
-    locs = rng.uniform(-2, np.array(grid_size) + 2, (npts, 3)).astype(np.float32)
+    locs = rng.uniform(-2, np.array(grid_size) + 2, (npts, ndim)).astype(np.float32)
     locs[:9] = np.round(locs[:9])
     locs[9:12] = np.array(grid_size, dtype=np.float32) - 0.25
     knobs = {"grid_accum": accum}
     if g_cap:
         knobs["g_cap"] = g_cap
     try:
-        _plan.set_tuning(3, **knobs)
+        _plan.set_tuning(ndim, **knobs)
         _run(dev, grid_size, locs, nbatch, width, kernel, pad, 1, 1.7, "always", cplx, seed=32)
     finally:
         with _plan._tuning_lock:
-            _plan._tuning.pop(3, None)
+            _plan._tuning.pop(ndim, None)
         _plan.clear_plan_cache()
 
 

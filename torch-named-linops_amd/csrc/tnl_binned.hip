@@ -827,29 +827,69 @@ __global__ __launch_bounds__(GW * NW) void grid_binned_kernel(
 typedef float mf32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int MF_THREADS = 256;
-constexpr int MB = 256;          // points per batch (one per thread)
-constexpr int MF_PSTR = 516;     // staging plane stride (floats): 512 + 4
-constexpr int MF_NPL = 16;       // accumulator planes
-constexpr int MF_WROW = 8;       // floats per point row of a window
-constexpr size_t MF_LDS_BATCH = (size_t)MB * (3 * MF_WROW + MF_NPL) * 4 + MB;
-constexpr size_t MF_LDS_STAGE = (size_t)MF_NPL * MF_PSTR * 4;
-constexpr size_t MF_LDS = MF_LDS_BATCH > MF_LDS_STAGE ? MF_LDS_BATCH : MF_LDS_STAGE;
+constexpr int MF_PSTR = 516;  // staging plane stride (floats): 512 + 4
+constexpr int MF_NPL = 16;    // accumulator planes
 
-// 8-cell tile-local window of one axis for the point at coordinate t,
-// written to row[0..7]: each tap's weight lands in the slot of the cell it
+// Per-dimension shape of the matrix-core gridding kernel.  A tile has 512
+// cells in 32 blocks of 16; block b = 4 u + v, where u in [0, 8) is split
+// between the wave pairs by parity and v in [0, 4) is not:
+//   3-D, 8x8x8 tiles: u = x plane, v = 4x4 (y, z) patch (2 (y / 4) + z / 4)
+//   2-D, 16x32 tiles: u = z / 4, v = y / 4
+// Per point of a batch, LDS holds the "U" window (3-D: x, 8 cells; 2-D: z,
+// 32 cells), the "V" windows (3-D: y then z, 8 cells each; 2-D: y, 16
+// cells) and the 16 channel planes.
+template <int NDIM>
+struct MfShape;
+template <>
+struct MfShape<3> {
+  static constexpr int MB = 256;  // points per batch (one per thread)
+  static constexpr int RA = 8;    // floats per point: U window
+  static constexpr int RB = 16;   // floats per point: V windows
+};
+template <>
+struct MfShape<2> {
+  static constexpr int MB = 128;
+  static constexpr int RA = 32;
+  static constexpr int RB = 16;
+};
+
+template <int NDIM>
+constexpr size_t mf_lds_bytes() {
+  using S = MfShape<NDIM>;
+  const size_t batch = (size_t)S::MB * (S::RA + S::RB + MF_NPL) * 4 + (size_t)S::MB;
+  const size_t stage = (size_t)MF_NPL * MF_PSTR * 4;
+  return batch > stage ? batch : stage;
+}
+
+// Slot orders of the tile-local windows, chosen so that a lane reads the
+// four values it needs with one b128 load (or a pair with one b64 load).
+enum MfOrder {
+  MF_X8,    // 3-D x: [W0, W2, W4, W6, W1, W3, W5, W7] (a wave's parity)
+  MF_PAIR8, // 3-D y, z: [W0, W4, W1, W5, W2, W6, W3, W7] (pairs {j, j + 4})
+  MF_Y16,   // 2-D y: slot 4 (y % 4) + y / 4 (lane y % 4 reads y / 4 = 0..3)
+  MF_Z32    // 2-D z: slot 16 ((z / 4) % 2) + 4 (z % 4) + z / 8 (parity, lane)
+};
+
+template <int ORD>
+__device__ __forceinline__ int mf_slot(int l) {
+  if constexpr (ORD == MF_X8) return (l & 1) * 4 + (l >> 1);
+  else if constexpr (ORD == MF_PAIR8) return 2 * (l & 3) + (l >> 2);
+  else if constexpr (ORD == MF_Y16) return (l & 3) * 4 + (l >> 2);
+  else return ((l >> 2) & 1) * 16 + (l & 3) * 4 + (l >> 3);
+}
+
+// TL-cell tile-local window of axis d for the point at coordinate t,
+// written to row[0..TL): each tap's weight lands in the slot of the cell it
 // falls on (taps wrap around the grid under circular padding; no two taps
-// share a cell, as the footprint never exceeds the grid), zero elsewhere.  Slot
-// order: [W0, W2, W4, W6, W1, W3, W5, W7] when PAIRS is false (x: a wave
-// reads its four planes of one parity with one b128 load), [W0, W4, W1, W5,
-// W2, W6, W3, W7] when true (y, z: a lane reads the pair {W[j], W[j + 4]}
-// with one b64 load).  Returns the mask of cells some tap falls on.
-template <bool PAIRS>
+// share a cell, as the footprint never exceeds the grid), zero elsewhere.
+// Returns the mask of cells some tap falls on.
+template <int TL, int ORD>
 __device__ __forceinline__ unsigned mf_axis_place(float t, int d, int origin,
                                                   const InterpGeom& G,
                                                   float* __restrict__ row) {
   float4* r4 = reinterpret_cast<float4*>(row);
-  r4[0] = make_float4(0.f, 0.f, 0.f, 0.f);
-  r4[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int i = 0; i < TL / 4; ++i) r4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
   unsigned pm = 0;
   const TapRange r = tap_range(t, G.half[d], G.ntap[d], G.n[d], G.pad);
   // The first kept tap wrapped into [0, n) under circular padding: one add
@@ -872,8 +912,8 @@ __device__ __forceinline__ unsigned mf_axis_place(float t, int d, int origin,
   for (int k = 0; k < r.count; ++k) {
     int l = lo + k;
     if (G.pad == TNL_PAD_CIRCULAR && origin + l >= n) l -= n;
-    if ((unsigned)l < 8u) {
-      row[PAIRS ? 2 * (l & 3) + (l >> 2) : (l & 1) * 4 + (l >> 1)] =
+    if ((unsigned)l < (unsigned)TL) {
+      row[mf_slot<ORD>(l)] =
           window(G.kernel, fminf(fabsf((g0f + (float)k) - t) * G.inv_half[d], 1.0f),
                  G.beta);
       pm |= 1u << l;
@@ -917,10 +957,280 @@ __global__ void pack_points_kernel(const float* __restrict__ vals, int64_t npts,
   }
 }
 
-// REC: the points' values come from the point-major records (rec, caller
-// order) and their coordinates from the caller's locs, instead of the
-// coil-major input and slocs.
-template <bool CPLX, bool REC>
+// Shared-memory carve-up and lane roles of the matrix-core gridding
+// kernels (one tile per workgroup of 4 waves).
+template <int NDIM>
+struct MfLds {
+  float* wa;     // [MB][RA]: U window of each point of the batch
+  float* wb;     // [MB][RB]: V windows
+  float* v;      // [MB][16]: channel planes
+  unsigned* km;  // [MB / 4]: block masks of the groups of 4 points
+  float* stage;  // [16][MF_PSTR]: the tile's planes, after the batches
+  __device__ explicit MfLds(unsigned char* raw) {
+    using S = MfShape<NDIM>;
+    wa = reinterpret_cast<float*>(raw);
+    wb = wa + S::MB * S::RA;
+    v = wb + S::MB * S::RB;
+    km = reinterpret_cast<unsigned*>(v + S::MB * MF_NPL);
+    stage = reinterpret_cast<float*>(raw);
+  }
+};
+
+struct MfLane {
+  int tid, lane, xpar, par;  // thread, lane, U parity of the wave, group parity
+  int n16, k4, yy, zz;       // B/D column, A column / D row group, A row 4 yy + zz
+  __device__ MfLane() {
+    tid = threadIdx.x;
+    lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    xpar = wv >> 1;
+    par = wv & 1;
+    n16 = lane & 15;
+    k4 = lane >> 4;
+    yy = n16 >> 2;
+    zz = n16 & 3;
+  }
+};
+
+// Phase 1, lanes = points: the tile-local windows of this thread's point
+// (zero past nb), its channel planes, and the block masks OR-ed over each
+// group of 4 points.
+template <int NDIM, bool CPLX>
+__device__ __forceinline__ void mf_phase1(const MfLds<NDIM>& S, const MfLane& ln, int nb,
+                                          const float* pt, const typename Ch<CPLX>::T* pv,
+                                          const int* origin, const InterpGeom& G) {
+  using Sh = MfShape<NDIM>;
+  constexpr int CH = CPLX ? MF_NPL / 2 : MF_NPL;
+  const int tid = ln.tid;
+  if (tid >= Sh::MB) return;
+  const bool live = tid < nb;
+  unsigned m = 0;
+  float* ra = S.wa + tid * Sh::RA;
+  float* rb = S.wb + tid * Sh::RB;
+  if (live) {
+    if constexpr (NDIM == 3) {
+      const unsigned px = mf_axis_place<8, MF_X8>(pt[0], 0, origin[0], G, ra);
+      const unsigned py = mf_axis_place<8, MF_PAIR8>(pt[1], 1, origin[1], G, rb);
+      const unsigned pz = mf_axis_place<8, MF_PAIR8>(pt[2], 2, origin[2], G, rb + 8);
+      m = tile_blocks32(px, py, pz);
+    } else {
+      const unsigned py = mf_axis_place<16, MF_Y16>(pt[0], 1, origin[1], G, rb);
+      const unsigned pz = mf_axis_place<32, MF_Z32>(pt[1], 2, origin[2], G, ra);
+      m = tile_blocks32_2d(py, pz);
+    }
+  } else {
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < Sh::RA / 4; ++i) reinterpret_cast<float4*>(ra)[i] = z4;
+#pragma unroll
+    for (int i = 0; i < Sh::RB / 4; ++i) reinterpret_cast<float4*>(rb)[i] = z4;
+  }
+  float tv[MF_NPL];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    if constexpr (CPLX) {
+      tv[2 * c] = live ? pv[c].x : 0.f;
+      tv[2 * c + 1] = live ? pv[c].y : 0.f;
+    } else {
+      tv[c] = live ? pv[c] : 0.f;
+    }
+  }
+  float4* dv = reinterpret_cast<float4*>(S.v + tid * MF_NPL);
+#pragma unroll
+  for (int i = 0; i < MF_NPL / 4; ++i)
+    dv[i] = make_float4(tv[4 * i], tv[4 * i + 1], tv[4 * i + 2], tv[4 * i + 3]);
+  // blocks touched by the group of 4 points (bit 4 u + v)
+  m |= (unsigned)__shfl_xor((int)m, 1);
+  m |= (unsigned)__shfl_xor((int)m, 2);
+  if ((tid & 3) == 0) S.km[tid >> 2] = m;
+}
+
+// Phase 2: this wave's blocks (U of one parity), every other pair of point
+// groups (8 points; a pair shares one decision per block and gets two
+// MFMAs per live block).  Lane i holds this wave's block mask of pair
+// par + 2 i, OR-ed over its two groups; the pairs with live blocks are
+// walked from a ballot, and the next pair's operands are read from LDS
+// while the current pair's MFMAs run.
+template <int NDIM>
+__device__ __forceinline__ void mf_phase2(mf32x4 (&acc)[16], const MfLds<NDIM>& S,
+                                          const MfLane& ln, int nb) {
+  using Sh = MfShape<NDIM>;
+  constexpr int RA = Sh::RA, RB = Sh::RB;
+  const int lane = ln.lane, xpar = ln.xpar, par = ln.par;
+  const int k4 = ln.k4, yy = ln.yy, zz = ln.zz, n16 = ln.n16;
+  const int nkg = (nb + 3) >> 2;
+  const int npg = (nkg + 1) >> 1;
+  const int pgl = par + 2 * lane;
+  unsigned lm = 0;
+  if (lane < Sh::MB / 16 && pgl < npg) {
+    unsigned mm = S.km[2 * pgl];
+    if (2 * pgl + 1 < nkg) mm |= S.km[2 * pgl + 1];
+    // this wave's blocks: u = xpar, xpar + 2, ...
+#pragma unroll
+    for (int j = 0; j < 4; ++j) lm |= ((mm >> (4 * (2 * j + xpar))) & 15u) << (4 * j);
+  }
+  uint64_t live = __ballot(lm != 0);
+  struct Ops {
+    float4 a4;  // U weights of this wave's 4 blocks u = 2 j + xpar
+    float4 b4;  // 3-D: {wy(yy), wy(yy + 4), wz(zz), wz(zz + 4)}; 2-D: wy(4 v + yy)
+    float bv;
+  };
+  // A (window) and B (value) operand parts of group g for this lane
+  auto fetch = [&](int g) {
+    const int pnt = g * 4 + k4;
+    Ops o;
+    if constexpr (NDIM == 3) {
+      o.a4 = *reinterpret_cast<const float4*>(S.wa + pnt * RA + 4 * xpar);
+      const float2 y2 = *reinterpret_cast<const float2*>(S.wb + pnt * RB + 2 * yy);
+      const float2 z2 = *reinterpret_cast<const float2*>(S.wb + pnt * RB + 8 + 2 * zz);
+      o.b4 = make_float4(y2.x, y2.y, z2.x, z2.y);
+    } else {
+      o.a4 = *reinterpret_cast<const float4*>(S.wa + pnt * RA + 16 * xpar + 4 * zz);
+      o.b4 = *reinterpret_cast<const float4*>(S.wb + pnt * RB + 4 * yy);
+    }
+    o.bv = S.v[pnt * MF_NPL + n16];
+    return o;
+  };
+  auto vweights = [&](const Ops& o, float* w4) {
+    if constexpr (NDIM == 3) {
+      w4[0] = o.b4.x * o.b4.z; w4[1] = o.b4.x * o.b4.w;
+      w4[2] = o.b4.y * o.b4.z; w4[3] = o.b4.y * o.b4.w;
+    } else {
+      w4[0] = o.b4.x; w4[1] = o.b4.y; w4[2] = o.b4.z; w4[3] = o.b4.w;
+    }
+  };
+  auto pair = [&](const Ops& o0, const Ops& o1, unsigned hm) {
+    float wv0[4], wv1[4];
+    vweights(o0, wv0);
+    vweights(o1, wv1);
+    const float wu0[4] = {o0.a4.x, o0.a4.y, o0.a4.z, o0.a4.w};
+    const float wu1[4] = {o1.a4.x, o1.a4.y, o1.a4.z, o1.a4.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if ((hm >> (4 * j + q)) & 1u) {
+          acc[4 * j + q] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+              wu0[j] * wv0[q], o0.bv, acc[4 * j + q], 0, 0, 0);
+          acc[4 * j + q] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+              wu1[j] * wv1[q], o1.bv, acc[4 * j + q], 0, 0, 0);
+        }
+  };
+  auto next = [&](int prev, unsigned* hm) {
+    const bool more = live != 0;
+    const int j = more ? __builtin_ctzll(live) : prev;
+    if (more) live &= live - 1;
+    *hm = more ? (unsigned)__builtin_amdgcn_readlane((int)lm, j) : 0u;
+    return j;
+  };
+  // two operand sets in turn: while one pair's MFMAs issue, the other set
+  // is being filled (no register copies between them)
+  if (live) {
+    unsigned ha, hb;
+    int ia = next(0, &ha);
+    Ops a0 = fetch(2 * (par + 2 * ia)), a1 = fetch(2 * (par + 2 * ia) + 1);
+    for (;;) {
+      const int ib = next(ia, &hb);
+      const Ops b0 = fetch(2 * (par + 2 * ib)), b1 = fetch(2 * (par + 2 * ib) + 1);
+      pair(a0, a1, ha);
+      if (!hb) break;
+      ia = next(ib, &ha);
+      a0 = fetch(2 * (par + 2 * ia));
+      a1 = fetch(2 * (par + 2 * ia) + 1);
+      pair(b0, b1, hb);
+      if (!ha) break;
+    }
+  }
+}
+
+// End of a tile: the two partial sums of each block meet in LDS (one wave
+// of a pair stores, the other adds) and the tile goes to HBM once,
+// coalesced along z (global atomics for split tiles).  Starts with the
+// barrier after which the batch buffers become the staging area.
+template <int NDIM, bool CPLX>
+__device__ __forceinline__ void mf_flush(const mf32x4 (&acc)[16], const MfLds<NDIM>& S,
+                                         const MfLane& ln, const int* origin,
+                                         const InterpGeom& G, float* __restrict__ out,
+                                         int64_t c0, int nc, bool shared) {
+  using T = typename Ch<CPLX>::T;
+  constexpr int CH = CPLX ? MF_NPL / 2 : MF_NPL;
+  float* stage = S.stage;
+  __syncthreads();
+  // Lane holds D rows 4 k4 + i of plane n16, i.e. cells (yy, zz) = (k4, i)
+  // of each of its blocks:
+  //   3-D block (u, v): x = u, y = 4 (v / 2) + k4, z = 4 (v % 2) + i
+  //   2-D block (u, v): y = 4 v + k4, z = 4 u + i
+  // stage cell index: 3-D 64 x + 8 y + z, 2-D 32 y + z
+  auto stage_at = [&](int j, int q) {
+    const int u = 2 * j + ln.xpar;
+    int cell;
+    if constexpr (NDIM == 3) cell = u * 64 + (4 * (q >> 1) + ln.k4) * 8 + 4 * (q & 1);
+    else cell = (4 * q + ln.k4) * 32 + 4 * u;
+    return reinterpret_cast<float4*>(stage + ln.n16 * MF_PSTR + cell);
+  };
+  if (ln.par == 0) {
+#pragma unroll
+    for (int b = 0; b < 16; ++b)
+      *stage_at(b >> 2, b & 3) = make_float4(acc[b][0], acc[b][1], acc[b][2], acc[b][3]);
+  }
+  __syncthreads();
+  if (ln.par == 1) {
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+      float4* sp = stage_at(b >> 2, b & 3);
+      float4 o4 = *sp;
+      o4.x += acc[b][0]; o4.y += acc[b][1]; o4.z += acc[b][2]; o4.w += acc[b][3];
+      *sp = o4;
+    }
+  }
+  __syncthreads();
+  T* __restrict__ o = reinterpret_cast<T*>(out);
+  for (int cell = ln.tid; cell < 512; cell += MF_THREADS) {
+    int lx, ly, lz;
+    if constexpr (NDIM == 3) {
+      lx = cell >> 6; ly = (cell >> 3) & 7; lz = cell & 7;
+    } else {
+      lx = 0; ly = cell >> 5; lz = cell & 31;
+    }
+    const int64_t gx = origin[0] + lx, gy = origin[1] + ly, gz = origin[2] + lz;
+    if (gx >= G.n[0] || gy >= G.n[1] || gz >= G.n[2]) continue;
+    const int64_t g0 = (gx * G.n[1] + gy) * G.n[2] + gz;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      if (c >= nc) break;
+      const int64_t gi = (c0 + c) * G.ncells + g0;
+      if constexpr (CPLX) {
+        const float a0 = stage[(2 * c) * MF_PSTR + cell];
+        const float a1 = stage[(2 * c + 1) * MF_PSTR + cell];
+        if (!shared) {
+          o[gi] = make_float2(a0, a1);
+        } else {
+          unsafeAtomicAdd(out + 2 * gi, a0);
+          unsafeAtomicAdd(out + 2 * gi + 1, a1);
+        }
+      } else {
+        const float a0 = stage[c * MF_PSTR + cell];
+        if (!shared) o[gi] = a0;
+        else unsafeAtomicAdd(out + gi, a0);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void tile_origin(const BinGeom& B, int tile, int* origin) {
+  int r = tile;
+  for (int d = 2; d >= 0; --d) {
+    origin[d] = (r % B.ntiles[d]) * B.tile[d];
+    r /= B.ntiles[d];
+  }
+}
+
+// One problem.  REC: the points' values come from the point-major records
+// (rec, caller order) and their coordinates from the caller's locs, instead
+// of the coil-major input and slocs.  Persistent: each workgroup walks
+// (work item, channel chunk) units and loads the first batch of its next
+// unit while it finishes the current one.
+template <int NDIM, bool CPLX, bool REC>
 __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4))) void grid_mfma_kernel(
     const float* __restrict__ vals, float* __restrict__ out, int64_t nbatch,
     InterpGeom G, BinGeom B, const int* __restrict__ perm,
@@ -929,28 +1239,21 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
     const float* __restrict__ locs, const float4* __restrict__ rec) {
   using C = Ch<CPLX>;
   using T = typename C::T;
+  constexpr int MB = MfShape<NDIM>::MB;
   constexpr int CH = CPLX ? MF_NPL / 2 : MF_NPL;  // channels per chunk
-  static_assert(MB == MF_THREADS && MB / 16 <= 64, "one point per thread; a lane per pair");
+  static_assert(MB <= MF_THREADS && MB / 16 <= 64, "one point per thread; a lane per pair");
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  float* s_wx = reinterpret_cast<float*>(lds_raw);  // [MB][MF_WROW], x order
-  float* s_wy = s_wx + MB * MF_WROW;                // [MB][MF_WROW], pair order
-  float* s_wz = s_wy + MB * MF_WROW;                // [MB][MF_WROW], pair order
-  float* s_v = s_wz + MB * MF_WROW;                 // [MB][16]
-  unsigned* s_km = reinterpret_cast<unsigned*>(s_v + MB * MF_NPL);  // [MB / 4]
-  float* stage = reinterpret_cast<float*>(lds_raw);  // [16][MF_PSTR], after batches
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int xpar = wv >> 1, par = wv & 1;  // x-plane parity, point-group parity
-  const int n16 = lane & 15, k4 = lane >> 4;  // B/D column, A column / D row group
-  const int yy = n16 >> 2, zz = n16 & 3;      // A row m = 4 yy + zz
+  const MfLds<NDIM> S(lds_raw);
+  const MfLane ln;
+  const int tid = ln.tid;
   const T* __restrict__ v = reinterpret_cast<const T*>(vals);
   const int nitems = *nitems_ptr;
 
   // raw inputs of this thread's point of a batch, loaded one batch ahead
   // (across channel chunks and work items too)
-  float pt[3] = {0.f, 0.f, 0.f};
+  float pt[NDIM];
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) pt[d] = 0.f;
   T pv[CH];
   auto load = [&](int start, int64_t c0, int b0, int nb) {
     if (tid < nb) {
@@ -959,7 +1262,7 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
         const int64_t p = perm[q];
         const float4* r = rec + ((c0 / CH) * B.npts + p) * MF_REC4;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) pt[d] = locs[p * 3 + d];
+        for (int d = 0; d < NDIM; ++d) pt[d] = locs[p * NDIM + d];
         float f[MF_NPL];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -973,7 +1276,7 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
         }
       } else {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) pt[d] = slocs[(int64_t)q * 3 + d];
+        for (int d = 0; d < NDIM; ++d) pt[d] = slocs[(int64_t)q * NDIM + d];
         const int p = perm[q];
 #pragma unroll
         for (int c = 0; c < CH; ++c)
@@ -988,7 +1291,6 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
   load(item.y, 0, 0, item.z < MB ? item.z : MB);
   for (;;) {
     const int start = item.y, count = item.z;
-    const bool shared = item.w != 0;
     const int nc = (int)(nbatch - c0 < CH ? nbatch - c0 : CH);
     // the next (item, channel chunk) of this workgroup
     int64_t it_n = it, c0_n = c0 + CH;
@@ -1000,194 +1302,26 @@ __global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4
     }
     const bool has_next = it_n < nitems;
     int origin[3];
-    {
-      int r = item.x;
-      for (int d = 2; d >= 0; --d) {
-        origin[d] = (r % B.ntiles[d]) * B.tile[d];
-        r /= B.ntiles[d];
-      }
-    }
-    {
-      mf32x4 acc[16];
+    tile_origin(B, item.x, origin);
+    mf32x4 acc[16];
 #pragma unroll
-      for (int b = 0; b < 16; ++b) acc[b] = (mf32x4){0.f, 0.f, 0.f, 0.f};
-      bool ahead = false;  // the next unit's first batch is loaded
-      for (int b0 = 0; b0 < count; b0 += MB) {
-        const int nb = count - b0 < MB ? count - b0 : MB;
-        __syncthreads();  // previous batch's MFMA reads (or the write-out) are done
-        // ---- phase 1, lanes = points
-        {
-          const bool live = tid < nb;
-          unsigned px = 0, py = 0, pz = 0;
-          if (live) {
-            px = mf_axis_place<false>(pt[0], 0, origin[0], G, s_wx + tid * MF_WROW);
-            py = mf_axis_place<true>(pt[1], 1, origin[1], G, s_wy + tid * MF_WROW);
-            pz = mf_axis_place<true>(pt[2], 2, origin[2], G, s_wz + tid * MF_WROW);
-          } else {
-            const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-            float4* dx = reinterpret_cast<float4*>(s_wx + tid * MF_WROW);
-            float4* dy = reinterpret_cast<float4*>(s_wy + tid * MF_WROW);
-            float4* dz = reinterpret_cast<float4*>(s_wz + tid * MF_WROW);
-            dx[0] = dx[1] = dy[0] = dy[1] = dz[0] = dz[1] = z4;
-          }
-          float tv[MF_NPL];
-#pragma unroll
-          for (int c = 0; c < CH; ++c) {
-            if constexpr (CPLX) {
-              tv[2 * c] = live ? pv[c].x : 0.f;
-              tv[2 * c + 1] = live ? pv[c].y : 0.f;
-            } else {
-              tv[c] = live ? pv[c] : 0.f;
-            }
-          }
-          float4* dv = reinterpret_cast<float4*>(s_v + tid * MF_NPL);
-#pragma unroll
-          for (int i = 0; i < MF_NPL / 4; ++i)
-            dv[i] = make_float4(tv[4 * i], tv[4 * i + 1], tv[4 * i + 2], tv[4 * i + 3]);
-          // blocks touched: bit 4 x + (2 py + pz) for x plane x, (y, z) patch
-          unsigned m = tile_blocks32(px, py, pz);
-          m |= (unsigned)__shfl_xor((int)m, 1);
-          m |= (unsigned)__shfl_xor((int)m, 2);
-          if ((tid & 3) == 0) s_km[tid >> 2] = m;
-        }
-        if (b0 + MB < count) {
-          load(start, c0, b0 + MB, count - b0 - MB < MB ? count - b0 - MB : MB);
-        } else if (has_next) {
-          load(item_n.y, c0_n, 0, item_n.z < MB ? item_n.z : MB);
-          ahead = true;
-        }
-        __syncthreads();
-        // ---- phase 2: this wave's x planes (one parity), every other pair
-        // of point groups (8 points; a pair shares one decision per block
-        // and gets two MFMAs per live block).  Lane i holds this wave's
-        // block mask of pair par + 2 i, OR-ed over its two groups; the pairs
-        // with live blocks are walked from a ballot, and the next pair's
-        // operands are read from LDS while the current pair's MFMAs run.
-        {
-          const int nkg = (nb + 3) >> 2;
-          const int npg = (nkg + 1) >> 1;
-          const int pgl = par + 2 * lane;
-          unsigned lm = 0;
-          if (lane < MB / 16 && pgl < npg) {
-            unsigned mm = s_km[2 * pgl];
-            if (2 * pgl + 1 < nkg) mm |= s_km[2 * pgl + 1];
-            // this wave's blocks: x planes xpar, xpar + 2, ...
-#pragma unroll
-            for (int j = 0; j < 4; ++j) lm |= ((mm >> (4 * (2 * j + xpar))) & 15u) << (4 * j);
-          }
-          uint64_t live = __ballot(lm != 0);
-          struct Ops {
-            float4 x4;
-            float2 y2, z2;
-            float bv;
-          };
-          // A (window) and B (value) operand parts of group g for this lane
-          auto fetch = [&](int g) {
-            const int pnt = g * 4 + k4;
-            Ops o;
-            o.x4 = *reinterpret_cast<const float4*>(s_wx + pnt * MF_WROW + 4 * xpar);
-            o.y2 = *reinterpret_cast<const float2*>(s_wy + pnt * MF_WROW + 2 * yy);
-            o.z2 = *reinterpret_cast<const float2*>(s_wz + pnt * MF_WROW + 2 * zz);
-            o.bv = s_v[pnt * MF_NPL + n16];
-            return o;
-          };
-          auto pair = [&](const Ops& o0, const Ops& o1, unsigned hm) {
-            const float wyz0[4] = {o0.y2.x * o0.z2.x, o0.y2.x * o0.z2.y, o0.y2.y * o0.z2.x,
-                                   o0.y2.y * o0.z2.y};
-            const float wyz1[4] = {o1.y2.x * o1.z2.x, o1.y2.x * o1.z2.y, o1.y2.y * o1.z2.x,
-                                   o1.y2.y * o1.z2.y};
-            const float wxs0[4] = {o0.x4.x, o0.x4.y, o0.x4.z, o0.x4.w};
-            const float wxs1[4] = {o1.x4.x, o1.x4.y, o1.x4.z, o1.x4.w};
-#pragma unroll
-            for (int xl = 0; xl < 4; ++xl)
-#pragma unroll
-              for (int p = 0; p < 4; ++p)
-                if ((hm >> (4 * xl + p)) & 1u) {
-                  acc[4 * xl + p] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                      wxs0[xl] * wyz0[p], o0.bv, acc[4 * xl + p], 0, 0, 0);
-                  acc[4 * xl + p] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                      wxs1[xl] * wyz1[p], o1.bv, acc[4 * xl + p], 0, 0, 0);
-                }
-          };
-          auto next = [&](int prev, unsigned* hm) {
-            const bool more = live != 0;
-            const int j = more ? __builtin_ctzll(live) : prev;
-            if (more) live &= live - 1;
-            *hm = more ? (unsigned)__builtin_amdgcn_readlane((int)lm, j) : 0u;
-            return j;
-          };
-          // two operand sets in turn: while one pair's MFMAs issue, the
-          // other set is being filled (no register copies between them)
-          if (live) {
-            unsigned ha, hb;
-            int ia = next(0, &ha);
-            Ops a0 = fetch(2 * (par + 2 * ia)), a1 = fetch(2 * (par + 2 * ia) + 1);
-            for (;;) {
-              const int ib = next(ia, &hb);
-              const Ops b0 = fetch(2 * (par + 2 * ib)), b1 = fetch(2 * (par + 2 * ib) + 1);
-              pair(a0, a1, ha);
-              if (!hb) break;
-              ia = next(ib, &ha);
-              a0 = fetch(2 * (par + 2 * ia));
-              a1 = fetch(2 * (par + 2 * ia) + 1);
-              pair(b0, b1, hb);
-              if (!ha) break;
-            }
-          }
-        }
-      }
-      if (!ahead && has_next) load(item_n.y, c0_n, 0, item_n.z < MB ? item_n.z : MB);
-      __syncthreads();  // batch buffers become the staging area
-      // ---- combine the pair's partial sums: lane holds cells
-      // (x, 4 py + k4, 4 pz + i), i < 4, of plane n16
-      auto stage_at = [&](int xl, int p) {
-        const int x = 2 * xl + xpar, y = 4 * (p >> 1) + k4, z0 = 4 * (p & 1);
-        return reinterpret_cast<float4*>(stage + n16 * MF_PSTR + x * 64 + y * 8 + z0);
-      };
-      if (par == 0) {
-#pragma unroll
-        for (int b = 0; b < 16; ++b)
-          *stage_at(b >> 2, b & 3) = make_float4(acc[b][0], acc[b][1], acc[b][2], acc[b][3]);
+    for (int b = 0; b < 16; ++b) acc[b] = (mf32x4){0.f, 0.f, 0.f, 0.f};
+    bool ahead = false;  // the next unit's first batch is loaded
+    for (int b0 = 0; b0 < count; b0 += MB) {
+      const int nb = count - b0 < MB ? count - b0 : MB;
+      __syncthreads();  // previous batch's MFMA reads (or the write-out) are done
+      mf_phase1<NDIM, CPLX>(S, ln, nb, pt, pv, origin, G);
+      if (b0 + MB < count) {
+        load(start, c0, b0 + MB, count - b0 - MB < MB ? count - b0 - MB : MB);
+      } else if (has_next) {
+        load(item_n.y, c0_n, 0, item_n.z < MB ? item_n.z : MB);
+        ahead = true;
       }
       __syncthreads();
-      if (par == 1) {
-#pragma unroll
-        for (int b = 0; b < 16; ++b) {
-          float4* sp = stage_at(b >> 2, b & 3);
-          float4 o4 = *sp;
-          o4.x += acc[b][0]; o4.y += acc[b][1]; o4.z += acc[b][2]; o4.w += acc[b][3];
-          *sp = o4;
-        }
-      }
-      __syncthreads();
-      // ---- write the tile: coalesced along z
-      T* __restrict__ o = reinterpret_cast<T*>(out);
-      for (int cell = tid; cell < 512; cell += MF_THREADS) {
-        const int lx = cell >> 6, ly = (cell >> 3) & 7, lz = cell & 7;
-        const int64_t gx = origin[0] + lx, gy = origin[1] + ly, gz = origin[2] + lz;
-        if (gx >= G.n[0] || gy >= G.n[1] || gz >= G.n[2]) continue;
-        const int64_t g0 = (gx * G.n[1] + gy) * G.n[2] + gz;
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-          if (c >= nc) break;
-          const int64_t gi = (c0 + c) * G.ncells + g0;
-          if constexpr (CPLX) {
-            const float a0 = stage[(2 * c) * MF_PSTR + cell];
-            const float a1 = stage[(2 * c + 1) * MF_PSTR + cell];
-            if (!shared) {
-              o[gi] = make_float2(a0, a1);
-            } else {
-              unsafeAtomicAdd(out + 2 * gi, a0);
-              unsafeAtomicAdd(out + 2 * gi + 1, a1);
-            }
-          } else {
-            const float a0 = stage[c * MF_PSTR + cell];
-            if (!shared) o[gi] = a0;
-            else unsafeAtomicAdd(out + gi, a0);
-          }
-        }
-      }
+      mf_phase2<NDIM>(acc, S, ln, nb);
     }
+    if (!ahead && has_next) load(item_n.y, c0_n, 0, item_n.z < MB ? item_n.z : MB);
+    mf_flush<NDIM, CPLX>(acc, S, ln, origin, G, out, c0, nc, item.w != 0);
     if (!has_next) break;
     it = it_n;
     item = item_n;
@@ -1276,10 +1410,10 @@ static size_t grid_lds_bytes(const BinGeom& B, int planes, int nt) {
   return (size_t)grid_lds_layout(B, planes, nt).bytes;
 }
 
-// The plan resolves the accumulator (matrix cores for 3-D 8x8x8 tiles
-// unless asked otherwise).
+// The plan resolves the accumulator (matrix cores for 3-D 8x8x8 and 2-D
+// 16x32 tiles unless asked otherwise).
 static bool use_grid_mfma(const InterpGeom& G, const tnl_plan_layout& L) {
-  return G.ndim == 3 && L.grid_accum == TNL_GRID_ACCUM_MFMA;
+  return G.ndim >= 2 && L.grid_accum == TNL_GRID_ACCUM_MFMA;
 }
 
 bool binned_supported(const InterpGeom& G, const tnl_plan_layout& L,
@@ -1327,7 +1461,7 @@ static int resident_workgroups(const void* kern, int threads, size_t lds) {
   return per_cu * ncu;
 }
 
-template <bool CPLX>
+template <int NDIM, bool CPLX>
 static int launch_grid_mfma(const float* vals, const float* locs, float* out, int64_t nbatch,
                             const InterpGeom& G, const tnl_plan_layout& L,
                             const char* ws, float* scratch, hipStream_t s) {
@@ -1342,13 +1476,14 @@ static int launch_grid_mfma(const float* vals, const float* locs, float* out, in
         vals, L.npts, nbatch, rec);
     TNL_LAUNCH_CHECK("pack_points");
   }
-  const auto kern = rec ? grid_mfma_kernel<CPLX, true> : grid_mfma_kernel<CPLX, false>;
-  if (int rc = allow_lds((const void*)kern, MF_LDS)) return rc;
-  int64_t nwg = resident_workgroups((const void*)kern, MF_THREADS, MF_LDS);
+  const auto kern = rec ? grid_mfma_kernel<NDIM, CPLX, true> : grid_mfma_kernel<NDIM, CPLX, false>;
+  constexpr size_t lds = mf_lds_bytes<NDIM>();
+  if (int rc = allow_lds((const void*)kern, lds)) return rc;
+  int64_t nwg = resident_workgroups((const void*)kern, MF_THREADS, lds);
   nwg = (nwg + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
   const int64_t need = (L.item_capacity + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
   if (nwg > need) nwg = need;
-  kern<<<(unsigned)nwg, MF_THREADS, MF_LDS, s>>>(
+  kern<<<(unsigned)nwg, MF_THREADS, lds, s>>>(
       vals, out, nbatch, G, B, (const int*)(ws + L.off_perm),
       (const float*)(ws + L.off_slocs), (const int*)(ws + L.off_tlist), items, nitems, locs,
       rec);
@@ -1357,7 +1492,7 @@ static int launch_grid_mfma(const float* vals, const float* locs, float* out, in
 }
 
 int64_t grid_scratch_bytes(const tnl_plan_layout& L, int64_t nbatch, int is_complex) {
-  if (L.ndim != 3 || L.grid_accum != TNL_GRID_ACCUM_MFMA || nbatch <= 0 || L.npts <= 0)
+  if (L.ndim < 2 || L.grid_accum != TNL_GRID_ACCUM_MFMA || nbatch <= 0 || L.npts <= 0)
     return 0;
   const int64_t ch = is_complex ? MF_NPL / 2 : MF_NPL;
   return (nbatch + ch - 1) / ch * L.npts * MF_REC4 * (int64_t)sizeof(float4);
@@ -1439,8 +1574,8 @@ static int dispatch_nt(bool adjoint, const float* vals, const float* locs, float
                        hipStream_t s) {
   int nt = G.ntap[0];
   for (int d = 1; d < 3; ++d) nt = G.ntap[d] > nt ? G.ntap[d] : nt;
-  if (NDIM == 3 && adjoint && use_grid_mfma(G, L))
-    return launch_grid_mfma<CPLX>(vals, locs, out, nbatch, G, L, ws, scratch, s);
+  if (adjoint && use_grid_mfma(G, L))
+    return launch_grid_mfma<NDIM, CPLX>(vals, locs, out, nbatch, G, L, ws, scratch, s);
   if (nt <= 5) return dispatch_cc<NDIM, CPLX, 5>(adjoint, vals, out, nbatch, G, L, ws, s);
   return dispatch_cc<NDIM, CPLX, 8>(adjoint, vals, out, nbatch, G, L, ws, s);
 }
@@ -1558,6 +1693,10 @@ int batch_prepare(const InterpGeom* G, const tnl_batch_problem* P, int nprob,
   L->nt = nt <= 5 ? 5 : 8;
   if (L->ndim == 3 && L->nt > 5)
     return fail(TNL_EINVAL, "3-D gridding covers at most 5 taps per axis");
+  L->grid_mfma = 1;
+  for (int i = 0; i < nprob; ++i)
+    if (P[i].layout->grid_accum != TNL_GRID_ACCUM_MFMA) L->grid_mfma = 0;
+  L->reserved = 0;
   // channels per workgroup: the largest power of two (<= 8) whose LDS fits
   // every problem under the smallest budget
   auto max_u = [&](int cc) {
@@ -1643,6 +1782,84 @@ static int launch_ungrid_batched(const tnl_batch_layout& L, const BatchDesc* D,
   return TNL_OK;
 }
 
+// Matrix-core gridding of a batch: one persistent launch walks the work
+// items of every problem (global item index -> problem by binary search,
+// unused capacity skipped), each item with all its channel chunks, through
+// the phases of grid_mfma_kernel.  Values are read from each problem's
+// coil-major input (no point-major records).
+template <int NDIM, bool CPLX>
+__global__ __launch_bounds__(MF_THREADS) __attribute__((amdgpu_waves_per_eu(3, 4))) void grid_mfma_batched_kernel(
+    const BatchDesc* __restrict__ D, int nprob, int64_t total,
+    const float* __restrict__ pbase, float* __restrict__ gbase) {
+  using C = Ch<CPLX>;
+  using T = typename C::T;
+  constexpr int MB = MfShape<NDIM>::MB;
+  constexpr int CH = CPLX ? MF_NPL / 2 : MF_NPL;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const MfLds<NDIM> S(lds_raw);
+  const MfLane ln;
+  float pt[NDIM];
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) pt[d] = 0.f;
+  T pv[CH];
+  for (int64_t g = xcd_remap(blockIdx.x, gridDim.x); g < total; g += gridDim.x) {
+    const BatchDesc& d = D[find_problem(D, nprob, g, true)];
+    const int64_t il = g - d.item_begin;
+    if (il >= *d.nitems) continue;
+    const int4 item = d.items[il];
+    const int start = item.y, count = item.z;
+    const T* __restrict__ v = reinterpret_cast<const T*>(pbase + d.pts_off);
+    float* out = gbase + d.grid_off;
+    const int64_t nbatch = d.nbatch;
+    int origin[3];
+    tile_origin(d.Bg, item.x, origin);
+    auto load = [&](int64_t c0, int b0, int nb) {
+      if (ln.tid < nb) {
+        const int q = d.tlist[start + b0 + ln.tid];
+#pragma unroll
+        for (int k = 0; k < NDIM; ++k) pt[k] = d.slocs[(int64_t)q * NDIM + k];
+        const int p = d.perm[q];
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+          pv[c] = c0 + c < nbatch ? v[(c0 + c) * d.Bg.npts + p] : C::zero();
+      }
+    };
+    for (int64_t c0 = 0; c0 < nbatch; c0 += CH) {
+      const int nc = (int)(nbatch - c0 < CH ? nbatch - c0 : CH);
+      mf32x4 acc[16];
+#pragma unroll
+      for (int b = 0; b < 16; ++b) acc[b] = (mf32x4){0.f, 0.f, 0.f, 0.f};
+      load(c0, 0, count < MB ? count : MB);
+      for (int b0 = 0; b0 < count; b0 += MB) {
+        const int nb = count - b0 < MB ? count - b0 : MB;
+        __syncthreads();  // previous batch's MFMA reads (or the write-out) are done
+        mf_phase1<NDIM, CPLX>(S, ln, nb, pt, pv, origin, d.G);
+        if (b0 + MB < count) load(c0, b0 + MB, count - b0 - MB < MB ? count - b0 - MB : MB);
+        __syncthreads();
+        mf_phase2<NDIM>(acc, S, ln, nb);
+      }
+      mf_flush<NDIM, CPLX>(acc, S, ln, origin, d.G, out, c0, nc, item.w != 0);
+    }
+  }
+}
+
+template <int NDIM, bool CPLX>
+static int launch_grid_mfma_batched(const tnl_batch_layout& L, const BatchDesc* D,
+                                    const float* in, float* out, hipStream_t s) {
+  grid_zero_shared_batched_kernel<CPLX><<<1024, 256, 0, s>>>(D, L.nprob, L.total_items, out);
+  TNL_LAUNCH_CHECK("grid_zero_shared_batched");
+  const auto kern = grid_mfma_batched_kernel<NDIM, CPLX>;
+  constexpr size_t lds = mf_lds_bytes<NDIM>();
+  if (int rc = allow_lds((const void*)kern, lds)) return rc;
+  int64_t nwg = resident_workgroups((const void*)kern, MF_THREADS, lds);
+  nwg = (nwg + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+  const int64_t need = (L.total_items + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+  if (nwg > need) nwg = need;
+  kern<<<(unsigned)nwg, MF_THREADS, lds, s>>>(D, L.nprob, L.total_items, in, out);
+  TNL_LAUNCH_CHECK("grid_mfma_batched");
+  return TNL_OK;
+}
+
 template <int NDIM, bool CPLX, int CC, int NT>
 static int launch_grid_batched(const tnl_batch_layout& L, const BatchDesc* D,
                                const float* in, float* out, hipStream_t s) {
@@ -1691,6 +1908,13 @@ int batch_run(bool adjoint, const tnl_batch_layout& L, const void* ws,
               const float* in, float* out, hipStream_t s) {
   const BatchDesc* D = (const BatchDesc*)ws;
   const bool cx = L.is_complex != 0;
+  if (adjoint && L.grid_mfma) {
+    if (L.ndim == 3)
+      return cx ? launch_grid_mfma_batched<3, true>(L, D, in, out, s)
+                : launch_grid_mfma_batched<3, false>(L, D, in, out, s);
+    return cx ? launch_grid_mfma_batched<2, true>(L, D, in, out, s)
+              : launch_grid_mfma_batched<2, false>(L, D, in, out, s);
+  }
   if (L.ndim == 3) {
     if (L.nt != 5) return fail(TNL_EINVAL, "3-D batches cover at most 5 taps per axis");
     return cx ? batch_dispatch_cc<3, true, 5>(adjoint, L, D, in, out, s)

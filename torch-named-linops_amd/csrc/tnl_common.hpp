@@ -198,11 +198,12 @@ __device__ __forceinline__ TapRange tap_range(float t, float half, int ntap,
   return r;
 }
 
-// Cells [0, 8) of a tile starting at cell `origin` of an axis of n cells
-// that the taps of coordinate t fall on (bit l = cell origin + l; taps wrap
-// under circular padding).  The tile-local footprint of one axis.
-__device__ __forceinline__ unsigned tile_cells8(float t, float half, int ntap,
-                                                int64_t n, int pad, int origin) {
+// Cells [0, tl) (tl <= 32) of a tile starting at cell `origin` of an axis
+// of n cells that the taps of coordinate t fall on (bit l = cell origin +
+// l; taps wrap under circular padding).  The tile-local footprint of one
+// axis.
+__device__ __forceinline__ unsigned tile_cells(float t, float half, int ntap,
+                                               int64_t n, int pad, int origin, int tl) {
   const TapRange r = tap_range(t, half, ntap, n, pad);
   int64_t a = r.g0;
   if (pad == TNL_PAD_CIRCULAR) {
@@ -215,8 +216,23 @@ __device__ __forceinline__ unsigned tile_cells8(float t, float half, int ntap,
   for (int k = 0; k < r.count; ++k) {
     int l = lo + k;
     if (pad == TNL_PAD_CIRCULAR && origin + l >= n) l -= (int)n;
-    if ((unsigned)l < 8u) m |= 1u << l;
+    if ((unsigned)l < (unsigned)tl) m |= 1u << l;
   }
+  return m;
+}
+
+// Blocks of a 16x32 (y, z) tile that a footprint with tile-local cell masks
+// (my: 16 cells, mz: 32 cells) touches: bit 4 u + v for the 4x4 block of
+// cells 4 u .. 4 u + 3 along z and 4 v .. 4 v + 3 along y.
+__device__ __forceinline__ unsigned tile_blocks32_2d(unsigned my, unsigned mz) {
+  unsigned yb = 0;
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    if ((my >> (4 * v)) & 15u) yb |= 1u << v;
+  unsigned m = 0;
+#pragma unroll
+  for (int u = 0; u < 8; ++u)
+    if ((mz >> (4 * u)) & 15u) m |= yb << (4 * u);
   return m;
 }
 

@@ -40,7 +40,7 @@ static int default_u_tile(int ndim) {
 static void default_g_tile(int ndim, int t[3]) {
   // innermost (contiguous) axis longest: coalesced tile writes
   if (ndim == 3) { t[0] = 8; t[1] = 8; t[2] = 8; }
-  else if (ndim == 2) { t[0] = 1; t[1] = 16; t[2] = 64; }
+  else if (ndim == 2) { t[0] = 1; t[1] = 16; t[2] = 32; }
   else { t[0] = 1; t[1] = 1; t[2] = 4096; }
 }
 
@@ -112,11 +112,14 @@ int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
     const int ga = tune ? tune->grid_accum : TNL_GRID_ACCUM_AUTO;
     if (ga < TNL_GRID_ACCUM_AUTO || ga > TNL_GRID_ACCUM_MFMA)
       return fail(TNL_EINVAL, "unknown grid_accum");
-    const bool tiles8 = ndim == 3 && L->g_tile[0] == 8 && L->g_tile[1] == 8 && L->g_tile[2] == 8;
-    if (ga == TNL_GRID_ACCUM_MFMA && !tiles8)
-      return fail(TNL_EINVAL, "the MFMA gridding accumulator needs a 3-D plan with 8x8x8 tiles");
+    const bool mf_tiles =
+        (ndim == 3 && L->g_tile[0] == 8 && L->g_tile[1] == 8 && L->g_tile[2] == 8) ||
+        (ndim == 2 && L->g_tile[1] == 16 && L->g_tile[2] == 32);
+    if (ga == TNL_GRID_ACCUM_MFMA && !mf_tiles)
+      return fail(TNL_EINVAL,
+                  "the MFMA gridding accumulator needs 8x8x8 tiles (3-D) or 16x32 tiles (2-D)");
     L->grid_accum = ga == TNL_GRID_ACCUM_AUTO
-                        ? (tiles8 ? TNL_GRID_ACCUM_MFMA : TNL_GRID_ACCUM_LDS)
+                        ? (mf_tiles ? TNL_GRID_ACCUM_MFMA : TNL_GRID_ACCUM_LDS)
                         : ga;
   }
   L->touch_capacity = npts * touch_per_pt;
@@ -509,11 +512,21 @@ __global__ __launch_bounds__(256) void item_sort_kernel(const int4* __restrict__
       int q = 0;
       if (i < count) {
         q = tlist[start + i];
-        unsigned m[3];
-        for (int d = 0; d < 3; ++d)
-          m[d] = tile_cells8(slocs[(int64_t)q * 3 + d], P.half[d], P.ntap[d], P.n[d], P.pad,
-                             origin[d]);
-        k = ((unsigned long long)tile_blocks32(m[0], m[1], m[2]) << 32) | (unsigned)perm[q];
+        unsigned mask;
+        if (P.ndim == 3) {
+          unsigned m[3];
+          for (int d = 0; d < 3; ++d)
+            m[d] = tile_cells(slocs[(int64_t)q * 3 + d], P.half[d], P.ntap[d], P.n[d], P.pad,
+                              origin[d], 8);
+          mask = tile_blocks32(m[0], m[1], m[2]);
+        } else {
+          const unsigned my = tile_cells(slocs[(int64_t)q * 2], P.half[1], P.ntap[1], P.n[1],
+                                         P.pad, origin[1], 16);
+          const unsigned mz = tile_cells(slocs[(int64_t)q * 2 + 1], P.half[2], P.ntap[2],
+                                         P.n[2], P.pad, origin[2], 32);
+          mask = tile_blocks32_2d(my, mz);
+        }
+        k = ((unsigned long long)mask << 32) | (unsigned)perm[q];
       }
       key[i] = k;
       val[i] = q;

@@ -136,6 +136,8 @@ class BatchLayout(ctypes.Structure):
         ("ungrid_cc", ctypes.c_int32),
         ("grid_cc", ctypes.c_int32),
         ("ungrid_chunks", ctypes.c_int32),
+        ("grid_mfma", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
         ("total_sub", ctypes.c_int64),
         ("total_items", ctypes.c_int64),
         ("ungrid_lds", ctypes.c_int64),
