@@ -144,3 +144,32 @@ def test_cg_reference_semantics_cpu():
     s3 = tl.DeviceCG(lambda v: A @ v, M=lambda r: Ainv @ r, check_every=1)
     s3.solve(b, tol=1e-9, max_iters=5)
     assert s3.iters <= 2
+
+
+def test_shifted_layout_replaces_centring_shifts():
+    """The batched / SENSE normal operators skip fftshift / ifftshift: the
+    image written at the wrapped cells of shifted_layout and transformed
+    without shifts equals the centred transform relabelled by N // 2 (even
+    and odd sizes, 1-D to 3-D), and the adjoint crop reads the same cells."""
+    from torchlinops_amd.linops.nufft import cfftn, cifftn, shifted_layout
+
+    g = torch.Generator().manual_seed(3)
+    for im, pd in (((5, 6), (10, 12)), ((7,), (15,)), ((4, 5, 3), (8, 9, 7))):
+        x = torch.randn(2, *im, dtype=torch.complex64, generator=g)
+        nd = len(im)
+        dim = tuple(range(-nd, 0))
+        ix, shift = shifted_layout(im, pd, "cpu")
+        full = torch.zeros(2, *pd, dtype=torch.complex64)
+        full[(Ellipsis,) + tuple(tl.linops.crop_slice_from_pad(tl.linops.pad_to_size(im, pd)))] = x
+        want = cfftn(full, dim=dim)
+        wrapped = torch.zeros(2, *pd, dtype=torch.complex64)
+        wrapped[(Ellipsis,) + ix] = x
+        got = torch.fft.fftn(wrapped, dim=dim, norm="ortho")
+        rolled = torch.roll(got, shifts=[int(s) for s in shift], dims=dim)
+        assert torch.allclose(rolled, want, atol=1e-5)
+        y = torch.randn(2, *pd, dtype=torch.complex64, generator=g)
+        crop = tuple(tl.linops.crop_slice_from_pad(tl.linops.pad_to_size(im, pd)))
+        want_adj = cifftn(y, dim=dim)[(Ellipsis,) + crop]
+        y_unshifted = torch.roll(y, shifts=[-int(s) for s in shift], dims=dim)
+        got_adj = torch.fft.ifftn(y_unshifted, dim=dim, norm="ortho")[(Ellipsis,) + ix]
+        assert torch.allclose(got_adj, want_adj, atol=1e-5)

@@ -23,8 +23,7 @@ import torch
 
 from .._lib import SegLayout, check, i64_array, lib, stream_of
 from ..functional._batch import BatchedInterp
-from ..linops.basic import crop_slice_from_pad, pad_to_size
-from ..linops.nufft import NUFFT, cfftn, cifftn
+from ..linops.nufft import NUFFT, shifted_layout
 
 __all__ = ["BatchedNUFFTNormal", "SegmentedCG"]
 
@@ -49,17 +48,22 @@ class BatchedNUFFTNormal:
         self.ncoils = [int(c) for c in ncoils]
         self.padded = [tuple(int(s * oversamp) for s in im) for im in self.im_sizes]
         betas, plocs = [], []
-        self.apod, self.crop = [], []
+        self.apod, self.index = [], []
         for i in range(n):
             w = float(widths[i])
             beta = float(NUFFT.beta(w, oversamp))
             betas.append({"beta": beta})
-            plocs.append(NUFFT.prep_locs(locs[i], self.im_sizes[i], self.padded[i]))
+            # no centring shifts: the image goes to the wrapped cells and the
+            # locations are relabelled (see shifted_layout)
+            index, shift = shifted_layout(self.im_sizes[i], self.padded[i], self.device)
+            self.index.append(index)
+            pl = NUFFT.prep_locs(locs[i], self.im_sizes[i], self.padded[i])
+            pad = torch.tensor(self.padded[i], dtype=torch.float32, device=pl.device)
+            plocs.append(torch.remainder(pl - shift.to(pl.device), pad).contiguous())
             apod = NUFFT.apodize_weights(self.im_sizes[i], self.padded[i], oversamp, w, beta)
             scale = w**self.ndim * (prod(self.im_sizes[i]) / prod(self.padded[i])) ** 0.5
             # forward scale 1/scale and adjoint scale 1/scale fold into one factor
             self.apod.append((apod / scale**2).to(self.device, torch.complex64))
-            self.crop.append(tuple(crop_slice_from_pad(pad_to_size(self.im_sizes[i], self.padded[i]))))
         self.apod_fwd = [NUFFT.apodize_weights(self.im_sizes[i], self.padded[i], oversamp,
                                                float(widths[i]), betas[i]["beta"])
                          .to(self.device, torch.complex64) for i in range(n)]
@@ -85,14 +89,14 @@ class BatchedNUFFTNormal:
     def __call__(self, x: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
         out = self.new_image() if out is None else out
         gv = self.interp.grid_views(self.G)
-        for xi, gi, ap, cr in zip(self.image_views(x), gv, self.apod_fwd, self.crop):
+        for xi, gi, ap, ix in zip(self.image_views(x), gv, self.apod_fwd, self.index):
             gi.zero_()
-            gi[(slice(None),) + cr] = xi * ap
-            gi.copy_(cfftn(gi, dim=self.dim, norm="ortho"))
+            gi[(slice(None),) + ix] = xi * ap
+            gi.copy_(torch.fft.fftn(gi, dim=self.dim, norm="ortho"))
         self.interp.ungrid(self.G, out=self.P)
         self.interp.grid(self.P, out=self.G)
-        for oi, gi, ap, cr in zip(self.image_views(out), gv, self.apod, self.crop):
-            oi.copy_(cifftn(gi, dim=self.dim, norm="ortho")[(slice(None),) + cr] * ap)
+        for oi, gi, ap, ix in zip(self.image_views(out), gv, self.apod, self.index):
+            oi.copy_(torch.fft.ifftn(gi, dim=self.dim, norm="ortho")[(slice(None),) + ix] * ap)
         return out
 
 

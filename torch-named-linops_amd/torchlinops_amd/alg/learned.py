@@ -37,8 +37,7 @@ from torch.autograd import Function
 
 from .._lib import check, lib, require_device, stream_of
 from ..functional._interp import interpolate, interpolate_adjoint
-from ..linops.basic import crop_slice_from_pad, pad_to_size
-from ..linops.nufft import NUFFT, cfftn, cifftn
+from ..linops.nufft import NUFFT, shifted_layout
 from .cg import DeviceCG, inner
 
 __all__ = ["SenseNormal", "PrecondNet", "conv3d", "to_channel_last", "pack_conv_weights",
@@ -189,14 +188,17 @@ class SenseNormal:
         self.padded = tuple(int(s * oversamp) for s in self.im_size)
         self.width = float(width)
         self.beta = float(NUFFT.beta(width, oversamp))
-        self.locs = NUFFT.prep_locs(locs.reshape(-1, self.ndim), self.im_size,
-                                    self.padded).contiguous()
+        # no centring shifts: the image goes to the wrapped cells and the
+        # locations are relabelled (see shifted_layout)
+        self.index, shift = shifted_layout(self.im_size, self.padded, maps.device)
+        pl = NUFFT.prep_locs(locs.reshape(-1, self.ndim), self.im_size, self.padded)
+        pad = torch.tensor(self.padded, dtype=torch.float32, device=pl.device)
+        self.locs = torch.remainder(pl - shift.to(pl.device), pad).contiguous()
         apod = NUFFT.apodize_weights(self.im_size, self.padded, oversamp, width, self.beta)
         scale = width**self.ndim * (math.prod(self.im_size) / math.prod(self.padded)) ** 0.5
         self.apod = apod.to(maps.device, torch.complex64)
         self.apod2 = (apod / scale**2).to(maps.device, torch.complex64)
         self.maps = maps.to(torch.complex64)
-        self.crop = tuple(crop_slice_from_pad(pad_to_size(self.im_size, self.padded)))
         self.lam = float(lam)
         self.kp = {"beta": self.beta}
 
@@ -205,11 +207,11 @@ class SenseNormal:
         sx = self.maps * x.unsqueeze(-self.ndim - 1) * self.apod  # [..., C, *im]
         g = torch.zeros(*sx.shape[: sx.ndim - self.ndim], *self.padded, dtype=torch.complex64,
                         device=x.device)
-        g[(Ellipsis,) + self.crop] = sx
-        g = cfftn(g, dim=self.dim, norm="ortho")
+        g[(Ellipsis,) + self.index] = sx
+        g = torch.fft.fftn(g, dim=self.dim, norm="ortho")
         k = interpolate(g, self.locs, self.width, kernel_params=self.kp)
         g = interpolate_adjoint(k, self.locs, self.padded, self.width, kernel_params=self.kp)
-        g = cifftn(g, dim=self.dim, norm="ortho")[(Ellipsis,) + self.crop] * self.apod2
+        g = torch.fft.ifftn(g, dim=self.dim, norm="ortho")[(Ellipsis,) + self.index] * self.apod2
         out = (self.maps.conj() * g).sum(-self.ndim - 1)
         return out.reshape(*lead, *self.im_size)
 

@@ -21,7 +21,7 @@ import torch
 import torch.nn as nn
 
 from ..nameddim import ND, NamedShape as NS, get_nd_shape
-from .basic import FFT, Diagonal, Identity, Pad, Scalar
+from .basic import FFT, Diagonal, Identity, Pad, Scalar, crop_slice_from_pad, pad_to_size
 from .chain import Chain
 from .interp import Interpolate
 
@@ -39,6 +39,32 @@ def cifftn(x, dim=None, norm="ortho"):
     x = torch.fft.ifftshift(x, dim=dim)
     x = torch.fft.ifftn(x, dim=dim, norm=norm)
     return torch.fft.fftshift(x, dim=dim)
+
+
+def shifted_layout(im_size, padded, device):
+    """Index tensors and location shift that replace the centring shifts of
+    cfftn / cifftn inside a NUFFT normal operator.
+
+    For an axis of n image cells centred on a padded axis of N cells
+    (crop slice [c0, c0 + n)), fftshift(fftn(ifftshift(pad(x)))) equals
+    fftn of the image written at the wrapped cells (i - N // 2) mod N,
+    relabelled by N // 2; under circular padding, interpolating the
+    relabelled grid at t equals interpolating the unshifted one at
+    (t - N // 2) mod N, and gridding likewise.  The adjoint crops from the
+    same wrapped cells.  Returns (index, shift): a tuple of broadcastable
+    index tensors (one per axis) for ``g[..., *index]`` and the per-axis
+    shift N // 2 to subtract from the locations (modulo N)."""
+    ndim = len(im_size)
+    crop = crop_slice_from_pad(pad_to_size(tuple(im_size), tuple(padded)))
+    index = []
+    for d, (sl, N) in enumerate(zip(crop, padded)):
+        start, stop, _ = sl.indices(N)
+        i = (torch.arange(start, stop, device=device) - N // 2) % N
+        shape = [1] * ndim
+        shape[d] = -1
+        index.append(i.view(shape))
+    shift = torch.tensor([N // 2 for N in padded], dtype=torch.float32, device=device)
+    return tuple(index), shift
 
 
 class NUFFT(Chain):
