@@ -796,7 +796,9 @@ __global__ __launch_bounds__(GW * NW) void grid_binned_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Gridding with matrix-core accumulation (3-D, 8x8x8 tiles).
+// Gridding with matrix-core accumulation (3-D 8x8x8 and 2-D 16x32 tiles;
+// 512 cells either way).  Described here for 3-D; the 2-D layout follows
+// (MfShape).
 //
 // For one tile, out[cell][plane] = sum_p A[cell][p] V[p][plane] with
 // A[cell][p] = wx_p(x) wy_p(y) wz_p(z), the point's separable weight at the

@@ -472,7 +472,7 @@ __global__ __launch_bounds__(256) void sub_bank_order_kernel(const int4* __restr
   }
 }
 
-// Matrix-core gridding (3-D, 8x8x8 tiles) accumulates blocks of 16 cells
+// Matrix-core gridding (3-D 8x8x8 / 2-D 16x32 tiles) accumulates blocks of 16 cells
 // for groups of 4 consecutive touches of a work item and skips the blocks
 // no touch of the group reaches.  Ordering each item's touches by their
 // block mask puts touches with equal masks side by side, so a group's
