@@ -87,6 +87,56 @@ def cpu_baseline(vals_c, locs, y, beta, target_s=6.0):
     }
 
 
+def sampling_leg(dev, locs, C: int, n: int, steps: int = 10, oversamp: float = 1.25):
+    """Sampling-mode NUFFT's two kernels at the headline size: the radial
+    points rounded to cells of the 1.25x padded grid (320^3 for 256^3), 8
+    coils, complex64.  gather = index (padded grid -> points), adjoint =
+    index_adjoint (points -> full padded grid).  Algorithmic bytes:
+    gather C*M*8 read + M*8 index + C*M*8 written; adjoint C*M*8 read +
+    M*8 order + (G+1)*8 starts + C*G*8 written."""
+    import numpy as np
+    import torch
+
+    from torchlinops_amd.functional import _index as FI
+
+    npad = int(n * oversamp)
+    cell = torch.remainder(torch.round((locs - n / 2) * oversamp) + npad // 2, npad).long()
+    imap = FI.index_map(tuple(cell[:, d] for d in range(3)), (npad,) * 3, dev)
+    M, G = imap.npts, imap.ngrid
+    gen = torch.Generator(device=dev).manual_seed(8)
+    grid = torch.randn(C, npad, npad, npad, dtype=torch.complex64, device=dev, generator=gen)
+    pts = torch.randn(C, M, dtype=torch.complex64, device=dev, generator=gen)
+    for _ in range(2):
+        FI.index_with_map(grid, imap)
+        FI.index_adjoint_with_map(pts, imap)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    torch.cuda.synchronize()
+    for e in ev:
+        e[0].record()
+        FI.index_with_map(grid, imap)
+        e[1].record()
+        FI.index_adjoint_with_map(pts, imap)
+        e[2].record()
+    torch.cuda.synchronize()
+    t_g = float(np.mean([e[0].elapsed_time(e[1]) for e in ev])) * 1e-3
+    t_a = float(np.mean([e[1].elapsed_time(e[2]) for e in ev])) * 1e-3
+    b_g = C * M * 16 + M * 8
+    b_a = C * M * 8 + M * 8 + (G + 1) * 8 + C * G * 8
+    distinct = int(torch.unique(imap.flat).numel())
+    return {
+        "workload": f"sampling NUFFT gather/scatter-add, {C} coils, {npad}^3 padded grid, "
+                    f"{M} radial points rounded to cells ({distinct} distinct)",
+        "gather_us": t_g * 1e6,
+        "adjoint_us": t_a * 1e6,
+        "gather_GBps": b_g / t_g / 1e9,
+        "adjoint_GBps": b_a / t_a / 1e9,
+        "gather_frac": b_g / t_g / HBM_PEAK,
+        "adjoint_frac": b_a / t_a / HBM_PEAK,
+        "gather_bytes": b_g,
+        "adjoint_bytes": b_a,
+    }
+
+
 def c3_batched_cg(dev, iters: int = 20, nprob: int = 64, seed: int = 3):
     """64 independent 2-D NUFFT problems of uneven size (grids 128..512^2,
     1e4..5e5 points on variable-density spirals, 1..32 coils, widths 4..6),
@@ -303,6 +353,8 @@ def main() -> int:
                     help="skip the batched uneven-problem CG leg")
     ap.add_argument("--no-c5", action="store_true",
                     help="skip the learned-preconditioner CG leg")
+    ap.add_argument("--no-sampling", action="store_true",
+                    help="skip the sampling-mode gather / scatter-add leg")
     ap.add_argument("--small", action="store_true", help="reduced size (debug only)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one elementwise copy of the 8 x 256^3 complex64 grid "
@@ -442,6 +494,11 @@ def main() -> int:
             vals.cpu().numpy(), locs.cpu().numpy(), ksp.cpu().numpy(), beta
         )
         log(f"cpu baseline: {out['cpu_baseline']['sample']}")
+    if not args.no_sampling and not args.small:
+        log("sampling: gather / scatter-add ...")
+        out["sampling"] = sampling_leg(dev, locs, C, n)
+        log(f"sampling: gather {out['sampling']['gather_us']:.1f} us, "
+            f"adjoint {out['sampling']['adjoint_us']:.1f} us")
     if not args.no_c3 and not args.small:
         del vals, ksp
         _plan.clear_plan_cache()

@@ -328,6 +328,40 @@ int tnl_fold(const void* x, void* y, int dtype, int64_t nbatch, int ndim,
              const int64_t* nblocks, const int64_t* block_size,
              const int64_t* im_size, const int64_t* stride, void* stream);
 
+/* ---- index (integer sampling gather) ------------------------------------
+ * out[n, p] = x[n, idx[p]] over a flattened grid x [nbatch, ngrid] and
+ * raveled cell indices idx [npts] (int64, in [0, ngrid)); pure copy,
+ * bit-exact.  Replaces torch.index_select(x_flat, -1, idx_flat) in
+ * multi_index (src/torchlinops/functional/_index/index.py:52-72), which
+ * index (index.py:17-49) and Sampling.fn (src/torchlinops/linops/
+ * sampling.py:96-98) call.  elem_bytes in {1, 2, 4, 8, 16} (complex
+ * values move as one element).  An index outside [0, ngrid) reads as
+ * zero; callers check ranges before the call.  out is fully written.
+ */
+int tnl_index(const void* x, const int64_t* idx, void* out, int elem_bytes,
+              int64_t nbatch, int64_t ngrid, int64_t npts, void* stream);
+
+/* ---- index_adjoint (integer sampling scatter-add) -----------------------
+ * out[n, c] = sum over points p with idx[p] == c of x[n, p]; cells no
+ * point hits are 0.  Replaces y.index_add_(-1, idx_flat, x_flat) in
+ * multi_grid (src/torchlinops/functional/_index/index.py:140-180), which
+ * index_adjoint (index.py:75-99) and Sampling.adj_fn (sampling.py:100-101)
+ * call.  The points are handed over grouped by cell: order [npts] lists
+ * the point numbers sorted by cell (stable, so ascending within a cell)
+ * and starts [ngrid + 1] is the offset of each cell's run in order.
+ * A cell with at most heavy_min points (every cell when nheavy == 0) is
+ * summed in ascending p from zero: bit-exact against the CPU's sequential
+ * `y[idx[p]] += x[p]`.  Cells with more points must all be listed in
+ * heavy [nheavy]; each is summed by one workgroup in a fixed order
+ * (deterministic, within rounding of the sequential sum).  dtype TNL_F32
+ * or TNL_F64, is_complex for interleaved (re, im) pairs.  out is fully
+ * written (no zeroing needed, no atomics).
+ */
+int tnl_index_adjoint(const void* x, const int64_t* order, const int64_t* starts,
+                      const int64_t* heavy, int64_t nheavy, int64_t heavy_min,
+                      void* out, int dtype, int is_complex, int64_t nbatch,
+                      int64_t ngrid, int64_t npts, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,12 @@ restatements of the NUFFT chain and of CG:
                           241-334 (prep_locs), 336-346 (beta),
                           348-365 (apodize_weights); pad_last.py:139-199;
                           src/torchlinops/utils/_fft.py:7-60 (centred FFT)
+  index / index_adjoint   src/torchlinops/functional/_index/index.py:17-99
+                          (gather; sequential scatter-add of index_add_,
+                          140-198 multi_grid / ravel)
+  nufft_sampling(_adjoint) src/torchlinops/linops/nufft.py:185-203
+                          (Pad -> centred FFT -> integer sampling),
+                          prep_locs sampling branch 326-333
   conjugate_gradients     src/torchlinops/alg/pcg.py:13-88
   dft / dft_adjoint       brute-force nonuniform DFT (the tutorial's exact
                           operator, tutorials/mri_reconstruction.py:224-258)
@@ -316,13 +322,74 @@ def cfftn(x, ndim, inverse=False):
     return np.fft.fftshift(x, axes=axes)
 
 
-def prep_locs(locs, grid_size, padded_size):
-    """Scale centred coords to the padded grid and wrap (nufft.py:315-334)."""
+def prep_locs(locs, grid_size, padded_size, nufft_mode="interpolate"):
+    """Scale centred coords to the padded grid and wrap (nufft.py:315-334);
+    the sampling branch rounds half to even before the wrap and returns
+    int64 (326-333)."""
     out = np.array(locs, dtype=np.float64, copy=True)
     for i in range(-len(grid_size), 0):
         out[..., i] = out[..., i] * (padded_size[i] / grid_size[i]) + padded_size[i] // 2
+        if nufft_mode == "sampling":
+            out[..., i] = np.round(out[..., i])
         out[..., i] = np.mod(out[..., i], padded_size[i])
+    if nufft_mode == "sampling":
+        out = out.astype(np.int64)
     return out
+
+
+def ravel(idx, grid_size):
+    """Row-major cell of each index vector, components taken mod the axis
+    size (index.py:183-198).  idx: tuple of broadcastable int arrays."""
+    idx = np.broadcast_arrays(*[np.asarray(i, dtype=np.int64) for i in idx])
+    flat = np.zeros(idx[0].shape, dtype=np.int64)
+    for i, n in zip(idx, grid_size):
+        flat = flat * int(n) + np.mod(i, int(n))
+    return flat
+
+
+def index(x, idx):
+    """x[..., D...] gathered at the index tuple over the last D axes
+    (multi_index, index.py:52-72): [N..., D...] -> [N..., I...]."""
+    x = np.asarray(x)
+    nd = len(idx)
+    grid_size = x.shape[x.ndim - nd:]
+    flat = ravel(idx, grid_size)
+    xf = x.reshape(x.shape[: x.ndim - nd] + (-1,))
+    return xf[..., flat.reshape(-1)].reshape(x.shape[: x.ndim - nd] + flat.shape)
+
+
+def index_adjoint(y, idx, grid_size):
+    """Sequential scatter-add: out[..., cell(p)] += y[..., p] for p in
+    ascending order into zeros (multi_grid's index_add_, index.py:140-180).
+    Accumulates in y's dtype, one point at a time."""
+    y = np.asarray(y)
+    flat = ravel(idx, grid_size)
+    nl = flat.ndim
+    batch = y.shape[: y.ndim - nl]
+    yf = y.reshape(batch + (-1,))
+    out = np.zeros(batch + (prod(grid_size),), dtype=y.dtype)
+    for p, c in enumerate(flat.reshape(-1)):
+        out[..., c] += yf[..., p]
+    return out.reshape(batch + tuple(grid_size))
+
+
+def nufft_sampling(x, locs, oversamp=1.25, prep=True):
+    """Sampling-mode NUFFT in float64: Pad -> centred FFT -> read at the
+    integer padded-grid cells of prep_locs (nufft.py:185-203)."""
+    ndim = locs.shape[-1]
+    grid_size = x.shape[x.ndim - ndim:]
+    padded = tuple(int(n * oversamp) for n in grid_size)
+    y = cfftn(center_pad(x, grid_size, padded), ndim)
+    lp = prep_locs(locs, grid_size, padded, "sampling") if prep else np.asarray(locs)
+    return index(y, tuple(lp[..., d] for d in range(ndim)))
+
+
+def nufft_sampling_adjoint(y, locs, grid_size, oversamp=1.25, prep=True):
+    ndim = locs.shape[-1]
+    padded = tuple(int(n * oversamp) for n in grid_size)
+    lp = prep_locs(locs, grid_size, padded, "sampling") if prep else np.asarray(locs)
+    g = index_adjoint(y, tuple(lp[..., d] for d in range(ndim)), padded)
+    return center_crop(cfftn(g, ndim, inverse=True), padded, grid_size)
 
 
 def nufft(x, locs, oversamp=1.25, width=4.0):

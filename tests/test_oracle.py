@@ -432,3 +432,84 @@ def test_conv3d_oracle_backward_matches_fd():
     bp = b.copy()
     bp[1] += h
     assert abs((L(x, w, bp) - L(x, w, b)) / h - db[1]) < 1e-5
+
+
+# ------------------------------------------------------------ sampling
+def test_index_mask_known_answer():
+    """The reference's from_mask case (tests/test_sampling.py:57-68): the
+    True cells (2, 3) and (5, 6) of an 8x8 mask read those two values."""
+    x = np.random.default_rng(0).standard_normal((8, 8))
+    mask = np.zeros((8, 8), bool)
+    mask[2, 3] = mask[5, 6] = True
+    got = O.index(x, tuple(np.nonzero(mask)))
+    assert got.shape == (2,)
+    assert got[0] == x[2, 3] and got[1] == x[5, 6]
+
+
+def test_index_brute_force_and_wrap():
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((2, 3, 5, 4))
+    i0 = rng.integers(-5, 10, (6, 7))  # negative and >= size wrap per axis
+    i1 = rng.integers(-4, 8, (6, 7))
+    got = O.index(x, (i0, i1))
+    assert got.shape == (2, 3, 6, 7)
+    for a, b in product(range(6), range(7)):
+        assert np.array_equal(got[:, :, a, b], x[:, :, i0[a, b] % 5, i1[a, b] % 4])
+    # a transposed index tuple reads other cells
+    assert not np.array_equal(O.index(x[..., :4, :4], (i1 % 4, i0 % 4)),
+                              O.index(x[..., :4, :4], (i0 % 4, i1 % 4)))
+
+
+def test_index_adjoint_counts_adjointness_and_add_at():
+    rng = np.random.default_rng(2)
+    grid = (6, 5, 4)
+    idx = tuple(rng.integers(0, n, (300,)) for n in grid)  # many repeats
+    flat = O.ravel(idx, grid)
+    # ones scatter to the hit count of each cell
+    counts = O.index_adjoint(np.ones(300), idx, grid)
+    assert np.array_equal(counts.reshape(-1), np.bincount(flat, minlength=prod(grid)))
+    # adjoint of index in float64
+    x = rng.standard_normal((3,) + grid) + 1j * rng.standard_normal((3,) + grid)
+    y = rng.standard_normal((3, 300)) + 1j * rng.standard_normal((3, 300))
+    lhs = np.vdot(y, O.index(x, idx))
+    rhs = np.vdot(O.index_adjoint(y, idx, grid), x)
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+    # float32: the sequential sum equals numpy's unbuffered add.at, bit for bit
+    y32 = rng.standard_normal((2, 300)).astype(np.float32)
+    want = np.zeros((2, prod(grid)), np.float32)
+    np.add.at(want, (slice(None), flat), y32)
+    got = O.index_adjoint(y32, idx, grid)
+    assert got.dtype == np.float32
+    assert np.array_equal(got.reshape(2, -1), want)
+
+
+def test_prep_locs_sampling_doctest():
+    """The reference's prep_locs doctest (linops/nufft.py:295-305): locs on
+    the 1.25x lattice of a 64^3 grid map to integer cells 0..79."""
+    import torch
+
+    torch.manual_seed(0)
+    locs = torch.rand(1000, 3) * 64 - 32
+    locs = (torch.round(locs * 1.25) / 1.25).numpy()
+    lp = O.prep_locs(locs, (64, 64, 64), (80, 80, 80), "sampling")
+    assert lp.dtype == np.int64
+    assert lp.min() == 0 and lp.max() == 79
+
+
+@pytest.mark.parametrize("grid_size,oversamp", [((16, 16), 1.25), ((12, 10), 1.5), ((5, 7, 6), 2.0)])
+def test_nufft_sampling_is_exact_dft_on_the_padded_lattice(grid_size, oversamp):
+    """At k = m N / N_pad the sampled, zero-padded centred FFT is the exact
+    DFT scaled by sqrt(prod N / prod N_pad) (odd and even sizes)."""
+    rng = np.random.default_rng(3)
+    ndim = len(grid_size)
+    padded = tuple(int(n * oversamp) for n in grid_size)
+    m = np.stack([rng.integers(-(p // 2), p - p // 2, 40) for p in padded], -1)
+    locs = m * np.array(grid_size) / np.array(padded)
+    x = rng.standard_normal(grid_size) + 1j * rng.standard_normal(grid_size)
+    want = O.dft(x, locs) * np.sqrt(prod(grid_size) / prod(padded))
+    got = O.nufft_sampling(x, locs, oversamp)
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+    y = rng.standard_normal(40) + 1j * rng.standard_normal(40)
+    lhs = np.vdot(y, got)
+    rhs = np.vdot(O.nufft_sampling_adjoint(y, locs, grid_size, oversamp), x)
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)

@@ -21,6 +21,7 @@ from .linops import (
     Interpolate,
     NamedLinop,
     Pad,
+    Sampling,
     Scalar,
     toeplitz_psf,
 )
@@ -31,7 +32,7 @@ __version__ = "0.1.0"
 __all__ = [
     "config", "functional", "linops", "alg", "nameddim", "library_path",
     "NamedLinop", "Chain", "Add", "Identity", "Diagonal", "Scalar", "Pad", "FFT", "Dense",
-    "Interpolate", "ArrayToBlocks", "BlocksToArray", "NUFFT", "toeplitz_psf",
+    "Interpolate", "ArrayToBlocks", "BlocksToArray", "NUFFT", "Sampling", "toeplitz_psf",
     "conjugate_gradients", "DeviceCG", "Dim", "NamedDimension", "NamedShape",
     "ELLIPSES", "ANY",
 ]

@@ -1,6 +1,17 @@
 """Functional API of the gridding path (mirrors torchlinops.functional)."""
 
 from ._batch import BatchedInterp
+from ._index import (
+    canonicalize_idx,
+    ensure_tensor_indexing,
+    index,
+    index_adjoint,
+    mask2idx,
+    multi_grid,
+    multi_index,
+    ravel,
+    slice2range,
+)
 from ._interp import grid, interp_grad_locs, interpolate, interpolate_adjoint, ungrid
 from ._nufft import center_crop, center_pad, nufft, nufft_adjoint
 from ._plan import clear_plan_cache
@@ -31,4 +42,13 @@ __all__ = [
     "nufft_adjoint",
     "center_pad",
     "center_crop",
+    "index",
+    "index_adjoint",
+    "multi_index",
+    "multi_grid",
+    "ravel",
+    "mask2idx",
+    "canonicalize_idx",
+    "slice2range",
+    "ensure_tensor_indexing",
 ]

@@ -1,9 +1,11 @@
-"""NUFFT linop (type II, "interpolate" mode) and its Toeplitz normal.
+"""NUFFT linop (type II) and its Toeplitz normal.
 
 Follows src/torchlinops/linops/nufft.py:
-  * chain Apodize -> Pad(int(N*os)) -> FFT(centred, ortho) ->
-    Interpolate(KB, beta) -> Scalar(1 / (w^D sqrt(prod N / prod N_pad)))
-    (nufft.py:123-184);
+  * "interpolate" mode: chain Apodize -> Pad(int(N*os)) -> FFT(centred,
+    ortho) -> Interpolate(KB, beta) -> Scalar(1 / (w^D sqrt(prod N /
+    prod N_pad))) (nufft.py:123-184);
+  * "sampling" mode: Pad -> FFT -> Sampling at integer padded-grid
+    locations, no apodization or scaling (185-203);
   * prep_locs (241-334), beta (336-346), apodize_weights (348-365);
   * adjoint (217-224); normal with ``toeplitz=True`` (226-239) builds
     Crop . F^H . (K *) . F . Pad with K the Fourier transform of the point
@@ -24,6 +26,7 @@ from ..nameddim import ND, NamedShape as NS, get_nd_shape
 from .basic import FFT, Diagonal, Identity, Pad, Scalar, crop_slice_from_pad, pad_to_size
 from .chain import Chain
 from .interp import Interpolate
+from .sampling import Sampling
 
 __all__ = ["NUFFT", "toeplitz_psf", "cfftn", "cifftn"]
 
@@ -72,10 +75,8 @@ class NUFFT(Chain):
                  batch_shape=None, oversamp: float = 1.25, width: float = 4.0,
                  mode: str = "interpolate", do_prep_locs: bool = True,
                  apodize_weights: Optional[torch.Tensor] = None, **options):
-        if mode != "interpolate":
-            raise NotImplementedError(
-                f"NUFFT mode {mode!r}: only 'interpolate' is implemented on this path"
-            )
+        if mode not in ("interpolate", "sampling"):
+            raise ValueError(f"Unrecognized NUFFT mode: {mode}")
         device = locs.device
         self.mode = mode
         self.options = options
@@ -92,7 +93,20 @@ class NUFFT(Chain):
         fft = FFT(ndim=locs.shape[-1], centered=True, norm="ortho", batch_shape=self.batch_shape,
                   grid_shapes=(pad.out_im_shape, self.input_kshape))
         grid_shape = fft._shape.output_grid_shape
-        locs_p = self.prep_locs(locs, grid_size, padded_size) if do_prep_locs else locs
+        locs_p = (self.prep_locs(locs, grid_size, padded_size, nufft_mode=mode)
+                  if do_prep_locs else locs)
+        if mode == "sampling":
+            if locs_p.is_complex() or locs_p.is_floating_point():
+                raise ValueError(
+                    f"Sampling linop requries integer-type locs but got {locs_p.dtype}"
+                )
+            interp = Sampling.from_stacked_idx(locs_p, dim=-1, input_size=padded_size,
+                                               output_shape=self.output_shape,
+                                               input_shape=grid_shape,
+                                               batch_shape=self.batch_shape)
+            super().__init__(pad, fft, interp, name="NUFFT")
+            self._finish(locs, grid_size, oversamp, width, pad, fft, interp)
+            return
         beta = self.beta(width, oversamp)
         weight = (self.apodize_weights(grid_size, padded_size, oversamp, width, beta).to(device)
                   if apodize_weights is None else apodize_weights)
@@ -108,6 +122,9 @@ class NUFFT(Chain):
         scale_factor = width**ndim * (prod(grid_size) / prod(padded_size)) ** 0.5
         scale = Scalar(weight=1.0 / scale_factor, ioshape=interp.oshape)
         super().__init__(apodize, pad, fft, interp, scale, name="NUFFT")
+        self._finish(locs, grid_size, oversamp, width, pad, fft, interp)
+
+    def _finish(self, locs, grid_size, oversamp, width, pad, fft, interp):
         self.locs = locs
         self.grid_size = tuple(grid_size)
         self.oversamp = oversamp
@@ -134,7 +151,9 @@ class NUFFT(Chain):
     @staticmethod
     def prep_locs(locs, grid_size, padded_size, pad_mode="circular", nufft_mode="interpolate"):
         """Centred coordinates in [-N/2, N/2] -> padded-grid units in
-        [0, N_pad) (scale by N_pad/N, shift by N_pad//2, wrap or clamp)."""
+        [0, N_pad) (scale by N_pad/N, shift by N_pad//2, wrap or clamp).
+        ``nufft_mode="sampling"`` rounds to the nearest cell before the
+        wrap and returns int64."""
         out = locs.clone()
         for i in range(-len(grid_size), 0):
             out[..., i] *= padded_size[i] / grid_size[i]
@@ -142,9 +161,14 @@ class NUFFT(Chain):
             if pad_mode == "zero":
                 out[..., i] = torch.clamp(out[..., i], 0, padded_size[i] - 1)
             elif pad_mode == "circular":
-                out[..., i] = torch.remainder(out[..., i], torch.tensor(padded_size[i]))
+                if nufft_mode == "sampling":
+                    out[..., i] = torch.remainder(torch.round(out[..., i]), padded_size[i])
+                else:
+                    out[..., i] = torch.remainder(out[..., i], torch.tensor(padded_size[i]))
             else:
                 raise ValueError(f"Unrecognized padding mode during prep: {pad_mode}")
+        if nufft_mode == "sampling":
+            out = out.to(torch.int64)
         return out
 
     @staticmethod
@@ -167,6 +191,8 @@ class NUFFT(Chain):
 
     @property
     def device(self):
+        if isinstance(self.interp, Sampling):
+            return self.interp.idx[0].device
         return self.interp.locs.device
 
 
@@ -188,6 +214,10 @@ def toeplitz_psf(nufft: NUFFT, inner=None, dtype=None, oversamp: float = 2.0):
     do not couple batch entries (all batch dims broadcast); the kernel is
     then shared by every batch entry and applied as a Diagonal.
     """
+    if isinstance(nufft.interp, Sampling):
+        raise NotImplementedError(
+            "Toeplitz embedding not yet implemented for Sampling-type NUFFT"
+        )
     dtype = dtype or torch.complex64
     device = nufft.device
     grid_size = nufft.pad.im_size

@@ -322,6 +322,17 @@ class NamedShape:
         else:
             self._set(name, value)
 
+    def __getitem__(self, name):
+        if name not in self._idx:
+            raise KeyError(f"{name} not in shape {list(self._idx)}")
+        return self._get(name)
+
+    def __setitem__(self, name, value):
+        self._set(name, value)
+
+    def __contains__(self, name) -> bool:
+        return name in self._idx
+
     @property
     def shapes(self) -> dict:
         return {k: self._get(k) for k in self._idx}
