@@ -137,6 +137,42 @@ def sampling_leg(dev, locs, C: int, n: int, steps: int = 10, oversamp: float = 1
     }
 
 
+def toeplitz_leg(dev, steps: int = 10, n: int = 128, C: int = 8):
+    """One Toeplitz normal apply A^H A x of a 3-D radial NUFFT (128^3 image,
+    8 coils, 8000x256 points, oversamp 1.25, Toeplitz grid 2x = 256^3):
+    the fused operator (pad_wrapped -> rocFFT -> K multiply -> rocFFT ->
+    crop_wrapped) against the reference-shaped chain (Pad, centred FFT,
+    Diagonal, centred FFT^H, Crop)."""
+    import numpy as np
+    import torch
+
+    import torchlinops_amd as tl
+
+    gen = torch.Generator().manual_seed(4)
+    locs = (radial_traj(8000, 256, n, gen, "cpu") - n / 2).reshape(-1, 3).to(dev)
+    x = torch.randn(C, n, n, n, dtype=torch.complex64, generator=gen).to(dev)
+    out = {}
+    for label, fused in (("fused", True), ("chain", False)):
+        A = tl.NUFFT(locs, (n, n, n), output_shape=("K",), batch_shape=("C",),
+                     oversamp=1.25, toeplitz=True, toeplitz_fused=fused)
+        N = A.N
+        for _ in range(2):
+            N(x)
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(steps)]
+        torch.cuda.synchronize()
+        for e in ev:
+            e[0].record()
+            N(x)
+            e[1].record()
+        torch.cuda.synchronize()
+        out[f"{label}_ms"] = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+        del A, N
+    out["speedup"] = out["chain_ms"] / out["fused_ms"]
+    out["workload"] = (f"Toeplitz normal apply, {C} coils, {n}^3 image, 8000x256 radial, "
+                       f"{2 * n}^3 Toeplitz grid, complex64")
+    return out
+
+
 def c3_batched_cg(dev, iters: int = 20, nprob: int = 64, seed: int = 3):
     """64 independent 2-D NUFFT problems of uneven size (grids 128..512^2,
     1e4..5e5 points on variable-density spirals, 1..32 coils, widths 4..6),
@@ -355,6 +391,8 @@ def main() -> int:
                     help="skip the learned-preconditioner CG leg")
     ap.add_argument("--no-sampling", action="store_true",
                     help="skip the sampling-mode gather / scatter-add leg")
+    ap.add_argument("--no-toeplitz", action="store_true",
+                    help="skip the Toeplitz normal leg")
     ap.add_argument("--small", action="store_true", help="reduced size (debug only)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one elementwise copy of the 8 x 256^3 complex64 grid "
@@ -499,6 +537,11 @@ def main() -> int:
         out["sampling"] = sampling_leg(dev, locs, C, n)
         log(f"sampling: gather {out['sampling']['gather_us']:.1f} us, "
             f"adjoint {out['sampling']['adjoint_us']:.1f} us")
+    if not args.no_toeplitz and not args.small:
+        log("toeplitz: normal apply, fused vs chain ...")
+        out["toeplitz"] = toeplitz_leg(dev)
+        log(f"toeplitz: fused {out['toeplitz']['fused_ms']:.2f} ms, "
+            f"chain {out['toeplitz']['chain_ms']:.2f} ms")
     if not args.no_c3 and not args.small:
         del vals, ksp
         _plan.clear_plan_cache()

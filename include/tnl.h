@@ -362,6 +362,27 @@ int tnl_index_adjoint(const void* x, const int64_t* order, const int64_t* starts
                       void* out, int dtype, int is_complex, int64_t nbatch,
                       int64_t ngrid, int64_t npts, void* stream);
 
+/* ---- Toeplitz normal: centred pad + ifftshift, fftshift + centred crop ---
+ * Image index i_d in [0, N_d) has centred coordinate c_d = i_d - N_d/2.
+ * tnl_pad_wrapped:  y[n, q] = x[n, i] where q_d = c_d mod P_d on every
+ *                   axis, 0 at every other cell of y [nbatch, *pad_size];
+ *                   equals ifftshift(Pad(x)) of the reference's centred pad
+ *                   (src/torchlinops/linops/pad_last.py:139-199) and FFT's
+ *                   ifftshift (src/torchlinops/utils/_fft.py:7-31).
+ * tnl_crop_wrapped: x[n, i] = y[n, (c mod P)]; equals Crop(fftshift(y)).
+ * Together they replace the four centring rolls and the pad/crop copies
+ * of the Toeplitz normal Crop . F^H . (K *) . F . Pad built at
+ * src/torchlinops/linops/nufft.py:226-239, the kernel K then being taken
+ * in natural FFT order.  Pure copies, bit-exact; elem_bytes in {4, 8, 16};
+ * 1 <= im_size <= pad_size; outputs fully written.
+ */
+int tnl_pad_wrapped(const void* x, void* y, int elem_bytes, int64_t nbatch,
+                    int ndim, const int64_t* im_size, const int64_t* pad_size,
+                    void* stream);
+int tnl_crop_wrapped(const void* y, void* x, int elem_bytes, int64_t nbatch,
+                     int ndim, const int64_t* im_size, const int64_t* pad_size,
+                     void* stream);
+
 #ifdef __cplusplus
 }
 #endif

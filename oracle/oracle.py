@@ -18,6 +18,8 @@ restatements of the NUFFT chain and of CG:
   nufft_sampling(_adjoint) src/torchlinops/linops/nufft.py:185-203
                           (Pad -> centred FFT -> integer sampling),
                           prep_locs sampling branch 326-333
+  pad_wrapped / crop_wrapped  centred pad + ifftshift, fftshift + crop
+                          (pad_last.py:139-199, utils/_fft.py:7-31)
   conjugate_gradients     src/torchlinops/alg/pcg.py:13-88
   dft / dft_adjoint       brute-force nonuniform DFT (the tutorial's exact
                           operator, tutorials/mri_reconstruction.py:224-258)
@@ -320,6 +322,19 @@ def cfftn(x, ndim, inverse=False):
     x = np.fft.ifftshift(x, axes=axes)
     x = (np.fft.ifftn if inverse else np.fft.fftn)(x, axes=axes, norm="ortho")
     return np.fft.fftshift(x, axes=axes)
+
+
+def pad_wrapped(x, im_size, pad_size):
+    """ifftshift of the centred zero-pad (pad_last.py:139-199 then
+    utils/_fft.py:7-31's ifftshift) over the last len(im_size) axes."""
+    ndim = len(im_size)
+    return np.fft.ifftshift(center_pad(x, im_size, pad_size), axes=tuple(range(-ndim, 0)))
+
+
+def crop_wrapped(y, im_size, pad_size):
+    """Centred crop of fftshift(y): the adjoint of pad_wrapped."""
+    ndim = len(im_size)
+    return center_crop(np.fft.fftshift(y, axes=tuple(range(-ndim, 0))), pad_size, im_size)
 
 
 def prep_locs(locs, grid_size, padded_size, nufft_mode="interpolate"):

@@ -119,8 +119,40 @@ def test_toeplitz_normal_matches_explicit_normal():
     err = rel_l2(toep.cpu().numpy(), explicit.cpu().numpy())
     print("toeplitz vs explicit normal rel-L2:", err)
     assert err < 2e-3
-    # Toeplitz chain: Pad, FFT, K, FFT^H, Crop
-    assert [type(l).__name__ for l in At.N.linops] == ["Pad", "FFT", "Diagonal", "FFT", "Pad"]
+    assert type(At.N).__name__ == "ToeplitzNormal"
+    # the reference's chain Pad, FFT, K, FFT^H, Crop gives the same operator
+    Ac = tl.NUFFT(locs, grid, output_shape=("K",), batch_shape=("C",), oversamp=2.0,
+                  toeplitz=True, toeplitz_fused=False)
+    assert [type(l).__name__ for l in Ac.N.linops] == ["Pad", "FFT", "Diagonal", "FFT", "Pad"]
+    chain = Ac.N(x)
+    assert rel_l2(toep.cpu().numpy(), chain.cpu().numpy()) < 1e-5
+    y = torch.randn_like(x)
+    lhs = torch.vdot(y.flatten(), toep.flatten())
+    rhs = torch.vdot(At.N.H(y).flatten(), x.flatten())
+    assert abs(lhs - rhs) < 1e-4 * abs(lhs)
+
+
+@pytest.mark.parametrize("im,pad,dtype", [
+    ((24, 24, 24), (48, 48, 48), torch.complex64),
+    ((5, 6, 7), (10, 12, 14), torch.complex128),
+    ((33, 20), (66, 40), torch.float32),
+    ((9,), (19,), torch.complex64),
+    ((16, 16), (16, 16), torch.complex64),
+])
+def test_wrapped_pad_crop_bit_exact(dev, im, pad, dtype):
+    from torchlinops_amd.linops.toeplitz import crop_wrapped, pad_wrapped
+
+    print(f"wrapped pad/crop im={im} pad={pad} {dtype}", flush=True)
+    g = torch.Generator().manual_seed(len(im) + pad[0])
+    x = torch.randn((3,) + im, generator=g).to(dtype) if not dtype.is_complex else \
+        torch.randn((3,) + im, dtype=dtype, generator=g)
+    y = pad_wrapped(x.to(dev), im, pad)
+    assert np.array_equal(y.cpu().numpy(), O.pad_wrapped(x.numpy(), im, pad))
+    z = torch.randn((3,) + pad, generator=g).to(dtype) if not dtype.is_complex else \
+        torch.randn((3,) + pad, dtype=dtype, generator=g)
+    c = crop_wrapped(z.to(dev), im, pad)
+    assert np.array_equal(c.cpu().numpy(), O.crop_wrapped(z.numpy(), im, pad))
+    assert torch.equal(crop_wrapped(y, im, pad).cpu(), x)
 
 
 def test_cg_on_nufft_normal():

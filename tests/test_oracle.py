@@ -513,3 +513,40 @@ def test_nufft_sampling_is_exact_dft_on_the_padded_lattice(grid_size, oversamp):
     lhs = np.vdot(y, got)
     rhs = np.vdot(O.nufft_sampling_adjoint(y, locs, grid_size, oversamp), x)
     assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+# ------------------------------------------------------ Toeplitz wraps
+@pytest.mark.parametrize("im,pad", [((4,), (9,)), ((5,), (10,)), ((5, 6), (10, 12)),
+                                    ((3, 4, 5), (6, 9, 10)), ((7,), (7,))])
+def test_wrapped_pad_crop_brute_force(im, pad):
+    """pad_wrapped puts image index i at (i - N//2) mod P, zeros elsewhere;
+    crop_wrapped reads it back (odd and even sizes)."""
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((2,) + im) + 1j * rng.standard_normal((2,) + im)
+    y = O.pad_wrapped(x, im, pad)
+    want = np.zeros((2,) + pad, complex)
+    for i in product(*(range(n) for n in im)):
+        q = tuple((a - n // 2) % p for a, n, p in zip(i, im, pad))
+        want[(slice(None),) + q] = x[(slice(None),) + i]
+    assert np.array_equal(y, want)
+    assert np.array_equal(O.crop_wrapped(y, im, pad), x)
+    # crop_wrapped is the adjoint of pad_wrapped
+    z = rng.standard_normal((2,) + pad) + 1j * rng.standard_normal((2,) + pad)
+    assert abs(np.vdot(z, y) - np.vdot(O.crop_wrapped(z, im, pad), x)) < 1e-12 * abs(np.vdot(z, y))
+
+
+@pytest.mark.parametrize("im,pad", [((6, 5), (12, 10)), ((5, 4, 3), (10, 8, 6)), ((9,), (18,))])
+def test_shift_free_toeplitz_equals_centred_chain(im, pad):
+    """crop_wrapped(ifftn(ifftshift(K) * fftn(pad_wrapped(x)))) equals the
+    reference's Crop . cifftn . (K *) . cfftn . Pad."""
+    rng = np.random.default_rng(6)
+    nd = len(im)
+    axes = tuple(range(-nd, 0))
+    x = rng.standard_normal((3,) + im) + 1j * rng.standard_normal((3,) + im)
+    K = rng.standard_normal(pad) + 1j * rng.standard_normal(pad)
+    chain = O.center_crop(O.cfftn(K * O.cfftn(O.center_pad(x, im, pad), nd), nd, inverse=True),
+                          pad, im)
+    y = np.fft.fftn(O.pad_wrapped(x, im, pad), axes=axes, norm="ortho")
+    y = np.fft.ifftn(np.fft.ifftshift(K, axes=axes) * y, axes=axes, norm="ortho")
+    fused = O.crop_wrapped(y, im, pad)
+    assert np.abs(fused - chain).max() < 1e-12 * np.abs(chain).max()

@@ -1,0 +1,152 @@
+// Centred zero-pad fused with ifftshift, and fftshift fused with the
+// centred crop, for gfx950: the two data-movement passes around the FFTs
+// of a Toeplitz normal operator.
+//
+// Image index i_d in [0, N_d) sits at centred coordinate c_d = i_d - N_d/2
+// (integer division).  The centred pad puts it at c_d + P_d/2 of the
+// padded grid and ifftshift moves that to q_d = c_d mod P_d, so
+//   pad_wrapped:  y[n, q] = x[n, i] if every axis has an image index whose
+//                 c_d == q_d (mod P_d), else 0;
+//   crop_wrapped: x[n, i] = y[n, (i - N/2) mod P]  (fftshift then crop).
+// Both are HBM-bound copies.  A workgroup walks whole rows of the last
+// axis: the row's outer coordinates are decoded once, lanes move
+// consecutive elements, so reads and writes are coalesced except at the
+// single wrap point of a row.
+#include "tnl_common.hpp"
+#include "tnl_internal.hpp"
+
+namespace tnl {
+
+namespace {
+
+template <int EB>
+struct WElem;
+template <>
+struct WElem<4> { using T = uint32_t; };
+template <>
+struct WElem<8> { using T = uint2; };
+template <>
+struct WElem<16> { using T = uint4; };
+
+struct WrapGeom {
+  int64_t N[3], P[3];  // left-padded with 1
+  int64_t nbatch;
+};
+
+// image index of padded-grid coordinate q on an axis, or -1
+__device__ __forceinline__ int64_t wrap_source(int64_t q, int64_t N, int64_t P) {
+  int64_t c = q;
+  if (c + N / 2 >= N) c -= P;
+  const int64_t i = c + N / 2;
+  return (i >= 0 && i < N) ? i : -1;
+}
+
+template <typename E>
+__global__ __launch_bounds__(256) void pad_wrapped_kernel(const E* __restrict__ x,
+                                                          E* __restrict__ y, WrapGeom g) {
+  const int64_t rows = g.nbatch * g.P[0] * g.P[1];
+  for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+    const int64_t q1 = r % g.P[1];
+    const int64_t t = r / g.P[1];
+    const int64_t q0 = t % g.P[0];
+    const int64_t n = t / g.P[0];
+    const int64_t i0 = wrap_source(q0, g.N[0], g.P[0]);
+    const int64_t i1 = wrap_source(q1, g.N[1], g.P[1]);
+    E* dst = y + r * g.P[2];
+    const bool row_ok = i0 >= 0 && i1 >= 0;
+    const E* src = x + ((n * g.N[0] + (row_ok ? i0 : 0)) * g.N[1] + (row_ok ? i1 : 0)) * g.N[2];
+    for (int64_t q2 = threadIdx.x; q2 < g.P[2]; q2 += blockDim.x) {
+      E v{};
+      if (row_ok) {
+        const int64_t i2 = wrap_source(q2, g.N[2], g.P[2]);
+        if (i2 >= 0) v = src[i2];
+      }
+      dst[q2] = v;
+    }
+  }
+}
+
+template <typename E>
+__global__ __launch_bounds__(256) void crop_wrapped_kernel(const E* __restrict__ y,
+                                                           E* __restrict__ x, WrapGeom g) {
+  const int64_t rows = g.nbatch * g.N[0] * g.N[1];
+  for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+    const int64_t i1 = r % g.N[1];
+    const int64_t t = r / g.N[1];
+    const int64_t i0 = t % g.N[0];
+    const int64_t n = t / g.N[0];
+    const int64_t q0 = ((i0 - g.N[0] / 2) % g.P[0] + g.P[0]) % g.P[0];
+    const int64_t q1 = ((i1 - g.N[1] / 2) % g.P[1] + g.P[1]) % g.P[1];
+    const E* src = y + ((n * g.P[0] + q0) * g.P[1] + q1) * g.P[2];
+    E* dst = x + r * g.N[2];
+    for (int64_t i2 = threadIdx.x; i2 < g.N[2]; i2 += blockDim.x) {
+      int64_t q2 = i2 - g.N[2] / 2;
+      if (q2 < 0) q2 += g.P[2];
+      dst[i2] = src[q2];
+    }
+  }
+}
+
+unsigned row_blocks(int64_t rows) {
+  const int64_t cap = 256 * 64;
+  return (unsigned)(rows < 1 ? 1 : (rows > cap ? cap : rows));
+}
+
+template <int EB>
+int launch_wrap(bool pad, const void* in, void* out, const WrapGeom& g, hipStream_t s) {
+  using E = typename WElem<EB>::T;
+  if (pad) {
+    pad_wrapped_kernel<E><<<row_blocks(g.nbatch * g.P[0] * g.P[1]), 256, 0, s>>>(
+        (const E*)in, (E*)out, g);
+    TNL_LAUNCH_CHECK("pad_wrapped");
+  } else {
+    crop_wrapped_kernel<E><<<row_blocks(g.nbatch * g.N[0] * g.N[1]), 256, 0, s>>>(
+        (const E*)in, (E*)out, g);
+    TNL_LAUNCH_CHECK("crop_wrapped");
+  }
+  return TNL_OK;
+}
+
+int wrap_entry(bool pad, const void* in, void* out, int elem_bytes, int64_t nbatch,
+               int ndim, const int64_t* im_size, const int64_t* pad_size, void* stream) {
+  if (ndim < 1 || ndim > 3) return fail(TNL_EINVAL, "ndim must be 1, 2 or 3");
+  if (!im_size || !pad_size) return fail(TNL_EINVAL, "null size array");
+  if (nbatch < 0) return fail(TNL_EINVAL, "nbatch must be >= 0");
+  WrapGeom g;
+  g.nbatch = nbatch;
+  for (int d = 0; d < 3; ++d) g.N[d] = g.P[d] = 1;
+  for (int d = 0; d < ndim; ++d) {
+    const int64_t N = im_size[d], P = pad_size[d];
+    if (N < 1 || P < N)
+      return fail(TNL_EINVAL, "need 1 <= im_size <= pad_size on every axis");
+    g.N[3 - ndim + d] = N;
+    g.P[3 - ndim + d] = P;
+  }
+  if (nbatch == 0) return TNL_OK;
+  if (!in || !out) return fail(TNL_EINVAL, "null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  switch (elem_bytes) {
+    case 4: return launch_wrap<4>(pad, in, out, g, s);
+    case 8: return launch_wrap<8>(pad, in, out, g, s);
+    case 16: return launch_wrap<16>(pad, in, out, g, s);
+    default: return fail(TNL_EINVAL, "elem_bytes must be 4, 8 or 16");
+  }
+}
+
+}  // namespace
+
+}  // namespace tnl
+
+using namespace tnl;
+
+extern "C" int tnl_pad_wrapped(const void* x, void* y, int elem_bytes, int64_t nbatch,
+                               int ndim, const int64_t* im_size,
+                               const int64_t* pad_size, void* stream) {
+  return wrap_entry(true, x, y, elem_bytes, nbatch, ndim, im_size, pad_size, stream);
+}
+
+extern "C" int tnl_crop_wrapped(const void* y, void* x, int elem_bytes, int64_t nbatch,
+                                int ndim, const int64_t* im_size,
+                                const int64_t* pad_size, void* stream) {
+  return wrap_entry(false, y, x, elem_bytes, nbatch, ndim, im_size, pad_size, stream);
+}

@@ -4,9 +4,10 @@ from .interp import ArrayToBlocks, BlocksToArray, Interpolate
 from .namedlinop import NamedLinop
 from .nufft import NUFFT, cfftn, cifftn, toeplitz_psf
 from .sampling import Sampling
+from .toeplitz import ToeplitzNormal
 
 __all__ = [
     "NamedLinop", "Chain", "Add", "Identity", "Diagonal", "Scalar", "Pad", "FFT", "Dense",
-    "Interpolate", "ArrayToBlocks", "BlocksToArray", "NUFFT", "Sampling", "toeplitz_psf",
+    "Interpolate", "ArrayToBlocks", "BlocksToArray", "NUFFT", "Sampling", "ToeplitzNormal", "toeplitz_psf",
     "cfftn", "cifftn", "pad_to_size", "crop_slice_from_pad",
 ]

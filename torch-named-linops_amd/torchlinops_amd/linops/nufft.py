@@ -9,7 +9,9 @@ Follows src/torchlinops/linops/nufft.py:
   * prep_locs (241-334), beta (336-346), apodize_weights (348-365);
   * adjoint (217-224); normal with ``toeplitz=True`` (226-239) builds
     Crop . F^H . (K *) . F . Pad with K the Fourier transform of the point
-    spread function on a 2x grid (toeplitz_psf, 395-494).
+    spread function on a 2x grid (toeplitz_psf, 395-494), applied as one
+    ToeplitzNormal (shift-free pad / crop copies around rocFFT; option
+    ``toeplitz_fused=False`` returns the reference's chain).
 The interpolation runs on the HIP kernels through Interpolate.
 """
 
@@ -27,6 +29,7 @@ from .basic import FFT, Diagonal, Identity, Pad, Scalar, crop_slice_from_pad, pa
 from .chain import Chain
 from .interp import Interpolate
 from .sampling import Sampling
+from .toeplitz import ToeplitzNormal
 
 __all__ = ["NUFFT", "toeplitz_psf", "cfftn", "cifftn"]
 
@@ -141,11 +144,15 @@ class NUFFT(Chain):
 
     def normal(self, inner=None):
         if self.options.get("toeplitz", False):
+            os_ = self.options.get("toeplitz_oversamp", 2.0)
             kernel = toeplitz_psf(self, inner, dtype=self.options.get("toeplitz_dtype"),
-                                  oversamp=self.options.get("toeplitz_oversamp", 2.0))
-            pad = Pad(scale_int(self.grid_size, self.options.get("toeplitz_oversamp", 2.0)),
-                      self.grid_size, in_shape=self.input_shape, batch_shape=self.batch_shape)
-            return pad.normal(self.fft.normal(kernel))
+                                  oversamp=os_)
+            pad = Pad(scale_int(self.grid_size, os_), self.grid_size,
+                      in_shape=self.input_shape, batch_shape=self.batch_shape)
+            if not self.options.get("toeplitz_fused", True):
+                return pad.normal(self.fft.normal(kernel))
+            return ToeplitzNormal(kernel.weight, self.grid_size, scale_int(self.grid_size, os_),
+                                  pad.ishape)
         return super().normal(inner)
 
     @staticmethod

@@ -1,0 +1,97 @@
+"""Toeplitz normal operator with the centring shifts folded into the pad.
+
+The reference applies a NUFFT's Toeplitz normal as the chain
+Crop . F^H . (K *) . F . Pad with centred FFTs (src/torchlinops/linops/
+nufft.py:226-239; fft.py:84-131; utils/_fft.py:7-60): a zero-pad copy,
+ifftshift / fftshift rolls around both FFTs and a crop copy.  Because
+ifftshift(K * fftshift(v)) = ifftshift(K) * v, the same operator is
+
+    crop_wrapped( ifftn( K_nat * fftn( pad_wrapped(x) ) ) )
+
+with K_nat = ifftshift(K) stored once, pad_wrapped = ifftshift . Pad and
+crop_wrapped = Crop . fftshift each one HIP copy (tnl_pad_wrapped /
+tnl_crop_wrapped), and the multiply done in place between the two rocFFT
+calls.  Five full passes of the padded grid (pad, two rolls, multiply, two
+rolls) become two partial copies and one multiply.
+"""
+
+from __future__ import annotations
+
+from math import prod
+
+import torch
+import torch.nn as nn
+
+from .._lib import check, i64_array, lib, require_device, stream_of
+from ..nameddim import NamedShape as NS
+from .namedlinop import NamedLinop
+
+__all__ = ["pad_wrapped", "crop_wrapped", "ToeplitzNormal"]
+
+
+def _wrap(pad: bool, x: torch.Tensor, im_size, pad_size) -> torch.Tensor:
+    require_device(x, op="pad_wrapped" if pad else "crop_wrapped")
+    nd = len(im_size)
+    src_size = tuple(im_size) if pad else tuple(pad_size)
+    if tuple(x.shape[x.ndim - nd:]) != src_size:
+        raise ValueError(f"Mismatched shapes: expected {src_size} but got "
+                         f"{tuple(x.shape[x.ndim - nd:])}")
+    batch = tuple(x.shape[: x.ndim - nd])
+    x = x.contiguous()
+    out = torch.empty(batch + (tuple(pad_size) if pad else tuple(im_size)),
+                      dtype=x.dtype, device=x.device)
+    if out.numel() == 0:
+        return out
+    fn = lib().tnl_pad_wrapped if pad else lib().tnl_crop_wrapped
+    with torch.cuda.device(x.device):
+        check(fn(x.data_ptr(), out.data_ptr(), x.element_size(), prod(batch), nd,
+                 i64_array(im_size), i64_array(pad_size), stream_of(x)))
+    return out
+
+
+def pad_wrapped(x: torch.Tensor, im_size, pad_size) -> torch.Tensor:
+    """``ifftshift(centred_pad(x))`` over the last ``len(im_size)`` axes."""
+    return _wrap(True, x, im_size, pad_size)
+
+
+def crop_wrapped(y: torch.Tensor, im_size, pad_size) -> torch.Tensor:
+    """``centred_crop(fftshift(y))`` over the last ``len(im_size)`` axes."""
+    return _wrap(False, y, im_size, pad_size)
+
+
+class ToeplitzNormal(NamedLinop):
+    """``Crop . F^H . (K *) . F . Pad`` applied as pad_wrapped -> fftn ->
+    multiply by ifftshift(K) -> ifftn -> crop_wrapped.  ``kernel`` is the
+    centred Fourier-domain PSF ``[..., *pad_size]`` of ``toeplitz_psf``;
+    the adjoint multiplies by its conjugate."""
+
+    def __init__(self, kernel: torch.Tensor, im_size, pad_size, ishape, name="ToeplitzNormal"):
+        super().__init__(NS(tuple(ishape), tuple(ishape)), name=name)
+        self.im_size = tuple(int(n) for n in im_size)
+        self.pad_size = tuple(int(n) for n in pad_size)
+        dims = tuple(range(-len(self.im_size), 0))
+        self.kernel = nn.Parameter(torch.fft.ifftshift(kernel, dim=dims).contiguous(),
+                                   requires_grad=False)
+
+    def _toeplitz(self, x, conj):
+        nd = len(self.im_size)
+        dims = tuple(range(-nd, 0))
+        y = pad_wrapped(x, self.im_size, self.pad_size)
+        y = torch.fft.fftn(y, dim=dims, norm="ortho")
+        y.mul_(self.kernel.conj() if conj else self.kernel)
+        y = torch.fft.ifftn(y, dim=dims, norm="ortho")
+        return crop_wrapped(y, self.im_size, self.pad_size)
+
+    @staticmethod
+    def fn(t, x, /):
+        return t._toeplitz(x, False)
+
+    @staticmethod
+    def adj_fn(t, x, /):
+        return t._toeplitz(x, True)
+
+    def size(self, dim):
+        shape = list(self.ishape)
+        if dim in shape[len(shape) - len(self.im_size):]:
+            return self.im_size[shape.index(dim) - len(shape) + len(self.im_size)]
+        return None
