@@ -173,6 +173,48 @@ def toeplitz_leg(dev, steps: int = 10, n: int = 128, C: int = 8):
     return out
 
 
+def nufft_leg(dev, steps: int = 10, n: int = 128, C: int = 8):
+    """NUFFT forward + adjoint through the linop (128^3 image, 8 coils,
+    8000x256 radial, oversamp 1.25, KB w=4): the fused path (weighted
+    wrapped pad, uncentred rocFFT, relabelled interpolation) against the
+    reference-shaped chain (Apodize, Pad, centred FFT, Interpolate,
+    Scalar)."""
+    import numpy as np
+    import torch
+
+    import torchlinops_amd as tl
+
+    gen = torch.Generator().manual_seed(6)
+    locs = (radial_traj(8000, 256, n, gen, "cpu") - n / 2).to(dev)
+    x = torch.randn(C, n, n, n, dtype=torch.complex64, generator=gen).to(dev)
+    y = torch.randn(C, 8000, 256, dtype=torch.complex64, generator=gen).to(dev)
+    out = {}
+    for label, fused in (("fused", True), ("chain", False)):
+        A = tl.NUFFT(locs, (n, n, n), output_shape=("R", "K"), batch_shape=("C",),
+                     oversamp=1.25, fused=fused)
+        AH = A.H
+        for _ in range(2):
+            A(x)
+            AH(y)
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+        torch.cuda.synchronize()
+        for e in ev:
+            e[0].record()
+            A(x)
+            e[1].record()
+            AH(y)
+            e[2].record()
+        torch.cuda.synchronize()
+        out[f"{label}_forward_ms"] = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+        out[f"{label}_adjoint_ms"] = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+        del A, AH
+    out["speedup"] = ((out["chain_forward_ms"] + out["chain_adjoint_ms"])
+                      / (out["fused_forward_ms"] + out["fused_adjoint_ms"]))
+    out["workload"] = (f"NUFFT linop forward + adjoint, {C} coils, {n}^3 image, 8000x256 "
+                       f"radial, oversamp 1.25, KB w=4, complex64")
+    return out
+
+
 def c3_batched_cg(dev, iters: int = 20, nprob: int = 64, seed: int = 3):
     """64 independent 2-D NUFFT problems of uneven size (grids 128..512^2,
     1e4..5e5 points on variable-density spirals, 1..32 coils, widths 4..6),
@@ -392,7 +434,7 @@ def main() -> int:
     ap.add_argument("--no-sampling", action="store_true",
                     help="skip the sampling-mode gather / scatter-add leg")
     ap.add_argument("--no-toeplitz", action="store_true",
-                    help="skip the Toeplitz normal leg")
+                    help="skip the NUFFT-chain and Toeplitz normal legs")
     ap.add_argument("--small", action="store_true", help="reduced size (debug only)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one elementwise copy of the 8 x 256^3 complex64 grid "
@@ -538,6 +580,12 @@ def main() -> int:
         log(f"sampling: gather {out['sampling']['gather_us']:.1f} us, "
             f"adjoint {out['sampling']['adjoint_us']:.1f} us")
     if not args.no_toeplitz and not args.small:
+        log("nufft: forward + adjoint, fused vs chain ...")
+        out["nufft_chain"] = nufft_leg(dev)
+        log(f"nufft: fused {out['nufft_chain']['fused_forward_ms']:.2f} + "
+            f"{out['nufft_chain']['fused_adjoint_ms']:.2f} ms, chain "
+            f"{out['nufft_chain']['chain_forward_ms']:.2f} + "
+            f"{out['nufft_chain']['chain_adjoint_ms']:.2f} ms")
         log("toeplitz: normal apply, fused vs chain ...")
         out["toeplitz"] = toeplitz_leg(dev)
         log(f"toeplitz: fused {out['toeplitz']['fused_ms']:.2f} ms, "

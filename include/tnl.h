@@ -373,15 +373,20 @@ int tnl_index_adjoint(const void* x, const int64_t* order, const int64_t* starts
  * Together they replace the four centring rolls and the pad/crop copies
  * of the Toeplitz normal Crop . F^H . (K *) . F . Pad built at
  * src/torchlinops/linops/nufft.py:226-239, the kernel K then being taken
- * in natural FFT order.  Pure copies, bit-exact; elem_bytes in {4, 8, 16};
+ * in natural FFT order.  weight: NULL (pure copies, bit-exact;
+ * elem_bytes in {4, 8, 16}) or float32 [*im_size] multiplying every copied
+ * value (float32 / complex64 data, elem_bytes 4 or 8): the NUFFT's
+ * Apodize and 1/scale (nufft.py:166-184) folded into the same pass, so
+ * that NUFFT = interp(fftn(pad_wrapped(x, apod/scale))) at locations
+ * relabelled by -pad_size/2 and NUFFT^H its mirror.
  * 1 <= im_size <= pad_size; outputs fully written.
  */
 int tnl_pad_wrapped(const void* x, void* y, int elem_bytes, int64_t nbatch,
                     int ndim, const int64_t* im_size, const int64_t* pad_size,
-                    void* stream);
+                    const float* weight, void* stream);
 int tnl_crop_wrapped(const void* y, void* x, int elem_bytes, int64_t nbatch,
                      int ndim, const int64_t* im_size, const int64_t* pad_size,
-                     void* stream);
+                     const float* weight, void* stream);
 
 #ifdef __cplusplus
 }

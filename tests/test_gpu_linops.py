@@ -153,6 +153,12 @@ def test_wrapped_pad_crop_bit_exact(dev, im, pad, dtype):
     c = crop_wrapped(z.to(dev), im, pad)
     assert np.array_equal(c.cpu().numpy(), O.crop_wrapped(z.numpy(), im, pad))
     assert torch.equal(crop_wrapped(y, im, pad).cpu(), x)
+    if dtype in (torch.float32, torch.complex64):
+        w = torch.rand(im, generator=g) + 0.5
+        yw = pad_wrapped(x.to(dev), im, pad, weight=w.to(dev))
+        assert np.array_equal(yw.cpu().numpy(), O.pad_wrapped(x.numpy() * w.numpy(), im, pad))
+        cw = crop_wrapped(z.to(dev), im, pad, weight=w.to(dev))
+        assert np.array_equal(cw.cpu().numpy(), O.crop_wrapped(z.numpy(), im, pad) * w.numpy())
 
 
 def test_cg_on_nufft_normal():
@@ -192,3 +198,34 @@ def test_cg_graph_replays_match_eager():
     assert rel_l2(xg.cpu().numpy(), xe.cpu().numpy()) < 1e-4
     for he, hg in zip(eager.history, graph.history):
         assert abs(he - hg) <= 1e-3 * max(he, 1e-12)
+
+
+@pytest.mark.parametrize("grid", [(20, 18), (16, 15, 14)])
+def test_fused_nufft_matches_chain(dev, grid):
+    """The fused NUFFT (weighted wrapped pad, uncentred FFT, relabelled
+    interpolation) against the reference-shaped chain: forward, adjoint,
+    normal, the functional API, and the autograd fallback."""
+    print(f"fused nufft grid={grid}", flush=True)
+    rng = np.random.default_rng(8)
+    nd = len(grid)
+    locs = torch.from_numpy((rng.uniform(-0.5, 0.5, (3, 400, nd)) * np.array(grid))
+                            .astype(np.float32)).to(dev)
+    A = tl.NUFFT(locs, grid, output_shape=("R", "K"), batch_shape=("C",), oversamp=1.25)
+    B = tl.NUFFT(locs, grid, output_shape=("R", "K"), batch_shape=("C",), oversamp=1.25,
+                 fused=False)
+    assert A._fused_on and not B._fused_on
+    x = torch.randn(2, *grid, dtype=torch.complex64, device=dev)
+    y = torch.randn(2, 3, 400, dtype=torch.complex64, device=dev)
+    assert rel_l2(A(x).cpu().numpy(), B(x).cpu().numpy()) < 1e-5
+    assert rel_l2(A.H(y).cpu().numpy(), B.H(y).cpu().numpy()) < 1e-5
+    assert rel_l2(A.N(x).cpu().numpy(), B.N(x).cpu().numpy()) < 1e-5
+    assert rel_l2(A.H.H(x).cpu().numpy(), B(x).cpu().numpy()) < 1e-5
+    assert [l.repr_name for l in A.N.linops] == [l.repr_name for l in B.N.linops]
+    f = F.nufft(x, locs, oversamp=1.25, width=4.0)
+    fh = F.nufft_adjoint(y, locs, grid, oversamp=1.25, width=4.0)
+    assert rel_l2(f.cpu().numpy(), B(x).cpu().numpy()) < 1e-5
+    assert rel_l2(fh.cpu().numpy(), B.H(y).cpu().numpy()) < 1e-5
+    xg = x.clone().requires_grad_()
+    out = A(xg)  # autograd needs the chain
+    out.abs().sum().backward()
+    assert xg.grad is not None and rel_l2(out.detach().cpu().numpy(), B(x).cpu().numpy()) < 1e-5

@@ -8,6 +8,9 @@
 //   pad_wrapped:  y[n, q] = x[n, i] if every axis has an image index whose
 //                 c_d == q_d (mod P_d), else 0;
 //   crop_wrapped: x[n, i] = y[n, (i - N/2) mod P]  (fftshift then crop).
+// An optional real weight w[i] (float32, one per image cell, shared by
+// the batch) multiplies each copied value: the NUFFT's apodization and
+// 1/scale folded into the same pass (float32 and complex64 data).
 // Both are HBM-bound copies.  A workgroup walks whole rows of the last
 // axis: the row's outer coordinates are decoded once, lanes move
 // consecutive elements, so reads and writes are coalesced except at the
@@ -41,9 +44,21 @@ __device__ __forceinline__ int64_t wrap_source(int64_t q, int64_t N, int64_t P) 
   return (i >= 0 && i < N) ? i : -1;
 }
 
-template <typename E>
+// value times a real weight: float32 and interleaved complex64 elements
+__device__ __forceinline__ uint32_t scaled(uint32_t v, float w) {
+  return __float_as_uint(__uint_as_float(v) * w);
+}
+__device__ __forceinline__ uint2 scaled(uint2 v, float w) {
+  return make_uint2(__float_as_uint(__uint_as_float(v.x) * w),
+                    __float_as_uint(__uint_as_float(v.y) * w));
+}
+__device__ __forceinline__ uint4 scaled(uint4 v, float) { return v; }  // not launched
+
+template <typename E, bool W>
 __global__ __launch_bounds__(256) void pad_wrapped_kernel(const E* __restrict__ x,
-                                                          E* __restrict__ y, WrapGeom g) {
+                                                          E* __restrict__ y,
+                                                          const float* __restrict__ w,
+                                                          WrapGeom g) {
   const int64_t rows = g.nbatch * g.P[0] * g.P[1];
   for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
     const int64_t q1 = r % g.P[1];
@@ -54,21 +69,27 @@ __global__ __launch_bounds__(256) void pad_wrapped_kernel(const E* __restrict__ 
     const int64_t i1 = wrap_source(q1, g.N[1], g.P[1]);
     E* dst = y + r * g.P[2];
     const bool row_ok = i0 >= 0 && i1 >= 0;
-    const E* src = x + ((n * g.N[0] + (row_ok ? i0 : 0)) * g.N[1] + (row_ok ? i1 : 0)) * g.N[2];
+    const int64_t img_row = ((row_ok ? i0 : 0) * g.N[1] + (row_ok ? i1 : 0)) * g.N[2];
+    const E* src = x + n * g.N[0] * g.N[1] * g.N[2] + img_row;
     for (int64_t q2 = threadIdx.x; q2 < g.P[2]; q2 += blockDim.x) {
       E v{};
       if (row_ok) {
         const int64_t i2 = wrap_source(q2, g.N[2], g.P[2]);
-        if (i2 >= 0) v = src[i2];
+        if (i2 >= 0) {
+          v = src[i2];
+          if (W) v = scaled(v, w[img_row + i2]);
+        }
       }
       dst[q2] = v;
     }
   }
 }
 
-template <typename E>
+template <typename E, bool W>
 __global__ __launch_bounds__(256) void crop_wrapped_kernel(const E* __restrict__ y,
-                                                           E* __restrict__ x, WrapGeom g) {
+                                                           E* __restrict__ x,
+                                                           const float* __restrict__ w,
+                                                           WrapGeom g) {
   const int64_t rows = g.nbatch * g.N[0] * g.N[1];
   for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
     const int64_t i1 = r % g.N[1];
@@ -79,10 +100,13 @@ __global__ __launch_bounds__(256) void crop_wrapped_kernel(const E* __restrict__
     const int64_t q1 = ((i1 - g.N[1] / 2) % g.P[1] + g.P[1]) % g.P[1];
     const E* src = y + ((n * g.P[0] + q0) * g.P[1] + q1) * g.P[2];
     E* dst = x + r * g.N[2];
+    const float* wrow = W ? w + (i0 * g.N[1] + i1) * g.N[2] : nullptr;
     for (int64_t i2 = threadIdx.x; i2 < g.N[2]; i2 += blockDim.x) {
       int64_t q2 = i2 - g.N[2] / 2;
       if (q2 < 0) q2 += g.P[2];
-      dst[i2] = src[q2];
+      E v = src[q2];
+      if (W) v = scaled(v, wrow[i2]);
+      dst[i2] = v;
     }
   }
 }
@@ -92,23 +116,25 @@ unsigned row_blocks(int64_t rows) {
   return (unsigned)(rows < 1 ? 1 : (rows > cap ? cap : rows));
 }
 
-template <int EB>
-int launch_wrap(bool pad, const void* in, void* out, const WrapGeom& g, hipStream_t s) {
+template <int EB, bool W>
+int launch_wrap(bool pad, const void* in, void* out, const float* w, const WrapGeom& g,
+                hipStream_t s) {
   using E = typename WElem<EB>::T;
   if (pad) {
-    pad_wrapped_kernel<E><<<row_blocks(g.nbatch * g.P[0] * g.P[1]), 256, 0, s>>>(
-        (const E*)in, (E*)out, g);
+    pad_wrapped_kernel<E, W><<<row_blocks(g.nbatch * g.P[0] * g.P[1]), 256, 0, s>>>(
+        (const E*)in, (E*)out, w, g);
     TNL_LAUNCH_CHECK("pad_wrapped");
   } else {
-    crop_wrapped_kernel<E><<<row_blocks(g.nbatch * g.N[0] * g.N[1]), 256, 0, s>>>(
-        (const E*)in, (E*)out, g);
+    crop_wrapped_kernel<E, W><<<row_blocks(g.nbatch * g.N[0] * g.N[1]), 256, 0, s>>>(
+        (const E*)in, (E*)out, w, g);
     TNL_LAUNCH_CHECK("crop_wrapped");
   }
   return TNL_OK;
 }
 
 int wrap_entry(bool pad, const void* in, void* out, int elem_bytes, int64_t nbatch,
-               int ndim, const int64_t* im_size, const int64_t* pad_size, void* stream) {
+               int ndim, const int64_t* im_size, const int64_t* pad_size,
+               const float* weight, void* stream) {
   if (ndim < 1 || ndim > 3) return fail(TNL_EINVAL, "ndim must be 1, 2 or 3");
   if (!im_size || !pad_size) return fail(TNL_EINVAL, "null size array");
   if (nbatch < 0) return fail(TNL_EINVAL, "nbatch must be >= 0");
@@ -125,10 +151,18 @@ int wrap_entry(bool pad, const void* in, void* out, int elem_bytes, int64_t nbat
   if (nbatch == 0) return TNL_OK;
   if (!in || !out) return fail(TNL_EINVAL, "null pointer");
   hipStream_t s = (hipStream_t)stream;
+  if (weight) {
+    switch (elem_bytes) {
+      case 4: return launch_wrap<4, true>(pad, in, out, weight, g, s);
+      case 8: return launch_wrap<8, true>(pad, in, out, weight, g, s);
+      default:
+        return fail(TNL_EINVAL, "a weight needs float32 or complex64 data (elem_bytes 4 or 8)");
+    }
+  }
   switch (elem_bytes) {
-    case 4: return launch_wrap<4>(pad, in, out, g, s);
-    case 8: return launch_wrap<8>(pad, in, out, g, s);
-    case 16: return launch_wrap<16>(pad, in, out, g, s);
+    case 4: return launch_wrap<4, false>(pad, in, out, nullptr, g, s);
+    case 8: return launch_wrap<8, false>(pad, in, out, nullptr, g, s);
+    case 16: return launch_wrap<16, false>(pad, in, out, nullptr, g, s);
     default: return fail(TNL_EINVAL, "elem_bytes must be 4, 8 or 16");
   }
 }
@@ -141,12 +175,16 @@ using namespace tnl;
 
 extern "C" int tnl_pad_wrapped(const void* x, void* y, int elem_bytes, int64_t nbatch,
                                int ndim, const int64_t* im_size,
-                               const int64_t* pad_size, void* stream) {
-  return wrap_entry(true, x, y, elem_bytes, nbatch, ndim, im_size, pad_size, stream);
+                               const int64_t* pad_size, const float* weight,
+                               void* stream) {
+  return wrap_entry(true, x, y, elem_bytes, nbatch, ndim, im_size, pad_size, weight,
+                    stream);
 }
 
 extern "C" int tnl_crop_wrapped(const void* y, void* x, int elem_bytes, int64_t nbatch,
                                 int ndim, const int64_t* im_size,
-                                const int64_t* pad_size, void* stream) {
-  return wrap_entry(false, y, x, elem_bytes, nbatch, ndim, im_size, pad_size, stream);
+                                const int64_t* pad_size, const float* weight,
+                                void* stream) {
+  return wrap_entry(false, y, x, elem_bytes, nbatch, ndim, im_size, pad_size, weight,
+                    stream);
 }

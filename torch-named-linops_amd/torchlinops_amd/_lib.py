@@ -222,8 +222,8 @@ SIGNATURES = {
         [_vp, _vp, _i32, _i64, _i32, _i64p, _i64p, _i64p, _i64p, _vp],
     ),
     "tnl_index": (_i32, [_vp, _vp, _vp, _i32, _i64, _i64, _i64, _vp]),
-    "tnl_pad_wrapped": (_i32, [_vp, _vp, _i32, _i64, _i32, _i64p, _i64p, _vp]),
-    "tnl_crop_wrapped": (_i32, [_vp, _vp, _i32, _i64, _i32, _i64p, _i64p, _vp]),
+    "tnl_pad_wrapped": (_i32, [_vp, _vp, _i32, _i64, _i32, _i64p, _i64p, _vp, _vp]),
+    "tnl_crop_wrapped": (_i32, [_vp, _vp, _i32, _i64, _i32, _i64p, _i64p, _vp, _vp]),
     "tnl_index_adjoint": (
         _i32, [_vp, _vp, _vp, _vp, _i64, _i64, _vp, _i32, _i32, _i64, _i64, _i64, _vp]),
 }

@@ -126,6 +126,61 @@ class NUFFT(Chain):
         scale = Scalar(weight=1.0 / scale_factor, ioshape=interp.oshape)
         super().__init__(apodize, pad, fft, interp, scale, name="NUFFT")
         self._finish(locs, grid_size, oversamp, width, pad, fft, interp)
+        if (tuple(weight.shape) == tuple(grid_size) and not weight.is_complex()
+                and options.get("fused", True)):
+            # Apodize . Pad . centred FFT . Interp . Scalar as one weighted
+            # wrapped pad, an uncentred FFT and interpolation at locations
+            # relabelled by -N_pad/2 (exact under circular padding)
+            shift = torch.tensor([n // 2 for n in padded_size], dtype=locs_p.dtype,
+                                 device=locs_p.device)
+            size = torch.tensor(padded_size, dtype=locs_p.dtype, device=locs_p.device)
+            self.register_buffer("_fused_weight", (weight.float() / scale_factor).contiguous(),
+                                 persistent=False)
+            self.register_buffer("_fused_locs",
+                                 torch.remainder(locs_p.float() - shift, size).contiguous(),
+                                 persistent=False)
+            self._fused_on = True
+        self._fused_dir = 1
+
+    _fused_on = False
+    _fused_dir = 1
+
+    def _fused_ok(self, x) -> bool:
+        return (self._fused_on and x.is_cuda and x.dtype == torch.complex64
+                and not (torch.is_grad_enabled() and x.requires_grad))
+
+    def _fused(self, x, forward: bool):
+        from .toeplitz import crop_wrapped, pad_wrapped
+        from .. import functional as F
+
+        grid, padded = self.grid_size, tuple(self.pad.pad_im_size)
+        dims = tuple(range(-len(grid), 0))
+        p = self.interp.interp_params
+        if forward:
+            y = pad_wrapped(x, grid, padded, weight=self._fused_weight)
+            y = torch.fft.fftn(y, dim=dims, norm="ortho")
+            return F.interpolate(y, self._fused_locs, **p)
+        g = F.interpolate_adjoint(x, self._fused_locs, padded, **p)
+        g = torch.fft.ifftn(g, dim=dims, norm="ortho")
+        return crop_wrapped(g, grid, padded, weight=self._fused_weight)
+
+    @staticmethod
+    def fn(nufft, x, /):
+        if nufft._fused_ok(x):
+            return nufft._fused(x, nufft._fused_dir > 0)
+        return Chain.fn(nufft, x)
+
+    @staticmethod
+    def adj_fn(nufft, x, /):
+        if nufft._fused_ok(x):
+            return nufft._fused(x, nufft._fused_dir < 0)
+        return Chain.adj_fn(nufft, x)
+
+    @staticmethod
+    def split(nufft, tile):
+        out = Chain.split(nufft, tile)
+        out._fused_on = False
+        return out
 
     def _finish(self, locs, grid_size, oversamp, width, pad, fft, interp):
         self.locs = locs
@@ -140,6 +195,7 @@ class NUFFT(Chain):
         adj = copy(self)
         adj._shape = adj._shape.H
         adj.linops = nn.ModuleList([l.adjoint() for l in reversed(self.linops)])
+        adj._fused_dir = -self._fused_dir
         return adj
 
     def normal(self, inner=None):
@@ -153,7 +209,18 @@ class NUFFT(Chain):
                 return pad.normal(self.fft.normal(kernel))
             return ToeplitzNormal(kernel.weight, self.grid_size, scale_int(self.grid_size, os_),
                                   pad.ishape)
-        return super().normal(inner)
+        normal = super().normal(inner)
+        if inner is None and self._fused_on and self._fused_dir > 0:
+            nufft = self
+
+            def fused_normal(chain, x, /):
+                if nufft._fused_ok(x):
+                    return nufft._fused(nufft._fused(x, True), False)
+                return Chain.fn(chain, x)
+
+            normal.fn = fused_normal
+            normal.adj_fn = fused_normal
+        return normal
 
     @staticmethod
     def prep_locs(locs, grid_size, padded_size, pad_mode="circular", nufft_mode="interpolate"):

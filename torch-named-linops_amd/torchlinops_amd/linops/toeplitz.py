@@ -29,8 +29,15 @@ from .namedlinop import NamedLinop
 __all__ = ["pad_wrapped", "crop_wrapped", "ToeplitzNormal"]
 
 
-def _wrap(pad: bool, x: torch.Tensor, im_size, pad_size) -> torch.Tensor:
+def _wrap(pad: bool, x: torch.Tensor, im_size, pad_size, weight=None) -> torch.Tensor:
     require_device(x, op="pad_wrapped" if pad else "crop_wrapped")
+    if weight is not None:
+        if weight.dtype != torch.float32 or tuple(weight.shape) != tuple(im_size):
+            raise ValueError(f"weight must be float32 of shape {tuple(im_size)}, got "
+                             f"{weight.dtype} {tuple(weight.shape)}")
+        if x.dtype not in (torch.float32, torch.complex64):
+            raise TypeError(f"a weight needs float32 or complex64 data, got {x.dtype}")
+        weight = weight.to(x.device).contiguous()
     nd = len(im_size)
     src_size = tuple(im_size) if pad else tuple(pad_size)
     if tuple(x.shape[x.ndim - nd:]) != src_size:
@@ -45,18 +52,21 @@ def _wrap(pad: bool, x: torch.Tensor, im_size, pad_size) -> torch.Tensor:
     fn = lib().tnl_pad_wrapped if pad else lib().tnl_crop_wrapped
     with torch.cuda.device(x.device):
         check(fn(x.data_ptr(), out.data_ptr(), x.element_size(), prod(batch), nd,
-                 i64_array(im_size), i64_array(pad_size), stream_of(x)))
+                 i64_array(im_size), i64_array(pad_size),
+                 weight.data_ptr() if weight is not None else None, stream_of(x)))
     return out
 
 
-def pad_wrapped(x: torch.Tensor, im_size, pad_size) -> torch.Tensor:
-    """``ifftshift(centred_pad(x))`` over the last ``len(im_size)`` axes."""
-    return _wrap(True, x, im_size, pad_size)
+def pad_wrapped(x: torch.Tensor, im_size, pad_size, weight=None) -> torch.Tensor:
+    """``ifftshift(centred_pad(x * weight))`` over the last ``len(im_size)``
+    axes (``weight``: optional real float32 ``[*im_size]``)."""
+    return _wrap(True, x, im_size, pad_size, weight)
 
 
-def crop_wrapped(y: torch.Tensor, im_size, pad_size) -> torch.Tensor:
-    """``centred_crop(fftshift(y))`` over the last ``len(im_size)`` axes."""
-    return _wrap(False, y, im_size, pad_size)
+def crop_wrapped(y: torch.Tensor, im_size, pad_size, weight=None) -> torch.Tensor:
+    """``centred_crop(fftshift(y)) * weight`` over the last ``len(im_size)``
+    axes."""
+    return _wrap(False, y, im_size, pad_size, weight)
 
 
 class ToeplitzNormal(NamedLinop):
