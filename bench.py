@@ -395,6 +395,19 @@ def c5_learned_cg(dev, seed: int = 5, train_n: int = 32, train_problems: int = 6
                                                    - x_ref) / nref)
     log(f"c5: learned PCG {learned['iters']} iterations, {learned['seconds']:.2f} s "
         f"(+ {infer_s * 1e3:.1f} ms inference), fallback {learned['fallback']}")
+    # classical baseline: l_2-optimal polynomial preconditioners p(A) on the
+    # normalised spectrum [0, 1 + lam] (each PCG iteration costs deg + 1 A's)
+    from torchlinops_amd.alg import poly as Pm
+
+    poly = {}
+    for deg in (1, 3):
+        coef, _ = Pm.l_2_opt(deg, 0.0, 1.0 + A.lam)
+        _, info = Lm.solve(A, y, lambda r, c=coef: Pm.apply_polynomial(A, c, r),
+                           tol=1e-6, max_iters=3000, check_every=10)
+        poly[f"deg{deg}"] = {"seconds": info["seconds"], "iters": info["iters"],
+                             "operator_applies": info["iters"] * (deg + 1),
+                             "fallback": info["fallback"]}
+        log(f"c5: poly(l_2, deg {deg}) PCG {info['iters']} iterations, {info['seconds']:.2f} s")
     total = learned["seconds"] + infer_s
     return {
         "workload": "SENSE NUFFT normal equations, 128^3 image, 8 coils, 8000x256 radial, "
@@ -409,6 +422,7 @@ def c5_learned_cg(dev, seed: int = 5, train_n: int = 32, train_problems: int = 6
         "inference_ms": infer_s * 1e3,
         "jacobi_pcg_s": jacobi["seconds"],
         "jacobi_pcg_iters": jacobi["iters"],
+        "poly_l2_pcg": poly,
         "error_after_20_iters_plain": e20_plain,
         "error_after_20_iters_learned": e20_learn,
         "time_ratio_learned_over_plain": total / plain["seconds"],

@@ -255,3 +255,73 @@ def test_index_helpers():
     t = torch.tensor([[1, -1], [7, 2]])
     assert torch.equal(F.ravel(t, (4, 3), dim=-1), torch.tensor([1 * 3 + 2, 3 * 3 + 2]))
     assert len(F.canonicalize_idx(torch.zeros(5, 3, dtype=torch.long))) == 3
+
+
+# ------------------------------------------------ polynomial preconditioner
+def test_poly_coefficients_closed_forms():
+    from torchlinops_amd.alg import poly as P
+
+    assert np.array_equal(P.ifista_coeffs(0), [1.0])
+    assert np.array_equal(P.ifista_coeffs(2), [3.0, -3.0, 1.0])
+    # degree 0: c0 = int x / int x^2 (l_2) and 2 / (a + b) (l_inf)
+    a, b = 0.2, 1.7
+    c2, _ = P.l_2_opt(0, a, b)
+    assert abs(c2[0] - 1.5 * (b**2 - a**2) / (b**3 - a**3)) < 1e-6
+    ci, _ = P.l_inf_opt(0, a, b)
+    assert abs(ci[0] - 2 / (a + b)) < 1e-6
+
+
+@pytest.mark.parametrize("degree", [1, 2, 4])
+def test_poly_l2_is_the_least_squares_fit(degree):
+    """l_2 coefficients equal a Gauss-Legendre least-squares fit of
+    x p(x) ~ 1 (exact quadrature for these degrees)."""
+    from torchlinops_amd.alg import poly as P
+
+    a, b = 0.0, 1.0
+    c, _ = P.l_2_opt(degree, a, b)
+    xg, wg = np.polynomial.legendre.leggauss(degree + 4)
+    x = (b - a) / 2 * xg + (a + b) / 2
+    w = np.sqrt(wg * (b - a) / 2)
+    V = x[:, None] ** (np.arange(degree + 1)[None, :] + 1)
+    ref = np.linalg.lstsq(V * w[:, None], w, rcond=None)[0]
+    assert np.allclose(c, ref, rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("degree", [1, 3, 5])
+def test_poly_linf_equioscillates(degree):
+    """|1 - x p(x)| <= 1/T_{d+1}((b+a)/(b-a)) on [a, b], with the bound
+    reached at d + 2 points (Chebyshev alternation)."""
+    from torchlinops_amd.alg import poly as P
+
+    a, b = 0.05, 1.0
+    _, p = P.l_inf_opt(degree, a, b)
+    x = np.linspace(a, b, 200001)
+    r = 1 - x * p(x)
+    bound = 1 / np.cosh((degree + 1) * np.arccosh((b + a) / (b - a)))
+    assert np.abs(r).max() <= bound * (1 + 1e-6)
+    peaks = np.sum(np.abs(np.abs(r) - bound) < 1e-6 * bound + 1e-12)
+    assert peaks >= degree + 2
+
+
+def test_polynomial_preconditioner_linop_on_a_diagonal():
+    """P(T) x on a diagonal T equals p(lambda) x, for the linop and for
+    Horner's rule on tensors."""
+    from torchlinops_amd.alg import poly as P
+
+    lam = torch.linspace(0.01, 1.0, 50, dtype=torch.float64)
+    T = tl.Diagonal(lam, ioshape=("N",))
+    x = torch.randn(50, dtype=torch.float64)
+    # (l_inf on [0, 1] at even d + 1 gives p(1) = 0, which the injectivity
+    # check rejects as the reference's does)
+    for norm, lo in (("l_2", 0.0), ("l_inf", 0.05), ("ifista", 0.0)):
+        c = P._coeffs(3, norm, lo, 1.0)
+        want = np.polynomial.polynomial.polyval(lam.numpy(), np.asarray(c, float)) * x.numpy()
+        got = P.polynomial_preconditioner(T, 3, norm, lower_eig=lo)(x)
+        assert np.allclose(got.numpy(), want, rtol=1e-6)
+        got2 = P.apply_polynomial(T, c, x)
+        assert np.allclose(got2.numpy(), want, rtol=1e-6)
+    assert isinstance(P.polynomial_preconditioner(T, -1), tl.Identity)
+    with pytest.raises(AssertionError, match="injective"):
+        P.l_inf_opt(3, 0.0, 1.0)
+    with pytest.raises(ValueError, match="Unknown"):
+        P.polynomial_preconditioner(T, 2, "l_1")

@@ -326,7 +326,8 @@ def solve(A: SenseNormal, y: torch.Tensor, d: Optional[torch.Tensor] = None,
     iterate (or from zero).  Returns (x, info)."""
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    M = None if d is None else (lambda r: d * r)
+    # d: None (plain CG), an image-space diagonal, or a callable r -> M r
+    M = None if d is None else (d if callable(d) else (lambda r: d * r))
     cg = DeviceCG(A, lam=0.0, M=M, check_every=check_every)
     x = cg.solve(y, tol=tol, max_iters=max_iters)
     iters = cg.iters
