@@ -44,17 +44,19 @@ struct Elem<8> { using T = uint2; };
 template <>
 struct Elem<16> { using T = uint4; };
 
-template <typename E>
+// I: type of the work-item index and its div / mod (uint32_t when every
+// item number fits, which avoids the 64-bit division sequence per item)
+template <typename E, typename I>
 __global__ __launch_bounds__(256) void index_gather_kernel(
     const E* __restrict__ x, const int64_t* __restrict__ idx, E* __restrict__ out,
     int64_t nbatch, int64_t ngrid, int64_t npts) {
-  const int64_t nchunk = (nbatch + IB - 1) / IB;
-  const int64_t items = npts * nchunk;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < items;
-       t += stride) {
-    const int64_t p = t % npts;
-    const int64_t n0 = (t / npts) * IB;
+  const I nchunk = (I)((nbatch + IB - 1) / IB);
+  const I items = (I)npts * nchunk;
+  const I np = (I)npts;
+  const I stride = (I)gridDim.x * blockDim.x;
+  for (I t = (I)blockIdx.x * blockDim.x + threadIdx.x; t < items; t += stride) {
+    const int64_t p = (int64_t)(t % np);
+    const int64_t n0 = (int64_t)(t / np) * IB;
     const int64_t c = idx[p];
     const bool ok = c >= 0 && c < ngrid;
     E v[IB];
@@ -115,18 +117,18 @@ __device__ __forceinline__ void zero_acc(Acc<S, L> (&acc)[IB]) {
     for (int l = 0; l < L; ++l) acc[j].v[l] = S(0);
 }
 
-template <typename S, int L>
+template <typename S, int L, typename I>
 __global__ __launch_bounds__(256) void index_adjoint_kernel(
     const S* __restrict__ x, const int64_t* __restrict__ order,
     const int64_t* __restrict__ starts, S* __restrict__ out, int64_t nbatch,
     int64_t ngrid, int64_t npts, int64_t heavy_min) {
-  const int64_t nchunk = (nbatch + IB - 1) / IB;
-  const int64_t items = ngrid * nchunk;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < items;
-       t += stride) {
-    const int64_t c = t % ngrid;
-    const int64_t n0 = (t / ngrid) * IB;
+  const I nchunk = (I)((nbatch + IB - 1) / IB);
+  const I items = (I)ngrid * nchunk;
+  const I ng = (I)ngrid;
+  const I stride = (I)gridDim.x * blockDim.x;
+  for (I t = (I)blockIdx.x * blockDim.x + threadIdx.x; t < items; t += stride) {
+    const int64_t c = (int64_t)(t % ng);
+    const int64_t n0 = (int64_t)(t / ng) * IB;
     int64_t k0 = starts[c], k1 = starts[c + 1];
     if (heavy_min > 0 && k1 - k0 > heavy_min) continue;  // heavy kernel's
     k0 = k0 < 0 ? 0 : k0;
@@ -201,8 +203,12 @@ int launch_gather(const void* x, const int64_t* idx, void* out, int64_t nbatch,
                   int64_t ngrid, int64_t npts, hipStream_t s) {
   using E = typename Elem<EB>::T;
   const int64_t items = npts * ((nbatch + IB - 1) / IB);
-  index_gather_kernel<E><<<launch_blocks(items), 256, 0, s>>>(
-      (const E*)x, idx, (E*)out, nbatch, ngrid, npts);
+  if (items + 256LL * 64 * 256 < (1LL << 32))
+    index_gather_kernel<E, uint32_t><<<launch_blocks(items), 256, 0, s>>>(
+        (const E*)x, idx, (E*)out, nbatch, ngrid, npts);
+  else
+    index_gather_kernel<E, int64_t><<<launch_blocks(items), 256, 0, s>>>(
+        (const E*)x, idx, (E*)out, nbatch, ngrid, npts);
   TNL_LAUNCH_CHECK("index_gather");
   return TNL_OK;
 }
@@ -213,9 +219,13 @@ int launch_adjoint(const void* x, const int64_t* order, const int64_t* starts,
                    void* out, int64_t nbatch, int64_t ngrid, int64_t npts,
                    hipStream_t s) {
   const int64_t nchunk = (nbatch + IB - 1) / IB;
-  index_adjoint_kernel<S, L><<<launch_blocks(ngrid * nchunk), 256, 0, s>>>(
-      (const S*)x, order, starts, (S*)out, nbatch, ngrid, npts,
-      nheavy > 0 ? heavy_min : 0);
+  const int64_t hm = nheavy > 0 ? heavy_min : 0;
+  if (ngrid * nchunk + 256LL * 64 * 256 < (1LL << 32))
+    index_adjoint_kernel<S, L, uint32_t><<<launch_blocks(ngrid * nchunk), 256, 0, s>>>(
+        (const S*)x, order, starts, (S*)out, nbatch, ngrid, npts, hm);
+  else
+    index_adjoint_kernel<S, L, int64_t><<<launch_blocks(ngrid * nchunk), 256, 0, s>>>(
+        (const S*)x, order, starts, (S*)out, nbatch, ngrid, npts, hm);
   TNL_LAUNCH_CHECK("index_adjoint");
   if (nheavy > 0) {
     const int64_t nb = nheavy * nchunk;

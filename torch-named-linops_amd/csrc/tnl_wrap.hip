@@ -54,20 +54,23 @@ __device__ __forceinline__ uint2 scaled(uint2 v, float w) {
 }
 __device__ __forceinline__ uint4 scaled(uint4 v, float) { return v; }  // not launched
 
-template <typename E, bool W>
+// I: type of the row number and its div / mod (uint32_t when every row
+// number fits, avoiding the 64-bit division sequence per row)
+template <typename E, bool W, typename I>
 __global__ __launch_bounds__(256) void pad_wrapped_kernel(const E* __restrict__ x,
                                                           E* __restrict__ y,
                                                           const float* __restrict__ w,
                                                           WrapGeom g) {
-  const int64_t rows = g.nbatch * g.P[0] * g.P[1];
-  for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
-    const int64_t q1 = r % g.P[1];
-    const int64_t t = r / g.P[1];
-    const int64_t q0 = t % g.P[0];
-    const int64_t n = t / g.P[0];
+  const I rows = (I)(g.nbatch * g.P[0] * g.P[1]);
+  const I p0 = (I)g.P[0], p1 = (I)g.P[1];
+  for (I r = blockIdx.x; r < rows; r += gridDim.x) {
+    const int64_t q1 = (int64_t)(r % p1);
+    const I t = r / p1;
+    const int64_t q0 = (int64_t)(t % p0);
+    const int64_t n = (int64_t)(t / p0);
     const int64_t i0 = wrap_source(q0, g.N[0], g.P[0]);
     const int64_t i1 = wrap_source(q1, g.N[1], g.P[1]);
-    E* dst = y + r * g.P[2];
+    E* dst = y + (int64_t)r * g.P[2];
     const bool row_ok = i0 >= 0 && i1 >= 0;
     const int64_t img_row = ((row_ok ? i0 : 0) * g.N[1] + (row_ok ? i1 : 0)) * g.N[2];
     const E* src = x + n * g.N[0] * g.N[1] * g.N[2] + img_row;
@@ -85,21 +88,24 @@ __global__ __launch_bounds__(256) void pad_wrapped_kernel(const E* __restrict__ 
   }
 }
 
-template <typename E, bool W>
+template <typename E, bool W, typename I>
 __global__ __launch_bounds__(256) void crop_wrapped_kernel(const E* __restrict__ y,
                                                            E* __restrict__ x,
                                                            const float* __restrict__ w,
                                                            WrapGeom g) {
-  const int64_t rows = g.nbatch * g.N[0] * g.N[1];
-  for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
-    const int64_t i1 = r % g.N[1];
-    const int64_t t = r / g.N[1];
-    const int64_t i0 = t % g.N[0];
-    const int64_t n = t / g.N[0];
-    const int64_t q0 = ((i0 - g.N[0] / 2) % g.P[0] + g.P[0]) % g.P[0];
-    const int64_t q1 = ((i1 - g.N[1] / 2) % g.P[1] + g.P[1]) % g.P[1];
+  const I rows = (I)(g.nbatch * g.N[0] * g.N[1]);
+  const I n0 = (I)g.N[0], n1 = (I)g.N[1];
+  for (I r = blockIdx.x; r < rows; r += gridDim.x) {
+    const int64_t i1 = (int64_t)(r % n1);
+    const I t = r / n1;
+    const int64_t i0 = (int64_t)(t % n0);
+    const int64_t n = (int64_t)(t / n0);
+    // i - N/2 lies in [-N/2, N/2) and P >= N: one conditional wrap
+    int64_t q0 = i0 - g.N[0] / 2, q1 = i1 - g.N[1] / 2;
+    if (q0 < 0) q0 += g.P[0];
+    if (q1 < 0) q1 += g.P[1];
     const E* src = y + ((n * g.P[0] + q0) * g.P[1] + q1) * g.P[2];
-    E* dst = x + r * g.N[2];
+    E* dst = x + (int64_t)r * g.N[2];
     const float* wrow = W ? w + (i0 * g.N[1] + i1) * g.N[2] : nullptr;
     for (int64_t i2 = threadIdx.x; i2 < g.N[2]; i2 += blockDim.x) {
       int64_t q2 = i2 - g.N[2] / 2;
@@ -120,13 +126,23 @@ template <int EB, bool W>
 int launch_wrap(bool pad, const void* in, void* out, const float* w, const WrapGeom& g,
                 hipStream_t s) {
   using E = typename WElem<EB>::T;
+  const int64_t rows = pad ? g.nbatch * g.P[0] * g.P[1] : g.nbatch * g.N[0] * g.N[1];
+  const bool small = rows + 256LL * 64 < (1LL << 32);
   if (pad) {
-    pad_wrapped_kernel<E, W><<<row_blocks(g.nbatch * g.P[0] * g.P[1]), 256, 0, s>>>(
-        (const E*)in, (E*)out, w, g);
+    if (small)
+      pad_wrapped_kernel<E, W, uint32_t><<<row_blocks(rows), 256, 0, s>>>(
+          (const E*)in, (E*)out, w, g);
+    else
+      pad_wrapped_kernel<E, W, int64_t><<<row_blocks(rows), 256, 0, s>>>(
+          (const E*)in, (E*)out, w, g);
     TNL_LAUNCH_CHECK("pad_wrapped");
   } else {
-    crop_wrapped_kernel<E, W><<<row_blocks(g.nbatch * g.N[0] * g.N[1]), 256, 0, s>>>(
-        (const E*)in, (E*)out, w, g);
+    if (small)
+      crop_wrapped_kernel<E, W, uint32_t><<<row_blocks(rows), 256, 0, s>>>(
+          (const E*)in, (E*)out, w, g);
+    else
+      crop_wrapped_kernel<E, W, int64_t><<<row_blocks(rows), 256, 0, s>>>(
+          (const E*)in, (E*)out, w, g);
     TNL_LAUNCH_CHECK("crop_wrapped");
   }
   return TNL_OK;
