@@ -24,6 +24,7 @@ import torch
 from .._lib import SegLayout, check, i64_array, lib, stream_of
 from ..functional._batch import BatchedInterp
 from ..linops.nufft import NUFFT, shifted_layout
+from ..linops.toeplitz import crop_wrapped, pad_wrapped
 
 __all__ = ["BatchedNUFFTNormal", "SegmentedCG"]
 
@@ -48,7 +49,7 @@ class BatchedNUFFTNormal:
         self.ncoils = [int(c) for c in ncoils]
         self.padded = [tuple(int(s * oversamp) for s in im) for im in self.im_sizes]
         betas, plocs = [], []
-        self.apod, self.index = [], []
+        self.apod, self.apod_fwd, self.index = [], [], []
         for i in range(n):
             w = float(widths[i])
             beta = float(NUFFT.beta(w, oversamp))
@@ -62,11 +63,12 @@ class BatchedNUFFTNormal:
             plocs.append(torch.remainder(pl - shift.to(pl.device), pad).contiguous())
             apod = NUFFT.apodize_weights(self.im_sizes[i], self.padded[i], oversamp, w, beta)
             scale = w**self.ndim * (prod(self.im_sizes[i]) / prod(self.padded[i])) ** 0.5
-            # forward scale 1/scale and adjoint scale 1/scale fold into one factor
-            self.apod.append((apod / scale**2).to(self.device, torch.complex64))
-        self.apod_fwd = [NUFFT.apodize_weights(self.im_sizes[i], self.padded[i], oversamp,
-                                               float(widths[i]), betas[i]["beta"])
-                         .to(self.device, torch.complex64) for i in range(n)]
+            # forward scale 1/scale and adjoint scale 1/scale fold into the
+            # adjoint's weight, the orthonormal FFT scalings 1/sqrt(prod P)
+            # into both (the FFTs run unscaled)
+            rp = prod(self.padded[i]) ** 0.5
+            self.apod.append((apod / (scale**2 * rp)).to(self.device, torch.float32).contiguous())
+            self.apod_fwd.append((apod / rp).to(self.device, torch.float32).contiguous())
         self.interp = BatchedInterp(plocs, self.padded, widths, self.ncoils,
                                     kernel_params=betas, tuning=tuning)
         self.G = self.interp.new_grid()
@@ -89,14 +91,18 @@ class BatchedNUFFTNormal:
     def __call__(self, x: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
         out = self.new_image() if out is None else out
         gv = self.interp.grid_views(self.G)
-        for xi, gi, ap, ix in zip(self.image_views(x), gv, self.apod_fwd, self.index):
-            gi.zero_()
-            gi[(slice(None),) + ix] = xi * ap
-            gi.copy_(torch.fft.fftn(gi, dim=self.dim, norm="ortho"))
+        for xi, gi, ap, im, pad in zip(self.image_views(x), gv, self.apod_fwd,
+                                       self.im_sizes, self.padded):
+            # apodized image at the wrapped cells (zeros elsewhere), then an
+            # unscaled FFT straight into the problem's grid view
+            torch.fft.fftn(pad_wrapped(xi, im, pad, weight=ap), dim=self.dim,
+                           norm="backward", out=gi)
         self.interp.ungrid(self.G, out=self.P)
         self.interp.grid(self.P, out=self.G)
-        for oi, gi, ap, ix in zip(self.image_views(out), gv, self.apod, self.index):
-            oi.copy_(torch.fft.ifftn(gi, dim=self.dim, norm="ortho")[(slice(None),) + ix] * ap)
+        for oi, gi, ap, im, pad in zip(self.image_views(out), gv, self.apod,
+                                       self.im_sizes, self.padded):
+            oi.copy_(crop_wrapped(torch.fft.ifftn(gi, dim=self.dim, norm="forward"), im, pad,
+                                  weight=ap))
         return out
 
 

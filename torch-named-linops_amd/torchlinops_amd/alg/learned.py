@@ -38,6 +38,7 @@ from torch.autograd import Function
 from .._lib import check, lib, require_device, stream_of
 from ..functional._interp import interpolate, interpolate_adjoint
 from ..linops.nufft import NUFFT, shifted_layout
+from ..linops.toeplitz import crop_wrapped, pad_wrapped
 from .cg import DeviceCG, inner
 
 __all__ = ["SenseNormal", "PrecondNet", "conv3d", "to_channel_last", "pack_conv_weights",
@@ -198,12 +199,26 @@ class SenseNormal:
         scale = width**self.ndim * (math.prod(self.im_size) / math.prod(self.padded)) ** 0.5
         self.apod = apod.to(maps.device, torch.complex64)
         self.apod2 = (apod / scale**2).to(maps.device, torch.complex64)
+        # without autograd: apodization (and 1/scale^2) with the orthonormal
+        # FFT scalings folded in, applied by the wrapped pad / crop kernels
+        rp = math.prod(self.padded) ** 0.5
+        self.w_in = (apod / rp).to(maps.device, torch.float32).contiguous()
+        self.w_out = (apod / (scale**2 * rp)).to(maps.device, torch.float32).contiguous()
         self.maps = maps.to(torch.complex64)
         self.lam = float(lam)
         self.kp = {"beta": self.beta}
 
     def normal_no_reg(self, x: torch.Tensor) -> torch.Tensor:
         lead = x.shape[: x.ndim - self.ndim]
+        if not (torch.is_grad_enabled() and x.requires_grad):
+            sx = self.maps * x.unsqueeze(-self.ndim - 1)  # [..., C, *im]
+            g = pad_wrapped(sx, self.im_size, self.padded, weight=self.w_in)
+            g = torch.fft.fftn(g, dim=self.dim, norm="backward")  # unscaled
+            k = interpolate(g, self.locs, self.width, kernel_params=self.kp)
+            g = interpolate_adjoint(k, self.locs, self.padded, self.width, kernel_params=self.kp)
+            g = torch.fft.ifftn(g, dim=self.dim, norm="forward")  # unscaled
+            g = crop_wrapped(g, self.im_size, self.padded, weight=self.w_out)
+            return (self.maps.conj() * g).sum(-self.ndim - 1).reshape(*lead, *self.im_size)
         sx = self.maps * x.unsqueeze(-self.ndim - 1) * self.apod  # [..., C, *im]
         g = torch.zeros(*sx.shape[: sx.ndim - self.ndim], *self.padded, dtype=torch.complex64,
                         device=x.device)

@@ -91,7 +91,9 @@ def _fused_parts(locs, grid_size, oversamp, width, device):
     p = init_nufft(grid_size, locs, oversamp, width, device)
     shift = torch.tensor([n // 2 for n in p.padded_size], dtype=torch.float32, device=device)
     size = torch.tensor(p.padded_size, dtype=torch.float32, device=device)
-    w = (p.apodize.float() / p.scale_factor).contiguous()
+    # apodization, 1/scale and the orthonormal 1/sqrt(prod N_pad) in one
+    # weight: the FFTs below run unscaled
+    w = (p.apodize.float() / (p.scale_factor * prod(p.padded_size) ** 0.5)).contiguous()
     lw = torch.remainder(p.locs.to(device).float() - shift, size).contiguous()
     parts = (p, w, lw)
     with _apod_lock:
@@ -111,7 +113,7 @@ def nufft(x, locs, oversamp: float = 1.25, width: float = 4.0):
     if _fusable(x):
         p, w, lw = _fused_parts(locs, grid_size, oversamp, width, x.device)
         y = torch.fft.fftn(pad_wrapped(x, p.grid_size, p.padded_size, weight=w),
-                           dim=p.dim, norm="ortho")
+                           dim=p.dim, norm="backward")
         return interpolate(y, lw, width, kernel="kaiser_bessel",
                            kernel_params=dict(beta=p.beta))
     p = init_nufft(grid_size, locs, oversamp, width, x.device)
@@ -130,7 +132,7 @@ def nufft_adjoint(x, locs, grid_size, oversamp: float = 1.25, width: float = 4.0
         p, w, lw = _fused_parts(locs, grid_size, oversamp, width, x.device)
         g = interpolate_adjoint(x, lw, p.padded_size, width, kernel="kaiser_bessel",
                                 kernel_params=dict(beta=p.beta))
-        g = torch.fft.ifftn(g, dim=p.dim, norm="ortho")
+        g = torch.fft.ifftn(g, dim=p.dim, norm="forward")
         return crop_wrapped(g, p.grid_size, p.padded_size, weight=w)
     p = init_nufft(grid_size, locs, oversamp, width, x.device)
     x = interpolate_adjoint(x / p.scale_factor, p.locs, p.padded_size, width,

@@ -134,8 +134,10 @@ class NUFFT(Chain):
             shift = torch.tensor([n // 2 for n in padded_size], dtype=locs_p.dtype,
                                  device=locs_p.device)
             size = torch.tensor(padded_size, dtype=locs_p.dtype, device=locs_p.device)
-            self.register_buffer("_fused_weight", (weight.float() / scale_factor).contiguous(),
-                                 persistent=False)
+            # apodization, 1/scale and the orthonormal FFT's 1/sqrt(prod
+            # N_pad) in one real weight; the FFTs then run unscaled
+            fw = weight.float() / (scale_factor * prod(padded_size) ** 0.5)
+            self.register_buffer("_fused_weight", fw.contiguous(), persistent=False)
             self.register_buffer("_fused_locs",
                                  torch.remainder(locs_p.float() - shift, size).contiguous(),
                                  persistent=False)
@@ -158,10 +160,10 @@ class NUFFT(Chain):
         p = self.interp.interp_params
         if forward:
             y = pad_wrapped(x, grid, padded, weight=self._fused_weight)
-            y = torch.fft.fftn(y, dim=dims, norm="ortho")
+            y = torch.fft.fftn(y, dim=dims, norm="backward")  # unscaled
             return F.interpolate(y, self._fused_locs, **p)
         g = F.interpolate_adjoint(x, self._fused_locs, padded, **p)
-        g = torch.fft.ifftn(g, dim=dims, norm="ortho")
+        g = torch.fft.ifftn(g, dim=dims, norm="forward")  # unscaled
         return crop_wrapped(g, grid, padded, weight=self._fused_weight)
 
     @staticmethod

@@ -8,7 +8,8 @@ ifftshift(K * fftshift(v)) = ifftshift(K) * v, the same operator is
 
     crop_wrapped( ifftn( K_nat * fftn( pad_wrapped(x) ) ) )
 
-with K_nat = ifftshift(K) stored once, pad_wrapped = ifftshift . Pad and
+with K_nat = ifftshift(K) / prod(pad_size) stored once (the orthonormal
+scalings folded in, both FFTs unscaled), pad_wrapped = ifftshift . Pad and
 crop_wrapped = Crop . fftshift each one HIP copy (tnl_pad_wrapped /
 tnl_crop_wrapped), and the multiply done in place between the two rocFFT
 calls.  Five full passes of the padded grid (pad, two rolls, multiply, two
@@ -80,16 +81,19 @@ class ToeplitzNormal(NamedLinop):
         self.im_size = tuple(int(n) for n in im_size)
         self.pad_size = tuple(int(n) for n in pad_size)
         dims = tuple(range(-len(self.im_size), 0))
-        self.kernel = nn.Parameter(torch.fft.ifftshift(kernel, dim=dims).contiguous(),
-                                   requires_grad=False)
+        # the two orthonormal FFT scalings (1/sqrt(P) each) are folded into
+        # K so that both transforms run unscaled (no extra pass each)
+        self.kernel = nn.Parameter(
+            (torch.fft.ifftshift(kernel, dim=dims) / prod(self.pad_size)).contiguous(),
+            requires_grad=False)
 
     def _toeplitz(self, x, conj):
         nd = len(self.im_size)
         dims = tuple(range(-nd, 0))
         y = pad_wrapped(x, self.im_size, self.pad_size)
-        y = torch.fft.fftn(y, dim=dims, norm="ortho")
+        y = torch.fft.fftn(y, dim=dims, norm="backward")  # unscaled
         y.mul_(self.kernel.conj() if conj else self.kernel)
-        y = torch.fft.ifftn(y, dim=dims, norm="ortho")
+        y = torch.fft.ifftn(y, dim=dims, norm="forward")  # unscaled
         return crop_wrapped(y, self.im_size, self.pad_size)
 
     @staticmethod
