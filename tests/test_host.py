@@ -89,6 +89,37 @@ def test_library_validation_maps_to_exceptions():
                               _lib.i64_array([9]), _lib.i64_array([1]), None))
 
 
+def test_index_and_wrap_entry_points_validate_before_launch():
+    """The sampling and wrapped pad/crop entry points reject bad arguments
+    on the host (no pointer is touched, no kernel is launched)."""
+    from torchlinops_amd import _lib
+
+    h = _lib.lib()
+    with pytest.raises(ValueError, match="elem_bytes"):
+        _lib.check(h.tnl_index(1, 1, 1, 3, 1, 8, 4, None))
+    with pytest.raises(ValueError, match=">= 0"):
+        _lib.check(h.tnl_index(None, None, None, 8, -1, 8, 4, None))
+    with pytest.raises(ValueError, match="empty grid"):
+        _lib.check(h.tnl_index(1, 1, 1, 8, 1, 0, 4, None))
+    assert h.tnl_index(None, None, None, 8, 1, 8, 0, None) == 0  # no points: no work
+    with pytest.raises(ValueError, match="heavy"):
+        _lib.check(h.tnl_index_adjoint(1, 1, 1, None, 2, 64, 1, 0, 1, 1, 8, 4, None))
+    with pytest.raises(ValueError, match="heavy"):
+        _lib.check(h.tnl_index_adjoint(1, 1, 1, 1, 2, 0, 1, 0, 1, 1, 8, 4, None))
+    with pytest.raises(ValueError, match="dtype"):
+        _lib.check(h.tnl_index_adjoint(1, 1, 1, None, 0, 0, 1, 5, 1, 1, 8, 4, None))
+    im, pad = _lib.i64_array([8, 8]), _lib.i64_array([16, 6])
+    with pytest.raises(ValueError, match="im_size <= pad_size"):
+        _lib.check(h.tnl_pad_wrapped(1, 1, 8, 1, 2, im, pad, None, None))
+    ok = _lib.i64_array([16, 16])
+    with pytest.raises(ValueError, match="ndim"):
+        _lib.check(h.tnl_crop_wrapped(1, 1, 8, 1, 4, im, ok, None, None))
+    with pytest.raises(ValueError, match="weight"):
+        _lib.check(h.tnl_pad_wrapped(1, 1, 16, 1, 2, im, ok, 1, None))
+    with pytest.raises(ValueError, match="elem_bytes"):
+        _lib.check(h.tnl_crop_wrapped(1, 1, 2, 1, 2, im, ok, None, None))
+
+
 def test_functional_errors_before_device_work():
     from torchlinops_amd import functional as F
 
