@@ -293,3 +293,53 @@ def test_grid_ws_records_match_direct_reads(dev, cplx, nbatch):
     small = torch.empty(nbytes.value - 16, dtype=torch.uint8, device=dev)
     with pytest.raises(ValueError, match="scratch"):
         _lib.check(h.tnl_grid_ws(*args, small.data_ptr(), small.numel(), _lib.stream_of(yv)))
+
+
+def test_c2_full_size_properties(dev):
+    """The headline configuration at full size: 8 coils, 256^3 complex64,
+    20000 x 512 radial points, KB w=4.  The planned kernels (LDS-staged
+    interpolation, matrix-core gridding with point records) against the
+    plan-free kernels, adjointness in float64, linearity, and the oracle
+    on 4096 sampled points of the interpolation."""
+    import torchlinops_amd.functional as F
+    from torchlinops_amd import config
+
+    print("c2 full size: 8 x 256^3, 20000 x 512 radial", flush=True)
+    n, C, spokes, nread = 256, 8, 20000, 512
+    g = torch.Generator().manual_seed(21)
+    u = torch.randn(spokes, 3, generator=g)
+    u = u / u.norm(dim=1, keepdim=True)
+    r = -n / 2 + torch.arange(nread, dtype=torch.float32) * (n / nread)
+    locs = torch.remainder(r[None, :, None] * u[:, None, :] + n / 2, n).reshape(-1, 3)
+    locs = locs.to(dev).contiguous()
+    gd = torch.Generator(device=dev).manual_seed(22)
+    x = torch.randn(C, n, n, n, dtype=torch.complex64, device=dev, generator=gd)
+    y = torch.randn(C, locs.shape[0], dtype=torch.complex64, device=dev, generator=gd)
+    kp = {"beta": BETA}
+    with config.using(interp_plan="always"):
+        ax = F.interpolate(x, locs, 4.0, kernel_params=kp)
+        ahy = F.interpolate_adjoint(y, locs, (n, n, n), 4.0, kernel_params=kp)
+        ax2 = F.interpolate(2 * x, locs, 4.0, kernel_params=kp)
+    with config.using(interp_plan="never"):
+        ax_d = F.interpolate(x, locs, 4.0, kernel_params=kp)
+        ahy_d = F.interpolate_adjoint(y, locs, (n, n, n), 4.0, kernel_params=kp)
+
+    def rel(a, b):
+        return float(torch.linalg.vector_norm((a - b).to(torch.complex128))
+                     / torch.linalg.vector_norm(b.to(torch.complex128)))
+
+    e_u, e_g = rel(ax, ax_d), rel(ahy, ahy_d)
+    lhs = torch.vdot(y.reshape(-1).to(torch.complex128), ax.reshape(-1).to(torch.complex128))
+    rhs = torch.vdot(ahy.reshape(-1).to(torch.complex128), x.reshape(-1).to(torch.complex128))
+    e_adj = float(abs(lhs - rhs) / abs(lhs))
+    e_lin = rel(ax2, 2 * ax)
+    print(f"plan vs direct: ungrid {e_u:.2e} grid {e_g:.2e}; adjointness {e_adj:.2e}; "
+          f"linearity {e_lin:.2e}", flush=True)
+    assert e_u <= TOL and e_g <= TOL and e_adj <= TOL and e_lin <= TOL
+    idx = torch.randperm(locs.shape[0], generator=g)[:4096]
+    want = O.ungrid(x.cpu().numpy(), locs[idx.to(dev)].cpu().numpy(), 4.0, beta=BETA,
+                    precision="f64", nthreads=8)
+    got = ax[:, idx.to(dev)].cpu().numpy()
+    e_o = rel_l2(got, want)
+    print(f"oracle on 4096 sampled points: {e_o:.2e}", flush=True)
+    assert e_o <= TOL
