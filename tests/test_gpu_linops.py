@@ -259,3 +259,18 @@ def test_polynomial_preconditioned_cg(dev):
         assert cg.history[-1] <= 1e-5
         assert cg.iters < plain.iters
         assert rel_l2(x.cpu().numpy(), x0.cpu().numpy()) < 1e-3
+
+
+def test_fused_nufft_follows_trajectory_edits(dev):
+    rng = np.random.default_rng(9)
+    grid = (16, 16)
+    locs = torch.from_numpy((rng.uniform(-0.5, 0.5, (300, 2)) * 16).astype(np.float32)).to(dev)
+    A = tl.NUFFT(locs, grid, output_shape=("K",), oversamp=1.25)
+    x = torch.randn(grid, dtype=torch.complex64, device=dev)
+    A(x)
+    with torch.no_grad():
+        A.interp.locs.add_(0.37)  # edit the prepared trajectory in place
+    B = tl.NUFFT(locs, grid, output_shape=("K",), oversamp=1.25, fused=False)
+    with torch.no_grad():
+        B.interp.locs.add_(0.37)
+    assert rel_l2(A(x).cpu().numpy(), B(x).cpu().numpy()) < 1e-5

@@ -131,21 +131,30 @@ class NUFFT(Chain):
             # Apodize . Pad . centred FFT . Interp . Scalar as one weighted
             # wrapped pad, an uncentred FFT and interpolation at locations
             # relabelled by -N_pad/2 (exact under circular padding)
-            shift = torch.tensor([n // 2 for n in padded_size], dtype=locs_p.dtype,
-                                 device=locs_p.device)
-            size = torch.tensor(padded_size, dtype=locs_p.dtype, device=locs_p.device)
             # apodization, 1/scale and the orthonormal FFT's 1/sqrt(prod
             # N_pad) in one real weight; the FFTs then run unscaled
             fw = weight.float() / (scale_factor * prod(padded_size) ** 0.5)
             self.register_buffer("_fused_weight", fw.contiguous(), persistent=False)
-            self.register_buffer("_fused_locs",
-                                 torch.remainder(locs_p.float() - shift, size).contiguous(),
+            self.register_buffer("_fused_locs", self._relabel(self.interp.locs),
                                  persistent=False)
+            self._fused_src = self._locs_key()
             self._fused_on = True
         self._fused_dir = 1
 
     _fused_on = False
     _fused_dir = 1
+
+    def _relabel(self, locs):
+        """Prepared locations moved by -N_pad/2 (mod N_pad): the k-space grid
+        in natural FFT order instead of centred."""
+        padded = tuple(self.pad.pad_im_size)
+        shift = torch.tensor([n // 2 for n in padded], dtype=torch.float32, device=locs.device)
+        size = torch.tensor(padded, dtype=torch.float32, device=locs.device)
+        return torch.remainder(locs.detach().float() - shift, size).contiguous()
+
+    def _locs_key(self):
+        l = self.interp.locs
+        return (l.data_ptr(), l._version, str(l.device))
 
     def _fused_ok(self, x) -> bool:
         return (self._fused_on and x.is_cuda and x.dtype == torch.complex64
@@ -158,6 +167,9 @@ class NUFFT(Chain):
         grid, padded = self.grid_size, tuple(self.pad.pad_im_size)
         dims = tuple(range(-len(grid), 0))
         p = self.interp.interp_params
+        if self._locs_key() != self._fused_src:  # the trajectory was edited or moved
+            self._fused_locs = self._relabel(self.interp.locs)
+            self._fused_src = self._locs_key()
         if forward:
             y = pad_wrapped(x, grid, padded, weight=self._fused_weight)
             y = torch.fft.fftn(y, dim=dims, norm="backward")  # unscaled
