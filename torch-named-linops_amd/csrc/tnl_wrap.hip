@@ -88,6 +88,49 @@ __global__ __launch_bounds__(256) void pad_wrapped_kernel(const E* __restrict__ 
   }
 }
 
+// complex64 pad with 16-byte stores: a lane writes the cell pair
+// (2 j, 2 j + 1) of a row (P2 even); a workgroup pass covers
+// max(1, 256 / (P2 / 2)) whole rows, or one row in 256-pair steps.
+template <bool W, typename I>
+__global__ __launch_bounds__(256) void pad_wrapped_pairs_kernel(const uint2* __restrict__ x,
+                                                                uint4* __restrict__ y,
+                                                                const float* __restrict__ w,
+                                                                WrapGeom g) {
+  const I rows = (I)(g.nbatch * g.P[0] * g.P[1]);
+  const I p0 = (I)g.P[0], p1 = (I)g.P[1];
+  const int pairs = (int)(g.P[2] / 2);
+  const int per = pairs >= 256 ? 1 : 256 / pairs;  // rows per pass
+  const int lr = (int)threadIdx.x / (pairs >= 256 ? 256 : pairs);
+  const int lp = (int)threadIdx.x % (pairs >= 256 ? 256 : pairs);
+  if (lr >= per) return;  // lanes past the last whole row of a pass
+  for (I r = (I)blockIdx.x * per + lr; r < rows; r += (I)gridDim.x * per) {
+    const int64_t q1 = (int64_t)(r % p1);
+    const I t = r / p1;
+    const int64_t q0 = (int64_t)(t % p0);
+    const int64_t n = (int64_t)(t / p0);
+    const int64_t i0 = wrap_source(q0, g.N[0], g.P[0]);
+    const int64_t i1 = wrap_source(q1, g.N[1], g.P[1]);
+    const bool row_ok = i0 >= 0 && i1 >= 0;
+    const int64_t img_row = ((row_ok ? i0 : 0) * g.N[1] + (row_ok ? i1 : 0)) * g.N[2];
+    const uint2* src = x + n * g.N[0] * g.N[1] * g.N[2] + img_row;
+    uint4* dst = y + (int64_t)r * pairs;
+    for (int j = lp; j < pairs; j += 256) {
+      uint2 v[2] = {make_uint2(0u, 0u), make_uint2(0u, 0u)};
+      if (row_ok) {
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int64_t i2 = wrap_source(2 * j + e, g.N[2], g.P[2]);
+          if (i2 >= 0) {
+            v[e] = src[i2];
+            if (W) v[e] = scaled(v[e], w[img_row + i2]);
+          }
+        }
+      }
+      dst[j] = make_uint4(v[0].x, v[0].y, v[1].x, v[1].y);
+    }
+  }
+}
+
 template <typename E, bool W, typename I>
 __global__ __launch_bounds__(256) void crop_wrapped_kernel(const E* __restrict__ y,
                                                            E* __restrict__ x,
@@ -127,7 +170,22 @@ int launch_wrap(bool pad, const void* in, void* out, const float* w, const WrapG
                 hipStream_t s) {
   using E = typename WElem<EB>::T;
   const int64_t rows = pad ? g.nbatch * g.P[0] * g.P[1] : g.nbatch * g.N[0] * g.N[1];
-  const bool small = rows + 256LL * 64 < (1LL << 32);
+  // 32-bit row numbers when a grid-stride step (at most 16384 workgroups
+  // x 256 rows) past the last row still fits
+  const bool small = rows + 256LL * 64 * 256 < (1LL << 32);
+  if (pad && EB == 8 && g.P[2] % 2 == 0 && ((uintptr_t)out & 15) == 0) {
+    const int64_t pairs = g.P[2] / 2;
+    const int64_t per = pairs >= 256 ? 1 : 256 / pairs;
+    const unsigned nb = row_blocks((rows + per - 1) / per);
+    if (small)
+      pad_wrapped_pairs_kernel<W, uint32_t><<<nb, 256, 0, s>>>((const uint2*)in, (uint4*)out,
+                                                              w, g);
+    else
+      pad_wrapped_pairs_kernel<W, int64_t><<<nb, 256, 0, s>>>((const uint2*)in, (uint4*)out,
+                                                             w, g);
+    TNL_LAUNCH_CHECK("pad_wrapped");
+    return TNL_OK;
+  }
   if (pad) {
     if (small)
       pad_wrapped_kernel<E, W, uint32_t><<<row_blocks(rows), 256, 0, s>>>(
