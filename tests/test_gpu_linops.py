@@ -274,3 +274,20 @@ def test_fused_nufft_follows_trajectory_edits(dev):
     with torch.no_grad():
         B.interp.locs.add_(0.37)
     assert rel_l2(A(x).cpu().numpy(), B(x).cpu().numpy()) < 1e-5
+
+
+def test_nufft_built_on_cpu_then_moved(dev):
+    """The reference's device test (tests/test_nufft.py:262-276): a NUFFT
+    built on the CPU reports its device, moves with .to(), and then runs
+    on the GPU (fused path) matching the plain-chain operator."""
+    rng = np.random.default_rng(13)
+    grid = (20, 20)
+    locs = torch.from_numpy((rng.uniform(-0.5, 0.5, (500, 2)) * 20).astype(np.float32))
+    A = tl.NUFFT(locs.clone(), grid, output_shape=("K",), oversamp=1.25)
+    assert A.device.type == "cpu"
+    A.to(dev)
+    assert A.device.type == "cuda"
+    B = tl.NUFFT(locs.clone().to(dev), grid, output_shape=("K",), oversamp=1.25, fused=False)
+    x = torch.randn(grid, dtype=torch.complex64, device=dev)
+    assert rel_l2(A(x).cpu().numpy(), B(x).cpu().numpy()) < 1e-5
+    assert rel_l2(A.N(x).cpu().numpy(), B.N(x).cpu().numpy()) < 1e-5
