@@ -69,8 +69,22 @@ class BatchedNUFFTNormal:
             rp = prod(self.padded[i]) ** 0.5
             self.apod.append((apod / (scale**2 * rp)).to(self.device, torch.float32).contiguous())
             self.apod_fwd.append((apod / rp).to(self.device, torch.float32).contiguous())
-        self.interp = BatchedInterp(plocs, self.padded, widths, self.ncoils,
-                                    kernel_params=betas, tuning=tuning)
+        # The grids are laid out grouped by padded size (stable order), so
+        # that each group's FFT is one batched transform over its coils;
+        # images keep the caller's order.
+        order = sorted(range(n), key=lambda i: self.padded[i])
+        self.interp = BatchedInterp([plocs[i] for i in order], [self.padded[i] for i in order],
+                                    [widths[i] for i in order], [self.ncoils[i] for i in order],
+                                    kernel_params=[betas[i] for i in order], tuning=tuning)
+        self._slot = [0] * n  # problem -> position in the grid layout
+        for pos, i in enumerate(order):
+            self._slot[i] = pos
+        self._groups = []  # (first position, end position, padded size)
+        for pos, i in enumerate(order):
+            if self._groups and self._groups[-1][2] == self.padded[i]:
+                self._groups[-1][1] = pos + 1
+            else:
+                self._groups.append([pos, pos + 1, self.padded[i]])
         self.G = self.interp.new_grid()
         self.P = self.interp.new_points()
         self.im_offsets = []
@@ -88,21 +102,37 @@ class BatchedNUFFTNormal:
         return [flat[o:o + c * prod(im)].view(c, *im) for o, c, im in
                 zip(self.im_offsets, self.ncoils, self.im_sizes)]
 
+    def _group_views(self, flat: torch.Tensor):
+        """One [sum of coils, *padded] view of the grid buffer per size group."""
+        offs, nb, gs = self.interp.grid_offsets, self.interp.nbatches, self.interp.grid_sizes
+        views = []
+        for a, b, pad in self._groups:
+            end = offs[b - 1] + nb[b - 1] * prod(gs[b - 1])
+            views.append(flat[offs[a]:end].view(sum(nb[a:b]), *pad))
+        return views
+
     def __call__(self, x: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
         out = self.new_image() if out is None else out
         gv = self.interp.grid_views(self.G)
-        for xi, gi, ap, im, pad in zip(self.image_views(x), gv, self.apod_fwd,
-                                       self.im_sizes, self.padded):
-            # apodized image at the wrapped cells (zeros elsewhere), then an
-            # unscaled FFT straight into the problem's grid view
-            torch.fft.fftn(pad_wrapped(xi, im, pad, weight=ap), dim=self.dim,
-                           norm="backward", out=gi)
+        for i, (xi, ap, im, pad) in enumerate(zip(self.image_views(x), self.apod_fwd,
+                                                   self.im_sizes, self.padded)):
+            # apodized image at the wrapped cells (zeros elsewhere)
+            pad_wrapped(xi, im, pad, weight=ap, out=gv[self._slot[i]])
+        for g in self._group_views(self.G):  # unscaled FFTs, one per size group
+            g.copy_(torch.fft.fftn(g, dim=self.dim, norm="backward"))
         self.interp.ungrid(self.G, out=self.P)
         self.interp.grid(self.P, out=self.G)
-        for oi, gi, ap, im, pad in zip(self.image_views(out), gv, self.apod,
-                                       self.im_sizes, self.padded):
-            oi.copy_(crop_wrapped(torch.fft.ifftn(gi, dim=self.dim, norm="forward"), im, pad,
-                                  weight=ap))
+        spec = []
+        for (a, b, _), g in zip(self._groups, self._group_views(self.G)):
+            z = torch.fft.ifftn(g, dim=self.dim, norm="forward")
+            row = 0
+            for pos in range(a, b):
+                c = self.interp.nbatches[pos]
+                spec.append(z[row:row + c])
+                row += c
+        for i, (oi, ap, im, pad) in enumerate(zip(self.image_views(out), self.apod,
+                                                   self.im_sizes, self.padded)):
+            crop_wrapped(spec[self._slot[i]], im, pad, weight=ap, out=oi)
         return out
 
 

@@ -30,7 +30,8 @@ from .namedlinop import NamedLinop
 __all__ = ["pad_wrapped", "crop_wrapped", "ToeplitzNormal"]
 
 
-def _wrap(pad: bool, x: torch.Tensor, im_size, pad_size, weight=None) -> torch.Tensor:
+def _wrap(pad: bool, x: torch.Tensor, im_size, pad_size, weight=None,
+          out: torch.Tensor = None) -> torch.Tensor:
     require_device(x, op="pad_wrapped" if pad else "crop_wrapped")
     if weight is not None:
         if weight.dtype != torch.float32 or tuple(weight.shape) != tuple(im_size):
@@ -46,8 +47,13 @@ def _wrap(pad: bool, x: torch.Tensor, im_size, pad_size, weight=None) -> torch.T
                          f"{tuple(x.shape[x.ndim - nd:])}")
     batch = tuple(x.shape[: x.ndim - nd])
     x = x.contiguous()
-    out = torch.empty(batch + (tuple(pad_size) if pad else tuple(im_size)),
-                      dtype=x.dtype, device=x.device)
+    shape = batch + (tuple(pad_size) if pad else tuple(im_size))
+    if out is None:
+        out = torch.empty(shape, dtype=x.dtype, device=x.device)
+    elif (tuple(out.shape) != shape or out.dtype != x.dtype or out.device != x.device
+          or not out.is_contiguous()):
+        raise ValueError(f"out must be a contiguous {x.dtype} tensor of shape {shape} on "
+                         f"{x.device}")
     if out.numel() == 0:
         return out
     fn = lib().tnl_pad_wrapped if pad else lib().tnl_crop_wrapped
@@ -58,16 +64,17 @@ def _wrap(pad: bool, x: torch.Tensor, im_size, pad_size, weight=None) -> torch.T
     return out
 
 
-def pad_wrapped(x: torch.Tensor, im_size, pad_size, weight=None) -> torch.Tensor:
+def pad_wrapped(x: torch.Tensor, im_size, pad_size, weight=None, out=None) -> torch.Tensor:
     """``ifftshift(centred_pad(x * weight))`` over the last ``len(im_size)``
-    axes (``weight``: optional real float32 ``[*im_size]``)."""
-    return _wrap(True, x, im_size, pad_size, weight)
+    axes (``weight``: optional real float32 ``[*im_size]``; ``out``:
+    optional contiguous destination, fully written)."""
+    return _wrap(True, x, im_size, pad_size, weight, out)
 
 
-def crop_wrapped(y: torch.Tensor, im_size, pad_size, weight=None) -> torch.Tensor:
+def crop_wrapped(y: torch.Tensor, im_size, pad_size, weight=None, out=None) -> torch.Tensor:
     """``centred_crop(fftshift(y)) * weight`` over the last ``len(im_size)``
     axes."""
-    return _wrap(False, y, im_size, pad_size, weight)
+    return _wrap(False, y, im_size, pad_size, weight, out)
 
 
 class ToeplitzNormal(NamedLinop):
