@@ -23,6 +23,7 @@ from __future__ import annotations
 
 from dataclasses import dataclass
 from math import prod
+from typing import Optional
 
 import torch
 from torch import Tensor
@@ -125,16 +126,18 @@ class IndexMap:
     ``flat[p]`` is point p's cell in ``[0, ngrid)``; ``order`` lists the
     points sorted by cell (stable: ascending within a cell) and
     ``starts[c]:starts[c+1]`` is cell c's run in ``order``; ``heavy``
-    lists the cells with more than ``HEAVY_MIN`` points.  Built once per
-    index set; ``index``/``index_adjoint`` of the Sampling linop reuse it.
+    lists the cells with more than ``HEAVY_MIN`` points.  The grouping is
+    built on first use by the scatter-add (the gather needs only
+    ``flat``).  Built once per index set; ``index``/``index_adjoint`` of
+    the Sampling linop reuse it.
     """
 
     flat: Tensor
-    order: Tensor
-    starts: Tensor
-    heavy: Tensor
     out_shape: tuple
     grid_size: tuple
+    order: Optional[Tensor] = None
+    starts: Optional[Tensor] = None
+    heavy: Optional[Tensor] = None
 
     @property
     def ngrid(self) -> int:
@@ -143,6 +146,18 @@ class IndexMap:
     @property
     def npts(self) -> int:
         return self.flat.numel()
+
+    def grouped(self) -> "IndexMap":
+        """Build ``order`` / ``starts`` / ``heavy`` if not built yet."""
+        if self.order is None:
+            ngrid = self.ngrid
+            self.order = torch.sort(self.flat, stable=True).indices.contiguous()
+            counts = torch.bincount(self.flat, minlength=ngrid)
+            starts = torch.zeros(ngrid + 1, dtype=torch.int64, device=self.flat.device)
+            starts[1:] = torch.cumsum(counts, 0)
+            self.starts = starts
+            self.heavy = torch.nonzero(counts > HEAVY_MIN).reshape(-1).contiguous()
+        return self
 
 
 def index_map(idx: tuple, grid_size: tuple, device=None, flat: Tensor = None) -> IndexMap:
@@ -157,13 +172,7 @@ def index_map(idx: tuple, grid_size: tuple, device=None, flat: Tensor = None) ->
     else:
         out_shape = tuple(flat.shape)
         flat = flat.to(device=device, dtype=torch.int64).reshape(-1).contiguous()
-    ngrid = prod(grid_size)
-    order = torch.sort(flat, stable=True).indices.contiguous()
-    counts = torch.bincount(flat, minlength=ngrid)
-    starts = torch.zeros(ngrid + 1, dtype=torch.int64, device=flat.device)
-    starts[1:] = torch.cumsum(counts, 0)
-    heavy = torch.nonzero(counts > HEAVY_MIN).reshape(-1).contiguous()
-    return IndexMap(flat, order, starts, heavy, out_shape, grid_size)
+    return IndexMap(flat, out_shape, grid_size)
 
 
 # ---- HIP launches --------------------------------------------------------
@@ -213,6 +222,7 @@ def _scatter(vals: Tensor, imap: IndexMap) -> Tensor:
     nbatch = prod(batch)
     if out.numel() == 0:
         return out
+    imap.grouped()
     with torch.cuda.device(vals.device):
         nheavy = imap.heavy.numel()
         check(lib().tnl_index_adjoint(x.data_ptr(), imap.order.data_ptr(),
