@@ -159,7 +159,7 @@ def test_segmented_cg_matches_oracle_cg(dev):
         want = O.conjugate_gradients(An, ys[i].astype(np.complex128), max_num_iters=iters)
         err = rel_l2(A.image_views(x)[i].cpu().numpy(), want)
         print(f"problem {i}: rel-L2 vs float64 CG after {iters} iterations = {err:.2e}, "
-              f"relative residual {float(res[i]):.2e}")
+              f"relative residual {float(res[A.position[i]]):.2e}")
         # the float32 operator is within 1e-5 of float64 (checked above);
         # CG carries that error times the condition number (~20 here)
         assert err < 2e-3, i

@@ -69,36 +69,62 @@ class BatchedNUFFTNormal:
             rp = prod(self.padded[i]) ** 0.5
             self.apod.append((apod / (scale**2 * rp)).to(self.device, torch.float32).contiguous())
             self.apod_fwd.append((apod / rp).to(self.device, torch.float32).contiguous())
-        # The grids are laid out grouped by padded size (stable order), so
-        # that each group's FFT is one batched transform over its coils;
-        # images keep the caller's order.
+        # Grids and images are laid out grouped by padded size (stable within
+        # a group): each group's FFT is one batched transform over its coils,
+        # and consecutive problems with equal image size and width share one
+        # weighted pad / crop launch.  (Sorting by width as well made the
+        # persistent batched kernels slower: 285 against 271 ms for C3.)  ``image_views`` still returns
+        # the problems in the caller's order; ``seg_elems`` (and so
+        # SegmentedCG's per-segment results) follow the memory order
+        # ``order`` (``position[i]`` is problem i's segment).
         order = sorted(range(n), key=lambda i: self.padded[i])
+        self.order = order
+        self.position = [0] * n
+        for pos, i in enumerate(order):
+            self.position[i] = pos
         self.interp = BatchedInterp([plocs[i] for i in order], [self.padded[i] for i in order],
                                     [widths[i] for i in order], [self.ncoils[i] for i in order],
                                     kernel_params=[betas[i] for i in order], tuning=tuning)
-        self._slot = [0] * n  # problem -> position in the grid layout
-        for pos, i in enumerate(order):
-            self._slot[i] = pos
-        self._groups = []  # (first position, end position, padded size)
+        self.G = self.interp.new_grid()
+        self.P = self.interp.new_points()
+        self.im_offsets = [0] * n
+        off = 0
+        for i in order:
+            self.im_offsets[i] = off
+            off += self.ncoils[i] * prod(self.im_sizes[i])
+        self.numel = off
+        self.seg_elems = [self.ncoils[i] * prod(self.im_sizes[i]) for i in order]
+        # FFT groups: [first position, end position, padded size]
+        self._groups = []
         for pos, i in enumerate(order):
             if self._groups and self._groups[-1][2] == self.padded[i]:
                 self._groups[-1][1] = pos + 1
             else:
                 self._groups.append([pos, pos + 1, self.padded[i]])
-        self.G = self.interp.new_grid()
-        self.P = self.interp.new_points()
-        self.im_offsets = []
-        off = 0
-        for c, im in zip(self.ncoils, self.im_sizes):
-            self.im_offsets.append(off)
-            off += c * prod(im)
-        self.numel = off
-        self.seg_elems = [c * prod(im) for c, im in zip(self.ncoils, self.im_sizes)]
+        # weight groups: one pad / crop launch each
+        self._wgroups = []
+        for fg, (a, b, pad) in enumerate(self._groups):
+            row = 0
+            for pos in range(a, b):
+                i = order[pos]
+                key = (self.im_sizes[i], float(widths[i]))
+                c = self.ncoils[i]
+                if self._wgroups and self._wgroups[-1]["fg"] == fg and \
+                        self._wgroups[-1]["key"] == key:
+                    self._wgroups[-1]["coils"] += c
+                else:
+                    self._wgroups.append(dict(
+                        fg=fg, key=key, row=row, coils=c, im=self.im_sizes[i], pad=pad,
+                        xoff=self.im_offsets[i], goff=self.interp.grid_offsets[pos],
+                        wf=self.apod_fwd[i], wa=self.apod[i]))
+                row += c
 
     def new_image(self) -> torch.Tensor:
         return torch.empty(self.numel, dtype=torch.complex64, device=self.device)
 
     def image_views(self, flat: torch.Tensor):
+        """Per-problem [coils, *im] views of a flat image buffer, in the
+        caller's problem order."""
         return [flat[o:o + c * prod(im)].view(c, *im) for o, c, im in
                 zip(self.im_offsets, self.ncoils, self.im_sizes)]
 
@@ -113,26 +139,22 @@ class BatchedNUFFTNormal:
 
     def __call__(self, x: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
         out = self.new_image() if out is None else out
-        gv = self.interp.grid_views(self.G)
-        for i, (xi, ap, im, pad) in enumerate(zip(self.image_views(x), self.apod_fwd,
-                                                   self.im_sizes, self.padded)):
-            # apodized image at the wrapped cells (zeros elsewhere)
-            pad_wrapped(xi, im, pad, weight=ap, out=gv[self._slot[i]])
+        for w in self._wgroups:
+            # apodized images at the wrapped cells (zeros elsewhere)
+            c, im, pad = w["coils"], w["im"], w["pad"]
+            xg = x[w["xoff"]:w["xoff"] + c * prod(im)].view(c, *im)
+            gg = self.G[w["goff"]:w["goff"] + c * prod(pad)].view(c, *pad)
+            pad_wrapped(xg, im, pad, weight=w["wf"], out=gg)
         for g in self._group_views(self.G):  # unscaled FFTs, one per size group
             g.copy_(torch.fft.fftn(g, dim=self.dim, norm="backward"))
         self.interp.ungrid(self.G, out=self.P)
         self.interp.grid(self.P, out=self.G)
-        spec = []
-        for (a, b, _), g in zip(self._groups, self._group_views(self.G)):
-            z = torch.fft.ifftn(g, dim=self.dim, norm="forward")
-            row = 0
-            for pos in range(a, b):
-                c = self.interp.nbatches[pos]
-                spec.append(z[row:row + c])
-                row += c
-        for i, (oi, ap, im, pad) in enumerate(zip(self.image_views(out), self.apod,
-                                                   self.im_sizes, self.padded)):
-            crop_wrapped(spec[self._slot[i]], im, pad, weight=ap, out=oi)
+        spec = [torch.fft.ifftn(g, dim=self.dim, norm="forward")
+                for g in self._group_views(self.G)]
+        for w in self._wgroups:
+            c, im, pad = w["coils"], w["im"], w["pad"]
+            og = out[w["xoff"]:w["xoff"] + c * prod(im)].view(c, *im)
+            crop_wrapped(spec[w["fg"]][w["row"]:w["row"] + c], im, pad, weight=w["wa"], out=og)
         return out
 
 
