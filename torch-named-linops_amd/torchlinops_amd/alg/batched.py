@@ -69,15 +69,19 @@ class BatchedNUFFTNormal:
             rp = prod(self.padded[i]) ** 0.5
             self.apod.append((apod / (scale**2 * rp)).to(self.device, torch.float32).contiguous())
             self.apod_fwd.append((apod / rp).to(self.device, torch.float32).contiguous())
-        # Grids and images are laid out grouped by padded size (stable within
-        # a group): each group's FFT is one batched transform over its coils,
-        # and consecutive problems with equal image size and width share one
-        # weighted pad / crop launch.  (Sorting by width as well made the
-        # persistent batched kernels slower: 285 against 271 ms for C3.)  ``image_views`` still returns
+        # Grids and images are laid out grouped by padded size, largest grids
+        # first and, within a group, the most coil-points first (the
+        # persistent batched kernels then take the longest work first): each
+        # group's FFT is one batched transform over its coils, and
+        # consecutive problems with equal image size and width share one
+        # weighted pad / crop launch.  Measured on 64 uneven problems: this
+        # order 267 ms for 20 CG iterations, ascending size 271 ms, grouping
+        # by width inside a size 285 ms.  ``image_views`` still returns
         # the problems in the caller's order; ``seg_elems`` (and so
         # SegmentedCG's per-segment results) follow the memory order
         # ``order`` (``position[i]`` is problem i's segment).
-        order = sorted(range(n), key=lambda i: self.padded[i])
+        work = [self.ncoils[i] * locs[i].numel() for i in range(n)]
+        order = sorted(range(n), key=lambda i: (-prod(self.padded[i]), -work[i]))
         self.order = order
         self.position = [0] * n
         for pos, i in enumerate(order):
