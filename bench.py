@@ -316,7 +316,7 @@ def c5_learned_cg(dev, seed: int = 5, train_n: int = 32, train_problems: int = 6
     bump coils, 8000 x 256 radial samples, 256^3 grid, lam = 1e-3 (relative
     to the largest eigenvalue); wall time to 1e-6 relative residual for
     plain CG and for CG preconditioned by the trained network (network
-    inference included), both judged by the plain residual."""
+    inference included), both checked by the plain residual."""
     import numpy as np
     import torch
 
@@ -455,7 +455,7 @@ def main() -> int:
                          "(1 GiB read + 1 GiB written) to calibrate HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        print(json.dumps({"error": "bench.py measures one GPU only; run with --gpus 1"}))
         return 2
 
     import numpy as np

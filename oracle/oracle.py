@@ -24,7 +24,7 @@ restatements of the NUFFT chain and of CG:
   dft / dft_adjoint       brute-force nonuniform DFT (the tutorial's exact
                           operator, tutorials/mri_reconstruction.py:224-258)
 
-Parity status per function is recorded in DESIGN.md; every function here is
+Every function here is
 pinned by a test in tests/test_oracle.py against a closed form, an
 independent library routine (scipy / numpy) or brute force.
 """

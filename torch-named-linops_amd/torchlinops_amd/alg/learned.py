@@ -20,7 +20,7 @@ kernels' autograd pair supplies the operator's backward).  This targets
 convergence directly; a constant d (plain CG) lies inside the family.
 
 ``solve`` runs preconditioned CG (``DeviceCG``) and keeps plain CG as the
-judge: the residual is recomputed with A itself, and if the preconditioned
+check: the residual is recomputed with A itself, and if the preconditioned
 run stalls or breaks down, plain CG continues from its iterate.
 No counterpart in the reference, whose CG (pcg.py) is unpreconditioned.
 """
@@ -332,12 +332,12 @@ def train_preconditioner(net: PrecondNet, data, steps: int = 300, lr: float = 3e
 
 def solve(A: SenseNormal, y: torch.Tensor, d: Optional[torch.Tensor] = None,
           tol: float = 1e-6, max_iters: int = 1000, check_every: int = 10,
-          judge_factor: float = 100.0):
+          check_factor: float = 100.0):
     """(Preconditioned) CG until its relative residual |r| / |y| is below
-    ``tol``.  Plain CG judges the result: the residual is recomputed as
+    ``tol``.  Plain CG checks the result: the residual is recomputed as
     |y - A x| / |y| with the operator itself, and if the preconditioned run
     did not converge, broke down (non-finite values) or its recomputed
-    residual exceeds ``judge_factor * tol``, plain CG continues from its
+    residual exceeds ``check_factor * tol``, plain CG continues from its
     iterate (or from zero).  Returns (x, info)."""
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -351,7 +351,7 @@ def solve(A: SenseNormal, y: torch.Tensor, d: Optional[torch.Tensor] = None,
     true_res = float(torch.linalg.vector_norm(y - A(x)) / ynorm)
     fallback = False
     if d is not None and not (converged and math.isfinite(true_res)
-                              and true_res <= judge_factor * tol):
+                              and true_res <= check_factor * tol):
         fallback = True
         x0 = x if math.isfinite(true_res) else None
         cg2 = DeviceCG(A, lam=0.0, M=None, check_every=check_every)
