@@ -99,6 +99,8 @@ typedef struct tnl_plan_layout {
   int64_t off_suboff, off_sub;
   int64_t off_gcounts, off_gstarts, off_gcursor, off_tlist;
   int64_t off_itemoff, off_items;
+  int64_t off_items_tmp; /* staging for the item ordering during the build
+                          * (3-D plans list their work items largest first)  */
   int64_t bytes;         /* total workspace bytes                             */
 } tnl_plan_layout;
 

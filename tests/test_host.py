@@ -39,7 +39,7 @@ def test_plan_layout_is_host_only_and_consistent():
     assert L.touch_capacity == 8 * 10_240_000
     offs = [L.off_counts, L.off_starts, L.off_cursor, L.off_perm, L.off_slocs,
             L.off_suboff, L.off_sub, L.off_gcounts, L.off_gstarts, L.off_gcursor,
-            L.off_tlist, L.off_itemoff, L.off_items]
+            L.off_tlist, L.off_itemoff, L.off_items, L.off_items_tmp]
     assert tuple(L.grid_size) == (256, 256, 256) and L.lds_budget == 53248
     assert all(o % 256 == 0 for o in offs)
     assert offs == sorted(offs) and L.bytes > offs[-1]

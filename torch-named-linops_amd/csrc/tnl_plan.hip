@@ -145,6 +145,7 @@ int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
   L->off_tlist = take(4 * std::max<int64_t>(L->touch_capacity, 1));
   L->off_itemoff = take(4 * (L->g_ntiles_total + 1));
   L->off_items = take(16 * L->item_capacity);
+  L->off_items_tmp = take(16 * L->item_capacity);
   L->bytes = off;
   return TNL_OK;
 }
@@ -366,6 +367,77 @@ __global__ void item_fill_kernel(const int* __restrict__ gcounts,
                                k > 1 ? 1 : 0);
     }
   }
+}
+
+// Gridding work items, largest first: a stable counting sort of the items by
+// touch-count class (count / 512, at most 15), classes in descending order,
+// the tile order kept inside a class.  The persistent gridding kernels walk
+// the items in this order, so the dense tiles (the centre of a radial
+// trajectory) start first instead of last; a finer, full sort by count
+// scrambles the tile order of uniform trajectories for no gain.  3-D plans
+// only: the batched 2-D problems ran slower with it.  One workgroup: per
+// chunk of 1024 items, a ballot per class gives each item its rank among
+// the chunk's items of its class.
+constexpr int ITEM_CLASSES = 16;
+
+__device__ __forceinline__ int item_class(int count) {
+  const int c = count / 512;
+  return c < ITEM_CLASSES - 1 ? c : ITEM_CLASSES - 1;
+}
+
+__global__ __launch_bounds__(1024) void item_order_kernel(const int4* __restrict__ items,
+                                                          const int* __restrict__ nitems_ptr,
+                                                          int4* __restrict__ out) {
+  __shared__ int base[ITEM_CLASSES];
+  __shared__ int wtot[1024 / 64][ITEM_CLASSES];
+  const int n = *nitems_ptr;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid < ITEM_CLASSES) base[tid] = 0;
+  __syncthreads();
+  for (int i = tid; i < n; i += 1024) atomicAdd(&base[item_class(items[i].z)], 1);
+  __syncthreads();
+  if (tid == 0) {  // exclusive offsets, largest class first
+    int acc = 0;
+    for (int k = ITEM_CLASSES - 1; k >= 0; --k) {
+      const int h = base[k];
+      base[k] = acc;
+      acc += h;
+    }
+  }
+  __syncthreads();
+  const uint64_t below = (1ull << lane) - 1ull;
+  for (int c0 = 0; c0 < n; c0 += 1024) {
+    const int i = c0 + tid;
+    const bool valid = i < n;
+    const int4 it = valid ? items[i] : make_int4(0, 0, 0, 0);
+    const int k = valid ? item_class(it.z) : -1;
+    int rank = 0;
+    for (int b = 0; b < ITEM_CLASSES; ++b) {
+      const uint64_t m = __ballot(k == b);
+      if (lane == 0) wtot[wave][b] = __popcll(m);
+      if (k == b) rank = __popcll(m & below);
+    }
+    __syncthreads();
+    if (valid) {
+      int pos = base[k] + rank;
+      for (int w = 0; w < wave; ++w) pos += wtot[w][k];
+      out[pos] = it;
+    }
+    __syncthreads();
+    if (tid < ITEM_CLASSES) {
+      int t = 0;
+      for (int w = 0; w < 1024 / 64; ++w) t += wtot[w][tid];
+      base[tid] += t;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void item_copy_kernel(const int4* __restrict__ src, const int* __restrict__ nitems_ptr,
+                                 int4* __restrict__ dst) {
+  const int n = *nitems_ptr;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 // Interpolation gathers with lane = point: in one LDS instruction every lane
@@ -681,6 +753,14 @@ extern "C" int tnl_plan_build(const float* locs, const tnl_plan_layout* layout,
     item_sort_kernel<<<(unsigned)nwg, 256, 0, s>>>(items, itemoff + L.g_ntiles_total, slocs,
                                                     perm, P, tlist);
     TNL_LAUNCH_CHECK("plan item sort");
+  }
+  if (npts > 0 && L.ndim == 3) {
+    int4* tmp = (int4*)(ws + L.off_items_tmp);
+    item_order_kernel<<<1, 1024, 0, s>>>(items, itemoff + L.g_ntiles_total, tmp);
+    const int64_t nwg = (L.item_capacity + 255) / 256;
+    item_copy_kernel<<<(unsigned)(nwg < 4096 ? (nwg < 1 ? 1 : nwg) : 4096), 256, 0, s>>>(
+        tmp, itemoff + L.g_ntiles_total, items);
+    TNL_LAUNCH_CHECK("plan item order");
   }
   return TNL_OK;
 }

@@ -98,6 +98,7 @@ class PlanLayout(ctypes.Structure):
         ("off_tlist", ctypes.c_int64),
         ("off_itemoff", ctypes.c_int64),
         ("off_items", ctypes.c_int64),
+        ("off_items_tmp", ctypes.c_int64),
         ("bytes", ctypes.c_int64),
     ]
 
