@@ -927,34 +927,48 @@ __device__ __forceinline__ unsigned mf_axis_place(float t, int d, int origin,
 // Point-major record of one point and channel chunk for the matrix-core
 // gridding kernel: the chunk's 16 planes (re/im of 8 coils, or 16 real
 // channels; zero past nbatch), 64 bytes, in the caller's point order (a
-// plain transpose of the coil-major input: both sides coalesced).
+// plain transpose of the coil-major input).  A workgroup moves a tile of
+// 256 points through LDS: each channel row is read with consecutive
+// lanes on consecutive points, and the tile's 16 KB of records is written
+// with consecutive lanes on consecutive 16-byte words.
 constexpr int MF_REC4 = 4;  // float4 per record
+constexpr int PACK_PTS = 256;
+constexpr int PACK_STR = MF_NPL + 1;  // LDS floats per point (odd: no bank conflicts)
 
 template <bool CPLX>
-__global__ void pack_points_kernel(const float* __restrict__ vals, int64_t npts,
-                                   int64_t nbatch, float4* __restrict__ rec) {
+__global__ __launch_bounds__(256) void pack_points_kernel(const float* __restrict__ vals,
+                                                          int64_t npts, int64_t nbatch,
+                                                          float4* __restrict__ rec) {
   constexpr int CH = CPLX ? MF_NPL / 2 : MF_NPL;
+  __shared__ float tile[PACK_PTS * PACK_STR];
   const int64_t nch = (nbatch + CH - 1) / CH;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npts;
-       p += (int64_t)gridDim.x * blockDim.x) {
-    for (int64_t ch = 0; ch < nch; ++ch) {
-      float f[MF_NPL];
+  const int64_t ntile = (npts + PACK_PTS - 1) / PACK_PTS;
+  const int tid = threadIdx.x;
+  for (int64_t u = blockIdx.x; u < ntile * nch; u += gridDim.x) {
+    const int64_t ch = u / ntile;
+    const int64_t p0 = (u - ch * ntile) * PACK_PTS;
+    const int np = (int)(npts - p0 < PACK_PTS ? npts - p0 : PACK_PTS);
+    __syncthreads();  // previous tile's records are written out
+    if (tid < np) {
+      const int64_t p = p0 + tid;
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         const int64_t cg = ch * CH + c;
         if constexpr (CPLX) {
           const float2 x = cg < nbatch ? reinterpret_cast<const float2*>(vals)[cg * npts + p]
                                        : make_float2(0.f, 0.f);
-          f[2 * c] = x.x;
-          f[2 * c + 1] = x.y;
+          tile[tid * PACK_STR + 2 * c] = x.x;
+          tile[tid * PACK_STR + 2 * c + 1] = x.y;
         } else {
-          f[c] = cg < nbatch ? vals[cg * npts + p] : 0.f;
+          tile[tid * PACK_STR + c] = cg < nbatch ? vals[cg * npts + p] : 0.f;
         }
       }
-      float4* r = rec + (ch * npts + p) * MF_REC4;
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        r[i] = make_float4(f[4 * i], f[4 * i + 1], f[4 * i + 2], f[4 * i + 3]);
+    }
+    __syncthreads();
+    float4* r = rec + (ch * npts + p0) * MF_REC4;
+    for (int j = tid; j < np * MF_REC4; j += 256) {
+      const float* t = tile + (j >> 2) * PACK_STR + 4 * (j & 3);
+      r[j] = make_float4(t[0], t[1], t[2], t[3]);
     }
   }
 }
@@ -1473,7 +1487,8 @@ static int launch_grid_mfma(const float* vals, const float* locs, float* out, in
   if (int rc = zero_split_tiles<CPLX>(out, nbatch, G, B, L, ws, s)) return rc;
   float4* rec = reinterpret_cast<float4*>(scratch);
   if (rec) {
-    const int64_t nb = (L.npts + 255) / 256;
+    const int64_t ch = CPLX ? MF_NPL / 2 : MF_NPL;
+    const int64_t nb = (L.npts + PACK_PTS - 1) / PACK_PTS * ((nbatch + ch - 1) / ch);
     pack_points_kernel<CPLX><<<(unsigned)(nb < 16384 ? nb : 16384), 256, 0, s>>>(
         vals, L.npts, nbatch, rec);
     TNL_LAUNCH_CHECK("pack_points");
