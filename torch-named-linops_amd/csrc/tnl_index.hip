@@ -21,6 +21,7 @@
 //          there and summed by a workgroup each: 256 contiguous sequential
 //          partial sums combined by a fixed LDS tree, so they stay
 //          deterministic but round differently from the sequential sum.
+//          Both roles share one launch (heavy workgroups first).
 #include "tnl_common.hpp"
 #include "tnl_internal.hpp"
 
@@ -117,20 +118,22 @@ __device__ __forceinline__ void zero_acc(Acc<S, L> (&acc)[IB]) {
     for (int l = 0; l < L; ++l) acc[j].v[l] = S(0);
 }
 
+// Light cells: a lane per (cell, batch chunk), grid-stride over the
+// workgroups vb of vg given to this role.
 template <typename S, int L, typename I>
-__global__ __launch_bounds__(256) void index_adjoint_kernel(
+__device__ __forceinline__ void adjoint_light(
     const S* __restrict__ x, const int64_t* __restrict__ order,
     const int64_t* __restrict__ starts, S* __restrict__ out, int64_t nbatch,
-    int64_t ngrid, int64_t npts, int64_t heavy_min) {
+    int64_t ngrid, int64_t npts, int64_t heavy_min, I vb, I vg) {
   const I nchunk = (I)((nbatch + IB - 1) / IB);
   const I items = (I)ngrid * nchunk;
   const I ng = (I)ngrid;
-  const I stride = (I)gridDim.x * blockDim.x;
-  for (I t = (I)blockIdx.x * blockDim.x + threadIdx.x; t < items; t += stride) {
+  const I stride = vg * (I)blockDim.x;
+  for (I t = vb * (I)blockDim.x + threadIdx.x; t < items; t += stride) {
     const int64_t c = (int64_t)(t % ng);
     const int64_t n0 = (int64_t)(t / ng) * IB;
     int64_t k0 = starts[c], k1 = starts[c + 1];
-    if (heavy_min > 0 && k1 - k0 > heavy_min) continue;  // heavy kernel's
+    if (heavy_min > 0 && k1 - k0 > heavy_min) continue;  // a heavy cell
     k0 = k0 < 0 ? 0 : k0;
     k1 = k1 > npts ? npts : k1;
     Acc<S, L> acc[IB];
@@ -146,19 +149,19 @@ __global__ __launch_bounds__(256) void index_adjoint_kernel(
   }
 }
 
-// One workgroup per (heavy cell, batch chunk): thread i sums the i-th of
-// 256 contiguous slices of the cell's run in order, then a fixed pairwise
-// LDS tree adds the 256 partial sums.
+// Heavy cells: one workgroup per (heavy cell, batch chunk), over the
+// workgroups vb of vg given to this role: thread i sums the i-th of 256
+// contiguous slices of the cell's run in order, then a fixed pairwise LDS
+// tree adds the 256 partial sums.
 template <typename S, int L>
-__global__ __launch_bounds__(256) void index_adjoint_heavy_kernel(
+__device__ __forceinline__ void adjoint_heavy(
     const S* __restrict__ x, const int64_t* __restrict__ order,
     const int64_t* __restrict__ starts, const int64_t* __restrict__ heavy,
     int64_t nheavy, S* __restrict__ out, int64_t nbatch, int64_t ngrid,
-    int64_t npts) {
-  __shared__ S red[256][IB * L];
+    int64_t npts, int64_t vb, int64_t vg, S (*red)[IB * L]) {
   const int tid = threadIdx.x;
   const int64_t nchunk = (nbatch + IB - 1) / IB;
-  for (int64_t b = blockIdx.x; b < nheavy * nchunk; b += gridDim.x) {
+  for (int64_t b = vb; b < nheavy * nchunk; b += vg) {
     const int64_t c = heavy[b % nheavy];
     const int64_t n0 = (b / nheavy) * IB;
     const bool ok = c >= 0 && c < ngrid;
@@ -191,6 +194,24 @@ __global__ __launch_bounds__(256) void index_adjoint_heavy_kernel(
   }
 }
 
+// One launch for both roles: the first hb workgroups take the heavy cells
+// (so they start first), the rest the light ones; the two write disjoint
+// cells and run side by side.
+template <typename S, int L, typename I>
+__global__ __launch_bounds__(256) void index_adjoint_kernel(
+    const S* __restrict__ x, const int64_t* __restrict__ order,
+    const int64_t* __restrict__ starts, const int64_t* __restrict__ heavy,
+    int64_t nheavy, S* __restrict__ out, int64_t nbatch, int64_t ngrid,
+    int64_t npts, int64_t heavy_min, unsigned hb) {
+  __shared__ S red[256][IB * L];
+  if (blockIdx.x < hb)
+    adjoint_heavy<S, L>(x, order, starts, heavy, nheavy, out, nbatch, ngrid, npts,
+                        (int64_t)blockIdx.x, (int64_t)hb, red);
+  else
+    adjoint_light<S, L, I>(x, order, starts, out, nbatch, ngrid, npts, heavy_min,
+                           (I)(blockIdx.x - hb), (I)(gridDim.x - hb));
+}
+
 unsigned launch_blocks(int64_t items) {
   // grid-stride: enough workgroups to fill 256 CUs several times over
   const int64_t want = (items + 255) / 256;
@@ -220,19 +241,16 @@ int launch_adjoint(const void* x, const int64_t* order, const int64_t* starts,
                    hipStream_t s) {
   const int64_t nchunk = (nbatch + IB - 1) / IB;
   const int64_t hm = nheavy > 0 ? heavy_min : 0;
+  const int64_t nh = nheavy * nchunk;
+  const unsigned hb = (unsigned)(nh < 65536 ? nh : 65536);  // heavy-role workgroups
+  const unsigned grid = hb + launch_blocks(ngrid * nchunk);
   if (ngrid * nchunk + 256LL * 64 * 256 < (1LL << 32))
-    index_adjoint_kernel<S, L, uint32_t><<<launch_blocks(ngrid * nchunk), 256, 0, s>>>(
-        (const S*)x, order, starts, (S*)out, nbatch, ngrid, npts, hm);
+    index_adjoint_kernel<S, L, uint32_t><<<grid, 256, 0, s>>>(
+        (const S*)x, order, starts, heavy, nheavy, (S*)out, nbatch, ngrid, npts, hm, hb);
   else
-    index_adjoint_kernel<S, L, int64_t><<<launch_blocks(ngrid * nchunk), 256, 0, s>>>(
-        (const S*)x, order, starts, (S*)out, nbatch, ngrid, npts, hm);
+    index_adjoint_kernel<S, L, int64_t><<<grid, 256, 0, s>>>(
+        (const S*)x, order, starts, heavy, nheavy, (S*)out, nbatch, ngrid, npts, hm, hb);
   TNL_LAUNCH_CHECK("index_adjoint");
-  if (nheavy > 0) {
-    const int64_t nb = nheavy * nchunk;
-    index_adjoint_heavy_kernel<S, L><<<(unsigned)(nb < 65536 ? nb : 65536), 256, 0, s>>>(
-        (const S*)x, order, starts, heavy, nheavy, (S*)out, nbatch, ngrid, npts);
-    TNL_LAUNCH_CHECK("index_adjoint_heavy");
-  }
   return TNL_OK;
 }
 
