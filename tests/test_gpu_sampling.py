@@ -236,3 +236,34 @@ def test_sampling_round_trip_full_size(dev):
     assert torch.equal(A(back), y)
     # checksum of the gather against a torch advanced-indexing read
     assert torch.equal(y, x.reshape(C, -1)[:, cells])
+
+
+def test_sampling_edge_cases(dev):
+    """Empty batch, empty point set (the scatter-add then writes zeros),
+    a single cell, and a batch that is not a multiple of the 4 entries a
+    lane handles."""
+    g = torch.Generator().manual_seed(30)
+    x0 = torch.randn(0, 6, 5, dtype=torch.complex64, generator=g).to(dev)
+    idx = (torch.tensor([1, 2], device=dev), torch.tensor([3, 4], device=dev))
+    assert F.index(x0, idx).shape == (0, 2)
+    assert F.index_adjoint(torch.zeros(0, 2, dtype=torch.complex64, device=dev), idx,
+                           (6, 5)).shape == (0, 6, 5)
+    empty = (torch.zeros(0, dtype=torch.long, device=dev),) * 2
+    z = F.index_adjoint(torch.zeros(3, 0, dtype=torch.complex64, device=dev), empty, (6, 5))
+    assert z.shape == (3, 6, 5) and not z.abs().any()
+    one = torch.randn(7, 1, 1, generator=g).to(dev)
+    ii = (torch.zeros(9, dtype=torch.long, device=dev),) * 2
+    got = F.index(one, ii)
+    assert torch.equal(got, one.reshape(7, 1).expand(7, 9))
+    back = F.index_adjoint(got, ii, (1, 1))
+    want = O.index_adjoint(got.cpu().numpy(), tuple(i.cpu().numpy() for i in ii), (1, 1))
+    assert np.array_equal(back.cpu().numpy(), want)
+
+
+def test_wrapped_pad_crop_empty_batch(dev):
+    from torchlinops_amd.linops.toeplitz import crop_wrapped, pad_wrapped
+
+    x = torch.zeros(0, 4, 4, dtype=torch.complex64, device=dev)
+    assert pad_wrapped(x, (4, 4), (8, 8)).shape == (0, 8, 8)
+    y = torch.zeros(0, 8, 8, dtype=torch.complex64, device=dev)
+    assert crop_wrapped(y, (4, 4), (8, 8)).shape == (0, 4, 4)
