@@ -123,6 +123,32 @@ def sampling_leg(dev, locs, C: int, n: int, steps: int = 10, oversamp: float = 1
     b_g = C * M * 16 + M * 8
     b_a = C * M * 8 + M * 8 + (G + 1) * 8 + C * G * 8
     distinct = int(torch.unique(imap.flat).numel())
+    del grid, pts
+    # the whole sampling-mode NUFFT linop (256^3 image -> 320^3 k-space),
+    # fused against the reference-shaped Pad / centred FFT / Sampling chain
+    import torchlinops_amd as tl
+
+    img = torch.randn(C, n, n, n, dtype=torch.complex64, device=dev, generator=gen)
+    ksp = torch.randn(C, M, dtype=torch.complex64, device=dev, generator=gen)
+    lin = {}
+    for label, fused in (("fused", True), ("chain", False)):
+        A = tl.NUFFT(locs - n / 2, (n, n, n), output_shape=("K",), batch_shape=("C",),
+                     oversamp=oversamp, mode="sampling", fused=fused)
+        AH = A.H
+        A(img)
+        AH(ksp)
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        torch.cuda.synchronize()
+        e[0].record()
+        for _ in range(3):
+            A(img)
+        e[1].record()
+        for _ in range(3):
+            AH(ksp)
+        e[2].record()
+        torch.cuda.synchronize()
+        lin[label] = (e[0].elapsed_time(e[1]) / 3, e[1].elapsed_time(e[2]) / 3)
+        del A, AH
     return {
         "workload": f"sampling NUFFT gather/scatter-add, {C} coils, {npad}^3 padded grid, "
                     f"{M} radial points rounded to cells ({distinct} distinct)",
@@ -134,6 +160,8 @@ def sampling_leg(dev, locs, C: int, n: int, steps: int = 10, oversamp: float = 1
         "adjoint_frac": b_a / t_a / HBM_PEAK,
         "gather_bytes": b_g,
         "adjoint_bytes": b_a,
+        "nufft_sampling_fused_ms": list(lin["fused"]),
+        "nufft_sampling_chain_ms": list(lin["chain"]),
     }
 
 
@@ -592,7 +620,9 @@ def main() -> int:
         log("sampling: gather / scatter-add ...")
         out["sampling"] = sampling_leg(dev, locs, C, n)
         log(f"sampling: gather {out['sampling']['gather_us']:.1f} us, "
-            f"adjoint {out['sampling']['adjoint_us']:.1f} us")
+            f"adjoint {out['sampling']['adjoint_us']:.1f} us; NUFFT linop forward/adjoint "
+            f"fused {out['sampling']['nufft_sampling_fused_ms']} ms, "
+            f"chain {out['sampling']['nufft_sampling_chain_ms']} ms")
     if not args.no_toeplitz and not args.small:
         log("nufft: forward + adjoint, fused vs chain ...")
         out["nufft_chain"] = nufft_leg(dev)

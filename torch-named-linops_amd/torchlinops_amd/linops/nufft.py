@@ -5,7 +5,9 @@ Follows src/torchlinops/linops/nufft.py:
     ortho) -> Interpolate(KB, beta) -> Scalar(1 / (w^D sqrt(prod N /
     prod N_pad))) (nufft.py:123-184);
   * "sampling" mode: Pad -> FFT -> Sampling at integer padded-grid
-    locations, no apodization or scaling (185-203);
+    locations, no apodization or scaling (185-203); applied fused as a
+    scaled wrapped pad, an uncentred FFT and the gather at relabelled
+    cells;
   * prep_locs (241-334), beta (336-346), apodize_weights (348-365);
   * adjoint (217-224); normal with ``toeplitz=True`` (226-239) builds
     Crop . F^H . (K *) . F . Pad with K the Fourier transform of the point
@@ -109,6 +111,16 @@ class NUFFT(Chain):
                                                batch_shape=self.batch_shape)
             super().__init__(pad, fft, interp, name="NUFFT")
             self._finish(locs, grid_size, oversamp, width, pad, fft, interp)
+            if options.get("fused", True):
+                # Pad . centred FFT . Sampling as a wrapped pad scaled by the
+                # orthonormal 1/sqrt(prod N_pad), an unscaled FFT and the
+                # gather at cells relabelled by -N_pad/2 (exact for cells)
+                fw = torch.full(tuple(grid_size), prod(padded_size) ** -0.5,
+                                dtype=torch.float32, device=device)
+                self.register_buffer("_fused_weight", fw, persistent=False)
+                self._fused_map = None
+                self._fused_src = None
+                self._fused_on = True
             return
         beta = self.beta(width, oversamp)
         weight = (self.apodize_weights(grid_size, padded_size, oversamp, width, beta).to(device)
@@ -153,8 +165,18 @@ class NUFFT(Chain):
         return torch.remainder(locs.detach().float() - shift, size).contiguous()
 
     def _locs_key(self):
+        if isinstance(self.interp, Sampling):
+            return tuple((i.data_ptr(), i._version, str(i.device)) for i in self.interp.idx)
         l = self.interp.locs
         return (l.data_ptr(), l._version, str(l.device))
+
+    def _sampling_map(self):
+        """Index map of the sampled cells relabelled by -N_pad/2 (mod N_pad)."""
+        from ..functional import _index as FI
+
+        padded = tuple(self.pad.pad_im_size)
+        idx = tuple(torch.remainder(i - n // 2, n) for i, n in zip(self.interp.idx, padded))
+        return FI.index_map(idx, padded, idx[0].device)
 
     def _fused_ok(self, x) -> bool:
         return (self._fused_on and x.is_cuda and x.dtype == torch.complex64
@@ -166,6 +188,19 @@ class NUFFT(Chain):
 
         grid, padded = self.grid_size, tuple(self.pad.pad_im_size)
         dims = tuple(range(-len(grid), 0))
+        if isinstance(self.interp, Sampling):
+            from ..functional import _index as FI
+
+            if self._locs_key() != self._fused_src:  # built, edited or moved
+                self._fused_map = self._sampling_map()
+                self._fused_src = self._locs_key()
+            if forward:
+                y = pad_wrapped(x, grid, padded, weight=self._fused_weight)
+                y = torch.fft.fftn(y, dim=dims, norm="backward")  # unscaled
+                return FI.index_with_map(y, self._fused_map)
+            g = FI.index_adjoint_with_map(x, self._fused_map)
+            g = torch.fft.ifftn(g, dim=dims, norm="forward")  # unscaled
+            return crop_wrapped(g, grid, padded, weight=self._fused_weight)
         p = self.interp.interp_params
         if self._locs_key() != self._fused_src:  # the trajectory was edited or moved
             self._fused_locs = self._relabel(self.interp.locs)
