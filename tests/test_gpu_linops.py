@@ -291,3 +291,20 @@ def test_nufft_built_on_cpu_then_moved(dev):
     x = torch.randn(grid, dtype=torch.complex64, device=dev)
     assert rel_l2(A(x).cpu().numpy(), B(x).cpu().numpy()) < 1e-5
     assert rel_l2(A.N(x).cpu().numpy(), B.N(x).cpu().numpy()) < 1e-5
+
+
+@pytest.mark.parametrize("n", [64, 45])
+def test_fused_nufft_1d_matches_chain_and_oracle(dev, n):
+    rng = np.random.default_rng(14)
+    locs = (rng.uniform(-0.5, 0.5, (300, 1)) * n).astype(np.float32)
+    A = tl.NUFFT(torch.from_numpy(locs).to(dev), (n,), output_shape=("K",), oversamp=2.0)
+    B = tl.NUFFT(torch.from_numpy(locs).to(dev), (n,), output_shape=("K",), oversamp=2.0,
+                 fused=False)
+    x = (rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))).astype(np.complex64)
+    xt = torch.from_numpy(x).to(dev)
+    got = A(xt).cpu().numpy()
+    assert rel_l2(got, B(xt).cpu().numpy()) < 1e-5
+    assert rel_l2(got, O.nufft(x.astype(np.complex128), locs.astype(np.float64), 2.0, 4.0)) < 1e-5
+    y = (rng.standard_normal((2, 300)) + 1j * rng.standard_normal((2, 300))).astype(np.complex64)
+    assert rel_l2(A.H(torch.from_numpy(y).to(dev)).cpu().numpy(),
+                  B.H(torch.from_numpy(y).to(dev)).cpu().numpy()) < 1e-5
