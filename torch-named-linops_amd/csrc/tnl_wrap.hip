@@ -160,6 +160,47 @@ __global__ __launch_bounds__(256) void crop_wrapped_kernel(const E* __restrict__
   }
 }
 
+// complex64 crop with 16-byte stores: a lane writes the cell pair
+// (2 j, 2 j + 1) of an output row (N2 even); the two sources are
+// (2 j - N2/2) mod P2 and the next cell, adjacent unless the pair straddles
+// the wrap.  A workgroup pass covers max(1, 256 / (N2 / 2)) rows.
+template <bool W, typename I>
+__global__ __launch_bounds__(256) void crop_wrapped_pairs_kernel(const uint2* __restrict__ y,
+                                                                 uint4* __restrict__ x,
+                                                                 const float* __restrict__ w,
+                                                                 WrapGeom g) {
+  const I rows = (I)(g.nbatch * g.N[0] * g.N[1]);
+  const I n0 = (I)g.N[0], n1 = (I)g.N[1];
+  const int pairs = (int)(g.N[2] / 2);
+  const int per = pairs >= 256 ? 1 : 256 / pairs;
+  const int lr = (int)threadIdx.x / (pairs >= 256 ? 256 : pairs);
+  const int lp = (int)threadIdx.x % (pairs >= 256 ? 256 : pairs);
+  if (lr >= per) return;
+  for (I r = (I)blockIdx.x * per + lr; r < rows; r += (I)gridDim.x * per) {
+    const int64_t i1 = (int64_t)(r % n1);
+    const I t = r / n1;
+    const int64_t i0 = (int64_t)(t % n0);
+    const int64_t n = (int64_t)(t / n0);
+    int64_t q0 = i0 - g.N[0] / 2, q1 = i1 - g.N[1] / 2;
+    if (q0 < 0) q0 += g.P[0];
+    if (q1 < 0) q1 += g.P[1];
+    const uint2* src = y + ((n * g.P[0] + q0) * g.P[1] + q1) * g.P[2];
+    const float* wrow = W ? w + (i0 * g.N[1] + i1) * g.N[2] : nullptr;
+    uint4* dst = x + (int64_t)r * pairs;
+    for (int j = lp; j < pairs; j += 256) {
+      uint2 v[2];
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        int64_t q2 = 2 * j + e - g.N[2] / 2;
+        if (q2 < 0) q2 += g.P[2];
+        v[e] = src[q2];
+        if (W) v[e] = scaled(v[e], wrow[2 * j + e]);
+      }
+      dst[j] = make_uint4(v[0].x, v[0].y, v[1].x, v[1].y);
+    }
+  }
+}
+
 unsigned row_blocks(int64_t rows) {
   const int64_t cap = 256 * 64;
   return (unsigned)(rows < 1 ? 1 : (rows > cap ? cap : rows));
@@ -184,6 +225,19 @@ int launch_wrap(bool pad, const void* in, void* out, const float* w, const WrapG
       pad_wrapped_pairs_kernel<W, int64_t><<<nb, 256, 0, s>>>((const uint2*)in, (uint4*)out,
                                                              w, g);
     TNL_LAUNCH_CHECK("pad_wrapped");
+    return TNL_OK;
+  }
+  if (!pad && EB == 8 && g.N[2] % 2 == 0 && ((uintptr_t)out & 15) == 0) {
+    const int64_t pairs = g.N[2] / 2;
+    const int64_t per = pairs >= 256 ? 1 : 256 / pairs;
+    const unsigned nb = row_blocks((rows + per - 1) / per);
+    if (small)
+      crop_wrapped_pairs_kernel<W, uint32_t><<<nb, 256, 0, s>>>((const uint2*)in, (uint4*)out,
+                                                               w, g);
+    else
+      crop_wrapped_pairs_kernel<W, int64_t><<<nb, 256, 0, s>>>((const uint2*)in, (uint4*)out,
+                                                              w, g);
+    TNL_LAUNCH_CHECK("crop_wrapped");
     return TNL_OK;
   }
   if (pad) {
