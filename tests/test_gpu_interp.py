@@ -55,6 +55,7 @@ CASES = [
     ((32, 40), 2000, 3, 4.0, "kaiser_bessel", "circular", 1),
     ((33, 31), 1500, 2, (3.0, 6.0), "kaiser_bessel", "zero", 1),
     ((24, 24), 800, 1, 2.0, "spline", "circular", 1),
+    ((96, 80), 3000, 11, 4.0, "kaiser_bessel", "circular", 1),  # 2-D matrix-core tiles, 2 chunks
     ((20, 24), 500, 2, 4.0, "spline", "circular", 2),
     ((16, 18, 20), 1500, 2, 4.0, "kaiser_bessel", "circular", 1),
     ((16, 16, 16), 1000, 5, (4.0, 3.0, 7.0), "kaiser_bessel", "zero", 1),
@@ -250,24 +251,28 @@ def test_grid_mfma_order_is_deterministic(dev):
     _plan.clear_plan_cache()
 
 
-@pytest.mark.parametrize("cplx,nbatch", [(True, 3), (True, 11), (False, 17)])
-def test_grid_ws_records_match_direct_reads(dev, cplx, nbatch):
+@pytest.mark.parametrize("n,cplx,nbatch", [
+    ((24, 32, 40), True, 3), ((24, 32, 40), True, 11), ((24, 32, 40), False, 17),
+    # 2-D matrix-core tiles (16 x 32); 80 leaves a half tile at the edge
+    ((96, 80), True, 11), ((96, 80), False, 17)])
+def test_grid_ws_records_match_direct_reads(dev, n, cplx, nbatch):
     """tnl_grid_ws (values re-packed point-major into the caller's scratch)
     gives bit-identical results to tnl_grid (values read from the
     coil-major input) through the C ABI: same kernel, same summation
-    order (no split tiles here); a scratch below tnl_grid_scratch_bytes is
-    rejected."""
+    order (no split tiles here), also with a partial last channel chunk;
+    a scratch below tnl_grid_scratch_bytes is rejected."""
     import ctypes
     from torchlinops_amd import _lib
     from torchlinops_amd.functional import _plan
 
+    print(f"grid_ws n={n} cplx={cplx} nbatch={nbatch}", flush=True)
     rng = np.random.default_rng(61)
-    n = (24, 32, 40)
-    locs = to_dev(rng.uniform(-2, np.array(n) + 2, (5000, 3)).astype(np.float32), dev)
+    nd = len(n)
+    locs = to_dev(rng.uniform(-2, np.array(n) + 2, (5000, nd)).astype(np.float32), dev)
     y = crandn(rng, (nbatch, 5000)) if cplx else rng.standard_normal((nbatch, 5000)).astype(np.float32)
     yv = to_dev(y, dev)
     yr = torch.view_as_real(yv) if cplx else yv
-    plan = _plan.build_plan(locs, 3, n, (4.0,) * 3, 0)
+    plan = _plan.build_plan(locs, nd, n, (4.0,) * nd, 0)
     assert plan.layout.grid_accum == 2
     h = _lib.lib()
     nbytes = ctypes.c_int64(0)
@@ -278,8 +283,8 @@ def test_grid_ws_records_match_direct_reads(dev, cplx, nbatch):
     for scratch in (None, torch.empty(nbytes.value, dtype=torch.uint8, device=dev)):
         out = torch.empty(nbatch, *n, dtype=yv.dtype, device=dev)
         outr = torch.view_as_real(out) if cplx else out
-        args = [yr.data_ptr(), locs.data_ptr(), outr.data_ptr(), nbatch, 5000, 3,
-                _lib.i64_array(n), _lib.f32_array((4.0,) * 3), 0, 1, 0, int(cplx), 1.7,
+        args = [yr.data_ptr(), locs.data_ptr(), outr.data_ptr(), nbatch, 5000, nd,
+                _lib.i64_array(n), _lib.f32_array((4.0,) * nd), 0, 1, 0, int(cplx), 1.7,
                 plan.layout, plan.ptr]
         if scratch is None:
             _lib.check(h.tnl_grid(*args, _lib.stream_of(yv)))
