@@ -553,22 +553,24 @@ __global__ __launch_bounds__(256) void sub_bank_order_kernel(const int4* __restr
 // scatter decide), so the summation order of an item is the same for every
 // plan of a trajectory.  One workgroup per item: bitonic sort of
 // (mask << 32 | original index) keys in LDS, each carrying its sorted
-// index; items above SORT_CAP touches (only with a g_cap tuned above it)
-// keep the scatter order.
-constexpr int SORT_CAP = 4096;
-
+// index; items above CAP touches (only with a g_cap tuned above 4096)
+// keep the scatter order.  CAP sizes the LDS key buffer (12 bytes a key):
+// plans with g_cap <= 2048 take the 24 KB form, twice as many resident
+// workgroups per CU as the 48 KB one.  Keys are unique within an item, so
+// both forms give the same order.
+template <int CAP>
 __global__ __launch_bounds__(256) void item_sort_kernel(const int4* __restrict__ items,
                                                         const int* __restrict__ nitems_ptr,
                                                         const float* __restrict__ slocs,
                                                         const int* __restrict__ perm,
                                                         PlanDev P, int* __restrict__ tlist) {
-  __shared__ unsigned long long key[SORT_CAP];
-  __shared__ int val[SORT_CAP];
+  __shared__ unsigned long long key[CAP];
+  __shared__ int val[CAP];
   const int nitems = *nitems_ptr;
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const int4 item = items[it];
     const int start = item.y, count = item.z;
-    if (count < 2 || count > SORT_CAP) continue;
+    if (count < 2 || count > CAP) continue;
     int origin[3];
     {
       int r = item.x;
@@ -750,8 +752,12 @@ extern "C" int tnl_plan_build(const float* locs, const tnl_plan_layout* layout,
   TNL_LAUNCH_CHECK("plan items");
   if (npts > 0 && L.grid_accum == TNL_GRID_ACCUM_MFMA) {
     const int64_t nwg = L.item_capacity < 8192 ? L.item_capacity : 8192;
-    item_sort_kernel<<<(unsigned)nwg, 256, 0, s>>>(items, itemoff + L.g_ntiles_total, slocs,
-                                                    perm, P, tlist);
+    if (L.g_cap <= 2048)
+      item_sort_kernel<2048><<<(unsigned)nwg, 256, 0, s>>>(items, itemoff + L.g_ntiles_total,
+                                                          slocs, perm, P, tlist);
+    else
+      item_sort_kernel<4096><<<(unsigned)nwg, 256, 0, s>>>(items, itemoff + L.g_ntiles_total,
+                                                          slocs, perm, P, tlist);
     TNL_LAUNCH_CHECK("plan item sort");
   }
   if (npts > 0 && L.ndim == 3) {
