@@ -447,17 +447,19 @@ __global__ void item_copy_kernel(const int4* __restrict__ src, const int* __rest
 // points start in different banks: by rank within their bank class (rank
 // by original index), then by class.  The order depends only on the
 // trajectory, so the plan stays deterministic.  One workgroup per
-// subproblem; subproblems above ORDER_CAP points (a max_sub tuned above
-// it) keep the bin order.
-constexpr int ORDER_CAP = 2048;
-
+// subproblem; subproblems above CAP points (a max_sub tuned above 2048)
+// keep the bin order.  CAP sizes the LDS buffers (24 bytes a point): plans
+// with max_sub <= 512 take the 12 KB form, which keeps more workgroups
+// resident per CU than the 48 KB one; both give the same order.
+template <int CAP>
 __global__ __launch_bounds__(256) void sub_bank_order_kernel(const int4* __restrict__ sub,
                                                              const int* __restrict__ nsub_ptr,
                                                              PlanDev P, int* __restrict__ perm,
                                                              float* __restrict__ slocs) {
-  __shared__ unsigned long long key[ORDER_CAP];
-  __shared__ float s_loc[ORDER_CAP * 3];
-  __shared__ int s_perm[ORDER_CAP];
+  static_assert(CAP <= 2048, "the key keeps the point's slot in 11 bits");
+  __shared__ unsigned long long key[CAP];
+  __shared__ float s_loc[CAP * 3];
+  __shared__ int s_perm[CAP];
   __shared__ int ccount[32], cstart[32];
   const int nsub = *nsub_ptr;
   int win[3];
@@ -465,7 +467,7 @@ __global__ __launch_bounds__(256) void sub_bank_order_kernel(const int4* __restr
   for (int si = blockIdx.x; si < nsub; si += gridDim.x) {
     const int4 sp = sub[si];
     const int bin = sp.x, start = sp.y, count = sp.z;
-    if (bin >= P.u_nbins || count < 2 || count > ORDER_CAP) continue;
+    if (bin >= P.u_nbins || count < 2 || count > CAP) continue;
     int origin[3];
     {
       int r = bin;
@@ -735,8 +737,12 @@ extern "C" int tnl_plan_build(const float* locs, const tnl_plan_layout* layout,
   TNL_LAUNCH_CHECK("plan subproblems");
   if (npts > 0) {
     const int64_t nwg = L.sub_capacity < 8192 ? L.sub_capacity : 8192;
-    sub_bank_order_kernel<<<(unsigned)nwg, 256, 0, s>>>(sub, suboff + L.u_nbins + 1, P, perm,
-                                                         slocs);
+    if (L.max_sub <= 512)
+      sub_bank_order_kernel<512><<<(unsigned)nwg, 256, 0, s>>>(sub, suboff + L.u_nbins + 1, P,
+                                                              perm, slocs);
+    else
+      sub_bank_order_kernel<2048><<<(unsigned)nwg, 256, 0, s>>>(sub, suboff + L.u_nbins + 1, P,
+                                                               perm, slocs);
     TNL_LAUNCH_CHECK("plan subproblem order");
   }
   scan_kernel<<<1, 1024, 0, s>>>(gcounts, gstarts, L.g_ntiles_total, 1, 0);
