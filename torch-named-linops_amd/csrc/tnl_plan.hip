@@ -440,6 +440,14 @@ __global__ void item_copy_kernel(const int4* __restrict__ src, const int* __rest
     dst[i] = src[i];
 }
 
+// Bitonic stages below run one thread per compare pair (n/2 of them) rather
+// than per element: pair p of a stage with distance j (a power of two) is
+// (i, i | j), where i is p with a zero bit inserted at j's position, so
+// every lane does work at every distance.
+__device__ __forceinline__ int bitonic_low(int p, int j) {
+  return ((p & ~(j - 1)) << 1) | (p & (j - 1));
+}
+
 // Interpolation gathers with lane = point: in one LDS instruction every lane
 // reads its point's window row at the same tap offset, so two lanes whose
 // rows start in the same bank (complex cells: 8-byte words, 32 bank pairs)
@@ -513,14 +521,12 @@ __global__ __launch_bounds__(256) void sub_bank_order_kernel(const int4* __restr
     for (int pass = 0; pass < 2; ++pass) {
       for (int k = 2; k <= n2; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-          for (int i = threadIdx.x; i < n2; i += blockDim.x) {
-            const int l = i ^ j;
-            if (l > i) {
-              const unsigned long long a = key[i], b = key[l];
-              if ((a > b) == ((i & k) == 0)) {
-                key[i] = b;
-                key[l] = a;
-              }
+          for (int p = threadIdx.x; p < (n2 >> 1); p += blockDim.x) {
+            const int i = bitonic_low(p, j), l = i | j;
+            const unsigned long long a = key[i], b = key[l];
+            if ((a > b) == ((i & k) == 0)) {
+              key[i] = b;
+              key[l] = a;
             }
           }
           __syncthreads();
@@ -610,18 +616,15 @@ __global__ __launch_bounds__(256) void item_sort_kernel(const int4* __restrict__
     __syncthreads();
     for (int k = 2; k <= n2; k <<= 1) {
       for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int i = threadIdx.x; i < n2; i += blockDim.x) {
-          const int l = i ^ j;
-          if (l > i) {
-            const unsigned long long a = key[i], b = key[l];
-            const bool up = (i & k) == 0;
-            if ((a > b) == up) {
-              key[i] = b;
-              key[l] = a;
-              const int t = val[i];
-              val[i] = val[l];
-              val[l] = t;
-            }
+        for (int p = threadIdx.x; p < (n2 >> 1); p += blockDim.x) {
+          const int i = bitonic_low(p, j), l = i | j;
+          const unsigned long long a = key[i], b = key[l];
+          if ((a > b) == ((i & k) == 0)) {
+            key[i] = b;
+            key[l] = a;
+            const int t = val[i];
+            val[i] = val[l];
+            val[l] = t;
           }
         }
         __syncthreads();
