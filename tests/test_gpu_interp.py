@@ -251,6 +251,47 @@ def test_grid_mfma_order_is_deterministic(dev):
     _plan.clear_plan_cache()
 
 
+@pytest.mark.parametrize("n", [(32, 32, 32), (96, 80)])
+def test_plan_sort_forms_agree(dev, n):
+    """The plan's two ordering passes (gridding-item touches by block mask,
+    subproblem points by LDS bank) size their LDS for the plan's caps:
+    g_cap <= 2048 and max_sub <= 512 take the small forms.  With 2000
+    uniform points no item or subproblem reaches those caps (about 30
+    points per 8^3 bin and 250 per 32x32 bin), so raising the caps to
+    g_cap 4096 / max_sub 2048 (the large forms) changes nothing else and
+    must give bit-identical interpolation and gridding."""
+    import torchlinops_amd.functional as F
+    from torchlinops_amd import config
+    from torchlinops_amd.functional import _plan
+
+    print(f"plan sort forms n={n}", flush=True)
+    nd = len(n)
+    rng = np.random.default_rng(71)
+    locs = to_dev(rng.uniform(0, np.array(n), (2000, nd)).astype(np.float32), dev)
+    x = to_dev(crandn(rng, (3, *n)), dev)
+    y = to_dev(crandn(rng, (3, 2000)), dev)
+    outs = []
+    try:
+        for knobs in ({}, {"g_cap": 4096, "max_sub": 2048}):
+            with _plan._tuning_lock:
+                _plan._tuning.pop(nd, None)
+            _plan.set_tuning(nd, **knobs)
+            with config.using(interp_plan="always"):
+                u = F.interpolate(x, locs, 4.0)
+                g = F.interpolate_adjoint(y, locs, n, 4.0)
+            torch.cuda.synchronize()
+            outs.append((u, g))
+    finally:
+        with _plan._tuning_lock:
+            _plan._tuning.pop(nd, None)
+        _plan.clear_plan_cache()
+    assert torch.equal(outs[0][0], outs[1][0])
+    assert torch.equal(outs[0][1], outs[1][1])
+    ref = O.grid(y.cpu().numpy(), locs.cpu().numpy(), n, 4.0, "kaiser_bessel", 1, "circular",
+                 1.0, "f64")
+    assert rel_l2(outs[0][1].cpu().numpy(), ref) < TOL
+
+
 @pytest.mark.parametrize("n,cplx,nbatch", [
     ((24, 32, 40), True, 3), ((24, 32, 40), True, 11), ((24, 32, 40), False, 17),
     # 2-D matrix-core tiles (16 x 32); 80 leaves a half tile at the edge
