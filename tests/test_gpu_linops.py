@@ -233,8 +233,12 @@ def test_fused_nufft_matches_chain(dev, grid):
 
 def test_polynomial_preconditioned_cg(dev):
     """CG on (A^H A + lam I) of a 2-D NUFFT with an l_2 / l_inf polynomial
-    preconditioner of the normalised operator reaches the plain-CG
-    solution in fewer iterations."""
+    preconditioner of the normalised operator (condition number about 100)
+    reaches a relative residual of 1e-4 in fewer iterations than plain CG,
+    and its solution's true residual agrees.  The iteration counts are
+    compared at 1e-4, not at 1e-5: in float32 both solvers' residuals
+    stall near 1e-5 on this problem, where the count is set by rounding
+    rather than by the preconditioner."""
     from torchlinops_amd.alg import DeviceCG, poly as Pm
 
     rng = np.random.default_rng(12)
@@ -242,23 +246,30 @@ def test_polynomial_preconditioned_cg(dev):
     locs = torch.from_numpy((rng.uniform(-0.5, 0.5, (3000, 2)) * 32).astype(np.float32)).to(dev)
     A = tl.NUFFT(locs, grid, output_shape=("K",), oversamp=2.0)
     N = A.N
-    v = torch.randn(grid, dtype=torch.complex64, device=dev)
+    g = torch.Generator(device=dev).manual_seed(13)
+    v = torch.randn(grid, dtype=torch.complex64, device=dev, generator=g)
     for _ in range(30):  # power iteration for the largest eigenvalue
         v = N(v)
         lmax = float(torch.linalg.vector_norm(v))
         v = v / lmax
     lam = 1e-2 * lmax
     op = lambda x: (N(x) + lam * x) / (lmax + lam)  # spectrum in (0, 1]
-    y = torch.randn(grid, dtype=torch.complex64, device=dev)
+    y = torch.randn(grid, dtype=torch.complex64, device=dev, generator=g)
+    tol = 1e-4
+
+    def true_residual(x):
+        return float(torch.linalg.vector_norm(op(x) - y) / torch.linalg.vector_norm(y))
+
     plain = DeviceCG(op, check_every=1)
-    x0 = plain.solve(y, tol=1e-5, max_iters=500)
+    plain.solve(y, tol=tol, max_iters=500)
     for norm, lo in (("l_2", 0.0), ("l_inf", lam / (lmax + lam))):
         c = Pm._coeffs(3, norm, lo, 1.0)
         cg = DeviceCG(op, M=lambda r, c=c: Pm.apply_polynomial(op, c, r), check_every=1)
-        x = cg.solve(y, tol=1e-5, max_iters=500)
-        assert cg.history[-1] <= 1e-5
+        x = cg.solve(y, tol=tol, max_iters=500)
+        print(f"poly {norm} deg 3: {cg.iters} iterations, plain CG {plain.iters}", flush=True)
+        assert cg.history[-1] <= tol
         assert cg.iters < plain.iters
-        assert rel_l2(x.cpu().numpy(), x0.cpu().numpy()) < 1e-3
+        assert true_residual(x) < 2 * tol
 
 
 def test_fused_nufft_follows_trajectory_edits(dev):
