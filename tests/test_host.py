@@ -139,11 +139,15 @@ def test_functional_errors_before_device_work():
         F.array_to_blocks(torch.zeros(9, 9), (3, 3), (1, 1), out=torch.zeros(1, dtype=torch.float64))
     with pytest.raises(RuntimeError, match="Fold expected input"):
         F.blocks_to_array(torch.zeros(2, 2, 3, 3), (9, 9), (3, 3), (2, 2))
-    # no CPU fallback: the hot ops refuse host tensors
+    # host tensors take the host path; ops with no host path still refuse them
+    assert F.interpolate(vals, locs, 4.0).shape == (5,)
+    assert F.array_to_blocks(torch.zeros(9, 9), (3, 3), (1, 1)).shape == (7, 7, 3, 3)
+    from torchlinops_amd.linops.toeplitz import pad_wrapped
+
     with pytest.raises(RuntimeError, match="HIP devices only"):
-        F.interpolate(vals, locs, 4.0)
-    with pytest.raises(RuntimeError, match="HIP devices only"):
-        F.array_to_blocks(torch.zeros(9, 9), (3, 3), (1, 1))
+        pad_wrapped(vals, (8, 8), (16, 16))
+    with pytest.raises(RuntimeError, match="one device"):
+        F.interpolate(vals, locs.to("meta"), 4.0)
 
 
 def test_get_nblocks_doc_examples():
@@ -154,3 +158,47 @@ def test_get_nblocks_doc_examples():
     assert get_nblocks((6, 6), (3, 3), (2, 2)) == (2, 2)
     assert get_nblocks((7, 7), (3, 3), (2, 2)) == (3, 3)
     assert get_nblocks((10, 10), (8, 8), (4, 4)) == (1, 1)
+
+
+def test_plan_cache_never_builds_under_graph_capture(monkeypatch):
+    """While a stream is being captured a plan build would only be recorded:
+    the cache must not build or store one then (a miss takes the plan-free
+    kernels), must hand out and pin a plan it already holds, and must report
+    every plan used to an active recorder."""
+    from torchlinops_amd import config
+    from torchlinops_amd.functional import _plan
+
+    _plan.clear_plan_cache()
+    _plan.release_plan_pins()
+    built = []
+
+    def fake_build(locs2d, ndim, grid_size, width, pad_code, tuning=None):
+        built.append(locs2d)
+        return _plan.Plan(layout=None, workspace=torch.empty(1, dtype=torch.uint8))
+
+    monkeypatch.setattr(_plan, "build_plan", fake_build)
+    locs = torch.rand(100, 3) * 16
+    args = (locs, locs, 3, (16, 16, 16), (4.0,) * 3, 1, 0)
+    capturing = {"on": True}
+    monkeypatch.setattr(_plan, "_capturing", lambda device: capturing["on"])
+    with config.using(interp_plan="always"):
+        # capture, cache miss: no build, no store, plan-free kernels
+        assert _plan.get_plan(*args) is None
+        assert built == []
+        # eager: builds and caches
+        capturing["on"] = False
+        plan = _plan.get_plan(*args)
+        assert plan is not None and len(built) == 1
+        # capture, cache hit: same plan, pinned, recorded, no build
+        capturing["on"] = True
+        sink = []
+        with _plan.record_plans(sink):
+            assert _plan.get_plan(*args) is plan
+        assert sink == [plan] and len(built) == 1
+        assert any(p is plan for p in _plan._cache._pinned.values())
+        # clearing the cache keeps the pin (a live graph may point at it)
+        _plan.clear_plan_cache()
+        assert any(p is plan for p in _plan._cache._pinned.values())
+        assert _plan.get_plan(*args) is None and len(built) == 1
+    _plan.release_plan_pins()
+    assert not _plan._cache._pinned

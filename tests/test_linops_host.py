@@ -205,12 +205,17 @@ def test_sampling_from_mask_and_range_error():
         tl.Sampling((torch.tensor([0, 1]),), input_size=(8, 8))
 
 
-def test_sampling_has_no_cpu_path():
-    A = tl.Sampling((torch.tensor([0, 3]),), input_size=(8,))
-    with pytest.raises(RuntimeError, match="HIP devices only"):
-        A(torch.randn(8))
-    with pytest.raises(RuntimeError, match="HIP devices only"):
-        tl.functional.index(torch.randn(8), (torch.tensor([1]),))
+def test_sampling_host_path():
+    """Host tensors take index_select / sequential index_add_, like the
+    reference's CPU branch."""
+    A = tl.Sampling((torch.tensor([0, 3, 3]),), input_size=(8,))
+    x = torch.randn(8)
+    assert torch.equal(A(x), x[[0, 3, 3]])
+    y = torch.tensor([1.0, 2.0, 4.0])
+    want = torch.zeros(8)
+    want[0], want[3] = 1.0, 6.0
+    assert torch.equal(A.H(y), want)
+    assert torch.equal(tl.functional.index(x, (torch.tensor([1]),)), x[[1]])
 
 
 def test_index_adjoint_bounds_error():

@@ -2,8 +2,10 @@
 
 Only plain pointers, sizes and a hipStream_t cross the boundary; torch is
 used here solely to hand over device pointers and the current stream.
-There is no CPU fallback: every entry point requires tensors on a HIP
-device and raises if the library is missing or the tensors live elsewhere.
+Every entry point takes tensors on a HIP device, and loading raises if the
+library is missing: device tensors never run anywhere but libtnl.  Host
+tensors are dispatched before this module, at the reference's own
+``is_cuda`` branch, to the package's torch restatement (functional/_host.py).
 """
 
 from __future__ import annotations
@@ -293,5 +295,5 @@ def require_device(*tensors: torch.Tensor, op: str) -> None:
         if not t.is_cuda:
             raise RuntimeError(
                 f"torchlinops_amd.{op} runs on HIP devices only; got a tensor on "
-                f"{t.device}. Move inputs to the GPU (there is no CPU path)."
+                f"{t.device}. Move inputs to the GPU (this op has no host path)."
             )

@@ -162,15 +162,21 @@ class DeviceCG:
     def _capture(self, n):
         """Run n iterations eagerly on a side stream (so plans and
         allocations exist), then record n iterations into a graph without
-        executing them."""
+        executing them.  The plans the recorded kernels read are kept in
+        ``self.graph_plans`` for as long as the graph is, so a plan-cache
+        eviction cannot free a workspace the graph points to."""
+        from ..functional._plan import record_plans
+
+        self.graph_plans = []
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
-            for _ in range(n):
-                self._iteration()
-        torch.cuda.current_stream().wait_stream(s)
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            for _ in range(n):
-                self._iteration()
+        with record_plans(self.graph_plans):
+            with torch.cuda.stream(s):
+                for _ in range(n):
+                    self._iteration()
+            torch.cuda.current_stream().wait_stream(s)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                for _ in range(n):
+                    self._iteration()
         return g

@@ -14,7 +14,7 @@ from ._index import (
 )
 from ._interp import grid, interp_grad_locs, interpolate, interpolate_adjoint, ungrid
 from ._nufft import center_crop, center_pad, nufft, nufft_adjoint
-from ._plan import clear_plan_cache
+from ._plan import clear_plan_cache, record_plans, release_plan_pins
 from ._unfold import (
     array_to_blocks,
     blocks_to_array,
@@ -38,6 +38,8 @@ __all__ = [
     "get_nblocks",
     "get_norm_weights",
     "clear_plan_cache",
+    "record_plans",
+    "release_plan_pins",
     "nufft",
     "nufft_adjoint",
     "center_pad",

@@ -11,7 +11,8 @@ src/torchlinops/functional/_index/index.py:
   * ``mask2idx`` / ``canonicalize_idx`` / ``slice2range`` /
     ``ensure_tensor_indexing``  102-137, 201-277
 The gather and the scatter-add run in libtnl (tnl_index /
-tnl_index_adjoint).  The scatter-add is a cell-major sum over a CSR map
+tnl_index_adjoint) for tensors on a HIP device; host tensors take
+``index_select`` / a sequential ``index_add_`` (``_host.py``).  The scatter-add is a cell-major sum over a CSR map
 of the points (``IndexMap``), so it is deterministic, and for cells hit by
 at most ``HEAVY_MIN`` points it equals the CPU's sequential index_add_ bit
 for bit (busier cells are summed by a workgroup each in a fixed order);
@@ -29,7 +30,8 @@ import torch
 from torch import Tensor
 from torch.autograd import Function
 
-from .._lib import TNL_F32, TNL_F64, check, lib, require_device, stream_of
+from .._lib import TNL_F32, TNL_F64, check, lib, stream_of
+from ._host import gather_host, scatter_add_host
 
 __all__ = [
     "index",
@@ -197,8 +199,12 @@ def _gather(vals: Tensor, imap: IndexMap) -> Tensor:
         )
     batch = tuple(vals.shape[: vals.ndim - nd])
     x = vals.contiguous()
-    out = torch.empty(batch + imap.out_shape, dtype=vals.dtype, device=vals.device)
     nbatch = prod(batch)
+    if not vals.is_cuda:
+        flat = imap.flat.to(vals.device)
+        out = gather_host(x.reshape(nbatch, imap.ngrid), flat)
+        return out.reshape(batch + imap.out_shape)
+    out = torch.empty(batch + imap.out_shape, dtype=vals.dtype, device=vals.device)
     if out.numel() == 0:
         return out
     with torch.cuda.device(vals.device):
@@ -209,7 +215,6 @@ def _gather(vals: Tensor, imap: IndexMap) -> Tensor:
 
 
 def _scatter(vals: Tensor, imap: IndexMap) -> Tensor:
-    code, cplx = _real_dtype_code(vals.dtype)
     nl = len(imap.out_shape)
     batch = tuple(vals.shape[: vals.ndim - nl])
     if tuple(vals.shape[vals.ndim - nl:]) != imap.out_shape:
@@ -218,8 +223,13 @@ def _scatter(vals: Tensor, imap: IndexMap) -> Tensor:
             f"{tuple(vals.shape)} and idx: {imap.out_shape}"
         )
     x = vals.contiguous()
-    out = torch.empty(batch + imap.grid_size, dtype=vals.dtype, device=vals.device)
     nbatch = prod(batch)
+    if not vals.is_cuda:
+        flat = imap.flat.to(vals.device)
+        out = scatter_add_host(x.reshape(nbatch, imap.npts), flat, imap.ngrid)
+        return out.reshape(batch + imap.grid_size)
+    code, cplx = _real_dtype_code(vals.dtype)
+    out = torch.empty(batch + imap.grid_size, dtype=vals.dtype, device=vals.device)
     if out.numel() == 0:
         return out
     imap.grouped()
@@ -258,13 +268,11 @@ class _IndexAdjointFn(Function):
 
 def index_with_map(vals: Tensor, imap: IndexMap) -> Tensor:
     """``index`` over a prebuilt ``IndexMap``: ``[N..., *grid] -> [N..., I...]``."""
-    require_device(vals, op="index")
     return _IndexFn.apply(vals, imap)
 
 
 def index_adjoint_with_map(vals: Tensor, imap: IndexMap) -> Tensor:
     """``index_adjoint`` over a prebuilt ``IndexMap``: ``[N..., I...] -> [N..., *grid]``."""
-    require_device(vals, op="index_adjoint")
     return _IndexAdjointFn.apply(vals, imap)
 
 
@@ -285,7 +293,6 @@ def index(vals: Tensor, idx: tuple, deterministic_backward: bool = False) -> Ten
             f"Input value with shape {vals.shape} cannot be indexed with index "
             f"tensors of length {len(idx)}"
         )
-    require_device(vals, op="index")
     nd = len(idx)
     grid_size = tuple(vals.shape[vals.ndim - nd:])
     if any(isinstance(i, slice) for i in idx):
@@ -308,7 +315,6 @@ def index(vals: Tensor, idx: tuple, deterministic_backward: bool = False) -> Ten
 def multi_index(x: Tensor, ndims: int, idx: Tensor, raveled: bool = False) -> Tensor:
     """Gather over the last ``ndims`` axes with stacked ``idx [ndims, I...]``
     (or raveled ``[I...]``): ``[N..., D...] -> [N..., I...]``."""
-    require_device(x, op="multi_index")
     grid_size = tuple(x.shape[x.ndim - ndims:])
     if raveled:
         imap = index_map(None, grid_size, x.device, flat=idx)
@@ -327,7 +333,6 @@ def index_adjoint(vals: Tensor, idx: tuple, grid_size: tuple) -> Tensor:
     for d, (dim_idx, dim_size) in enumerate(zip(idx, grid_size)):
         if (dim_idx >= dim_size).any() or (dim_idx < -dim_size).any():
             raise IndexError(f"Out-of-bounds index for grid of shape {grid_size}: idx[{d}]")
-    require_device(vals, op="index_adjoint")
     idx_stacked = torch.stack(tuple(idx), dim=0)
     return multi_grid(vals, idx_stacked, grid_size)
 
@@ -336,7 +341,6 @@ def multi_grid(x: Tensor, idx: Tensor, final_size: tuple, raveled: bool = False,
                ravel_dim: int = 0) -> Tensor:
     """Scatter-add ``x [N..., I...]`` into ``[N..., *final_size]`` at
     ``idx`` (stacked along ``ravel_dim``, or raveled ``[I...]``)."""
-    require_device(x, op="multi_grid")
     final_size = tuple(int(s) for s in final_size)
     if raveled:
         flat = idx

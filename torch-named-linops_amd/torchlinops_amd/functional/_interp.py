@@ -8,8 +8,10 @@ functional API:
   * shape preparation              ungrid.py:577-621, grid.py:522-568
   * kernel defaults                kernels.py:97-105, 245-252
   * autograd pair                  interp.py:13-236
-The arithmetic runs in the HIP kernels of libtnl (tnl_ungrid / tnl_grid);
-tensors must live on a HIP device.
+The arithmetic runs in the HIP kernels of libtnl (tnl_ungrid / tnl_grid)
+for tensors on a HIP device; host tensors take the package's torch
+restatement (``_host.py``), the reference's own dispatch on ``is_cuda``
+(grid.py:91, ungrid.py:86).
 
 Beyond the reference, the autograd pair also differentiates with respect to
 ``locs`` (the reference returns None there, interp.py:46-70, 157-181): the
@@ -34,9 +36,9 @@ from .._lib import (
     f32_array,
     i64_array,
     lib,
-    require_device,
     stream_of,
 )
+from ._host import grid_host, ungrid_host
 from ._plan import get_plan
 
 __all__ = [
@@ -96,23 +98,39 @@ def _check_block_width(width: tuple, is_complex: bool) -> None:
 
 
 def _check_dtypes(vals: torch.Tensor, locs: torch.Tensor) -> None:
-    if vals.dtype not in (torch.float32, torch.complex64):
-        raise TypeError(
-            f"interpolation kernels compute in float32/complex64; got {vals.dtype}"
-        )
+    ok = (torch.float32, torch.complex64)
+    if not vals.is_cuda:
+        ok = ok + (torch.float64, torch.complex128)
+    if vals.dtype not in ok:
+        where = "HIP kernels compute" if vals.is_cuda else "interpolation computes"
+        raise TypeError(f"the {where} in {'/'.join(str(d)[6:] for d in ok)}; got {vals.dtype}")
     if not locs.is_floating_point():
         raise TypeError(f"locs must be floating point, got {locs.dtype}")
 
 
-def _launch(adjoint: bool, vals_flat, locs, locs2d, out, nbatch, npts, ndim,
-            grid_size, width, kernel, norm, pad_mode, is_complex, kernel_params):
+def _same_device(op: str, *tensors) -> None:
+    devs = {t.device for t in tensors if t is not None}
+    if len(devs) > 1:
+        raise RuntimeError(f"{op}: expected all tensors on one device, got {sorted(map(str, devs))}")
+
+
+def _beta(kernel: str, kernel_params: dict) -> float:
+    return float(kernel_params.get("beta", 1.0)) if kernel == "kaiser_bessel" else 1.0
+
+
+def _check_options(kernel, norm, pad_mode) -> None:
     if kernel not in KERNELS:
         raise ValueError(f"Unrecognized kernel {kernel}")
     if norm not in (1, 2):
         raise ValueError(f"Unexpected norm: {norm}.")
     if pad_mode not in PAD_MODES:
         raise ValueError(f"Unrecognized pad_mode {pad_mode}")
-    beta = float(kernel_params.get("beta", 1.0)) if kernel == "kaiser_bessel" else 1.0
+
+
+def _launch(adjoint: bool, vals_flat, locs, locs2d, out, nbatch, npts, ndim,
+            grid_size, width, kernel, norm, pad_mode, is_complex, kernel_params):
+    _check_options(kernel, norm, pad_mode)
+    beta = _beta(kernel, kernel_params)
     pad_code = PAD_MODES[pad_mode]
     plan = get_plan(locs, locs2d, ndim, grid_size, width, norm, pad_code)
     args = [
@@ -173,10 +191,15 @@ def ungrid(
     npts = prod(locs_batch_shape)
     is_complex = torch.is_complex(vals)
     _check_block_width(width, is_complex)
-    require_device(vals, locs, op="ungrid")
+    _same_device("ungrid", vals, locs)
     _check_dtypes(vals, locs)
     nbatch = prod(batch_shape)
     vals_flat = vals.reshape(nbatch, *grid_size).contiguous()
+    if not vals.is_cuda:
+        _check_options(kernel, norm, pad_mode)
+        out = ungrid_host(vals_flat, locs.reshape(npts, ndim), grid_size, width, kernel,
+                          norm, pad_mode, _beta(kernel, kernel_params))
+        return out.reshape(*batch_shape, *locs_batch_shape)
     locs2d = locs.reshape(npts, ndim).to(torch.float32).contiguous()
     out = torch.empty(nbatch, npts, dtype=vals.dtype, device=vals.device)
     if is_complex:
@@ -219,7 +242,7 @@ def grid(
     width = _width_tuple(width, ndim)
     is_complex = torch.is_complex(vals)
     _check_block_width(width, is_complex)
-    require_device(vals, locs, op="grid")
+    _same_device("grid", vals, locs)
     _check_dtypes(vals, locs)
     if tuple(vals.shape[vals.ndim - nlb:]) != tuple(locs_batch_shape):
         raise RuntimeError(
@@ -228,6 +251,11 @@ def grid(
         )
     nbatch = prod(batch_shape)
     vals_flat = vals.reshape(nbatch, npts).contiguous()
+    if not vals.is_cuda:
+        _check_options(kernel, norm, pad_mode)
+        out = grid_host(vals_flat, locs.reshape(npts, ndim), grid_size, width, kernel,
+                        norm, pad_mode, _beta(kernel, kernel_params))
+        return out.reshape(*batch_shape, *grid_size)
     locs2d = locs.reshape(npts, ndim).to(torch.float32).contiguous()
     out = torch.empty(nbatch, *grid_size, dtype=vals.dtype, device=vals.device)
     if is_complex:
@@ -275,23 +303,27 @@ def interp_grad_locs(
     is_complex = torch.is_complex(grid_vals)
     if torch.is_complex(pt_vals) != is_complex:
         raise TypeError("grid_vals and pt_vals must both be real or both complex")
-    if kernel not in KERNELS:
-        raise ValueError(f"Unrecognized kernel {kernel}")
-    if norm not in (1, 2):
-        raise ValueError(f"Unexpected norm: {norm}.")
-    if pad_mode not in PAD_MODES:
-        raise ValueError(f"Unrecognized pad_mode {pad_mode}")
-    require_device(grid_vals, pt_vals, locs, op="interp_grad_locs")
+    _check_options(kernel, norm, pad_mode)
+    _same_device("interp_grad_locs", grid_vals, pt_vals, locs)
     _check_dtypes(grid_vals, locs)
     _check_dtypes(pt_vals, locs)
     g = grid_vals.reshape(nbatch, *grid_size).contiguous()
     h = pt_vals.reshape(nbatch, npts).contiguous()
+    if not grid_vals.is_cuda:
+        # host: differentiate the host interpolation with respect to locs
+        with torch.enable_grad():
+            l2 = locs.detach().reshape(npts, ndim).to(g.real.dtype).requires_grad_(True)
+            y = ungrid_host(g.detach(), l2, grid_size, width, kernel, norm, pad_mode,
+                            _beta(kernel, kernel_params))
+            loss = (h.detach().conj() * y).real.sum()
+            (gl,) = torch.autograd.grad(loss, l2)
+        return gl.reshape(locs.shape).to(locs.dtype)
     if is_complex:
         g = torch.view_as_real(g)
         h = torch.view_as_real(h)
     locs2d = locs.detach().reshape(npts, ndim).to(torch.float32).contiguous()
     out = torch.empty(npts, ndim, dtype=torch.float32, device=locs.device)
-    beta = float(kernel_params.get("beta", 1.0)) if kernel == "kaiser_bessel" else 1.0
+    beta = _beta(kernel, kernel_params)
     if npts > 0:
         with torch.cuda.device(locs.device):
             check(

@@ -6,6 +6,14 @@ only on the locations, grid size, kernel width, padding mode and tiling.  It
 is keyed on the identity and version counter of the caller's ``locs``
 tensor, so in-place edits of ``locs`` invalidate it, and a weak reference
 guards against a new tensor reusing a freed tensor's id.
+
+HIP graph capture: a plan build only records its kernels while a stream is
+being captured, so a plan stored then would hold an unfilled workspace until
+the first replay.  Under capture the cache therefore never builds: a cached
+plan is used (and pinned, so neither eviction nor ``clear_plan_cache`` frees a
+workspace the graph points to), and a miss takes the plan-free kernels.
+``record_plans`` lets a capturer keep its own references to the plans its
+graph uses; ``release_plan_pins`` drops the cache's pins.
 """
 
 from __future__ import annotations
@@ -13,6 +21,7 @@ from __future__ import annotations
 import threading
 import weakref
 from collections import OrderedDict
+from contextlib import contextmanager
 from dataclasses import dataclass
 from typing import Optional
 
@@ -22,7 +31,7 @@ from .. import config
 from .._lib import PlanLayout, Tuning, check, f32_array, i64_array, lib, stream_of
 
 __all__ = ["Plan", "get_plan", "build_plan", "plan_layout", "clear_plan_cache",
-           "set_tuning", "get_tuning"]
+           "set_tuning", "get_tuning", "record_plans", "release_plan_pins"]
 
 _MAX_PLANS = 4
 _AUTO_MIN_POINTS = 1 << 16
@@ -113,6 +122,7 @@ class _PlanCache:
     def __init__(self):
         self._plans: OrderedDict = OrderedDict()
         self._seen: OrderedDict = OrderedDict()
+        self._pinned: dict = {}
         self._lock = threading.Lock()
 
     @staticmethod
@@ -159,12 +169,55 @@ class _PlanCache:
             self._plans.clear()
             self._seen.clear()
 
+    def pin(self, plan):
+        with self._lock:
+            self._pinned[id(plan)] = plan
+
+    def release_pins(self):
+        with self._lock:
+            self._pinned.clear()
+
 
 _cache = _PlanCache()
+_recorders = threading.local()
 
 
 def clear_plan_cache() -> None:
+    """Drop the cached plans (plans pinned by a graph capture stay alive)."""
     _cache.clear()
+
+
+def release_plan_pins() -> None:
+    """Drop the cache's references to plans used under graph capture; call
+    once the graphs that recorded them are gone."""
+    _cache.release_pins()
+
+
+@contextmanager
+def record_plans(sink: list):
+    """Append every plan used by this thread while active to ``sink``, so a
+    graph capturer can hold the workspaces its graph reads."""
+    stack = getattr(_recorders, "stack", None)
+    if stack is None:
+        stack = _recorders.stack = []
+    stack.append(sink)
+    try:
+        yield sink
+    finally:
+        stack.pop()
+
+
+def _note(plan: "Plan", capturing: bool) -> "Plan":
+    for sink in getattr(_recorders, "stack", ()):
+        if not any(p is plan for p in sink):
+            sink.append(plan)
+    if capturing:
+        _cache.pin(plan)
+    return plan
+
+
+def _capturing(device) -> bool:
+    return device.type == "cuda" and torch.cuda.is_current_stream_capturing()
 
 
 def get_plan(locs, locs2d, ndim, grid_size, width, norm, pad_code) -> Optional[Plan]:
@@ -177,12 +230,16 @@ def get_plan(locs, locs2d, ndim, grid_size, width, norm, pad_code) -> Optional[P
     npts = locs2d.shape[0]
     if npts == 0 or npts > (1 << 28):
         return None
+    capturing = _capturing(locs2d.device)
     key = _cache.key(locs, ndim, grid_size, width, pad_code)
     plan = _cache.lookup(locs, key)
     if plan is not None:
-        return plan
+        return _note(plan, capturing)
+    if capturing:
+        # a build would only be recorded, not run: take the plan-free kernels
+        return None
     if mode == "auto" and npts < _AUTO_MIN_POINTS and not _cache.seen_before(locs, key):
         return None
     plan = build_plan(locs2d, ndim, grid_size, width, pad_code)
     _cache.store(locs, key, plan)
-    return plan
+    return _note(plan, False)

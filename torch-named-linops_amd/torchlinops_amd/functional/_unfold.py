@@ -5,7 +5,9 @@ Signatures, defaults, shapes and errors follow the reference:
   * ``fold``    src/torchlinops/functional/_unfold/fold.py:32-118, 520-575
   * autograd / norm weights  array_to_blocks.py:16-140
   * ``get_nblocks``          nblocks.py:6-35
-The copy / gather runs in libtnl (tnl_unfold / tnl_fold).
+The copy / gather runs in libtnl (tnl_unfold / tnl_fold) for tensors on a
+HIP device; host tensors take the package's torch restatement (``_host.py``),
+the reference's own dispatch on ``is_cuda`` (unfold.py:94, fold.py:89).
 """
 
 from __future__ import annotations
@@ -16,7 +18,8 @@ from typing import Optional
 import torch
 from torch.autograd import Function
 
-from .._lib import TNL_F32, TNL_F64, check, i64_array, lib, require_device, stream_of
+from .._lib import TNL_F32, TNL_F64, check, i64_array, lib, stream_of
+from ._host import fold_host, unfold_host
 
 __all__ = [
     "get_nblocks",
@@ -87,7 +90,6 @@ def unfold(
             f"Output and input dtypes must match but got output {output.dtype} != "
             f"input {x.dtype}"
         )
-    require_device(x, op="unfold")
     if mask is not None:
         mask = mask.to(x.device)
     full_shape = (*batch_shape, *nblocks, *block_size)
@@ -102,7 +104,12 @@ def unfold(
     is_complex = torch.is_complex(x)
     xr = _elem_view(x.reshape(nbatch, *im_size).contiguous())
     yr = _elem_view(y)
-    if nbatch > 0:
+    if not x.is_cuda:
+        # host tensors: the package's torch restatement (strided copy)
+        if nbatch > 0:
+            y.view(nbatch, *nblocks, *block_size).copy_(
+                unfold_host(x.reshape(nbatch, *im_size), block_size, stride))
+    elif nbatch > 0:
         with torch.cuda.device(x.device):
             check(
                 lib().tnl_unfold(
@@ -170,7 +177,6 @@ def fold(
         raise RuntimeError(
             f"Fold expected input with full size {(*nblocks, *block_size)} but got {x.shape}"
         )
-    require_device(x, op="fold")
     is_complex = torch.is_complex(x)
     real_dtype = x.real.dtype if is_complex else x.dtype
     if real_dtype == torch.float32:
@@ -190,6 +196,13 @@ def fold(
                 f"Output buffer of {output.numel()} elements cannot hold {full_shape}"
             )
         y = output.view(full_shape)
+    if not x.is_cuda:
+        # host tensors: the package's torch restatement (same summation order)
+        if nbatch > 0:
+            y.view(nbatch, *im_size).copy_(
+                fold_host(x.reshape(nbatch, *nblocks, *block_size), im_size, block_size,
+                          stride, nblocks))
+        return y
     xr = _elem_view(x.reshape(nbatch, *nblocks, *block_size).contiguous())
     yr = _elem_view(y)
     if nbatch > 0:

@@ -179,8 +179,14 @@ class NUFFT(Chain):
         return FI.index_map(idx, padded, idx[0].device)
 
     def _fused_ok(self, x) -> bool:
-        return (self._fused_on and x.is_cuda and x.dtype == torch.complex64
-                and not (torch.is_grad_enabled() and x.requires_grad))
+        if not (self._fused_on and x.is_cuda and x.dtype == torch.complex64):
+            return False
+        if not torch.is_grad_enabled():
+            return True
+        # the fused route relabels a detached copy of the trajectory: with a
+        # trainable trajectory (or input) the member chain keeps autograd
+        locs = getattr(self.interp, "locs", None)
+        return not (x.requires_grad or (locs is not None and locs.requires_grad))
 
     def _fused(self, x, forward: bool):
         from .toeplitz import crop_wrapped, pad_wrapped

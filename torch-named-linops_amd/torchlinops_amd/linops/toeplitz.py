@@ -97,11 +97,31 @@ class ToeplitzNormal(NamedLinop):
     def _toeplitz(self, x, conj):
         nd = len(self.im_size)
         dims = tuple(range(-nd, 0))
+        if not x.is_cuda or (torch.is_grad_enabled() and x.requires_grad):
+            return self._toeplitz_torch(x, conj)
         y = pad_wrapped(x, self.im_size, self.pad_size)
         y = torch.fft.fftn(y, dim=dims, norm="backward")  # unscaled
         y.mul_(self.kernel.conj() if conj else self.kernel)
         y = torch.fft.ifftn(y, dim=dims, norm="forward")  # unscaled
         return crop_wrapped(y, self.im_size, self.pad_size)
+
+    def _toeplitz_torch(self, x, conj):
+        """The same operator in differentiable torch ops (autograd, host
+        tensors): centred zero pad, ifftshift, unscaled FFTs around the
+        multiply, fftshift, centred crop."""
+        from .basic import crop_slice_from_pad, pad_to_size
+
+        nd = len(self.im_size)
+        dims = tuple(range(-nd, 0))
+        pad = pad_to_size(self.im_size, self.pad_size)
+        y = torch.nn.functional.pad(x, pad)
+        y = torch.fft.ifftshift(y, dim=dims)
+        y = torch.fft.fftn(y, dim=dims, norm="backward")
+        k = self.kernel.to(y.device)
+        y = y * (k.conj() if conj else k)
+        y = torch.fft.ifftn(y, dim=dims, norm="forward")
+        y = torch.fft.fftshift(y, dim=dims)
+        return y[(Ellipsis, *crop_slice_from_pad(pad))]
 
     @staticmethod
     def fn(t, x, /):
