@@ -132,6 +132,83 @@ def test_toeplitz_normal_matches_explicit_normal():
     assert abs(lhs - rhs) < 1e-4 * abs(lhs)
 
 
+@pytest.mark.parametrize("inner_type", ["simple_inner", "dense_inner", None])
+def test_toeplitz_with_inner_reference_fixture(inner_type):
+    """The reference's Toeplitz fixture (tests/test_toeplitz.py:29-131): 64^3
+    grid, 20 x 500 centred points, width 4, os 1.25, batch A = 2; inners
+    None, a Diagonal broadcast over A, and a Dense coupling A -> A1.  The
+    fused Toeplitz normal equals the reference-shaped chain and A^H inner A
+    (Toeplitz approximation on the 2x grid), passes the adjoint test and
+    keeps no NUFFT."""
+    dev = _gpu()
+    torch.manual_seed(29)
+    grid = (64, 64, 64)
+    locs = ((torch.rand(20, 500, 3) - 0.5) * 60).to(dev)
+    kw = dict(output_shape=("R", "K"), batch_shape=("A",), width=4.0, oversamp=1.25)
+    A = tl.NUFFT(locs.clone(), grid, **kw)
+    T = tl.NUFFT(locs.clone(), grid, toeplitz=True, **kw)
+    Tc = tl.NUFFT(locs.clone(), grid, toeplitz=True, toeplitz_fused=False, **kw)
+    inner = None
+    if inner_type == "simple_inner":
+        inner = tl.Diagonal(torch.randn(20, 500, device=dev), ioshape=("A", "R", "K"),
+                            broadcast_dims=("A",))
+    elif inner_type == "dense_inner":
+        inner = tl.Dense(torch.randn(2, 2, dtype=torch.complex64, device=dev),
+                         weightshape=("A", "A1"), ishape=("A", "R", "K"),
+                         oshape=("A1", "R", "K"))
+    N, Nc = T.normal(inner), Tc.normal(inner)
+    assert type(N).__name__ == "ToeplitzNormal"
+    assert not any(isinstance(m, tl.NUFFT) for m in N.modules())
+    x = torch.randn(2, *grid, dtype=torch.complex64, device=dev)
+    got = N(x)
+    want = A.H(A(x) if inner is None else inner(A(x)))
+    err = rel_l2(got.cpu().numpy(), want.cpu().numpy())
+    print(f"toeplitz[{inner_type}] vs A^H inner A rel-L2: {err:.2e}")
+    assert err < 2e-2
+    assert rel_l2(got.cpu().numpy(), Nc(x).cpu().numpy()) < 1e-5
+    y = torch.randn_like(got)
+    lhs = torch.vdot(y.flatten(), got.flatten())
+    rhs = torch.vdot(N.H(y).flatten(), x.flatten())
+    assert abs(lhs - rhs) < 1e-4 * abs(lhs)
+
+
+def test_toeplitz_normal_backward_matches_chain():
+    """With autograd the fused Toeplitz normal takes its differentiable torch
+    form: x.grad equals the reference-shaped chain's."""
+    dev = _gpu()
+    rng = np.random.default_rng(8)
+    grid = (20, 20, 20)
+    locs = torch.from_numpy((rng.uniform(-0.5, 0.5, (1500, 3)) * 20).astype(np.float32)).to(dev)
+    kw = dict(output_shape=("K",), batch_shape=("C",), oversamp=2.0, toeplitz=True)
+    At = tl.NUFFT(locs, grid, **kw)
+    Ac = tl.NUFFT(locs, grid, toeplitz_fused=False, **kw)
+    x0 = torch.randn(2, *grid, dtype=torch.complex64, device=dev)
+    xa = x0.clone().requires_grad_(True)
+    xb = x0.clone().requires_grad_(True)
+    At.N(xa).abs().pow(2).sum().backward()
+    Ac.N(xb).abs().pow(2).sum().backward()
+    assert xa.grad is not None
+    assert rel_l2(xa.grad.cpu().numpy(), xb.grad.cpu().numpy()) < 1e-5
+
+
+def test_fused_nufft_keeps_trajectory_gradient():
+    """A trainable trajectory (locs.requires_grad) sends the NUFFT through
+    its member chain, whose locs gradient the fused route would drop."""
+    dev = _gpu()
+    rng = np.random.default_rng(9)
+    grid = (32, 32)
+    locs = torch.from_numpy((rng.uniform(-0.5, 0.5, (800, 2)) * 32).astype(np.float32)).to(dev)
+    A = tl.NUFFT(locs.clone(), grid, output_shape=("K",), oversamp=2.0)
+    B = tl.NUFFT(locs.clone(), grid, output_shape=("K",), oversamp=2.0, fused=False)
+    x = torch.randn(*grid, dtype=torch.complex64, device=dev)
+    for op in (A, B):
+        op.interp.locs.requires_grad_(True)
+        op(x).abs().pow(2).sum().backward()
+    ga, gb = A.interp.locs.grad, B.interp.locs.grad
+    assert ga is not None and torch.isfinite(ga).all()
+    assert rel_l2(ga.cpu().numpy(), gb.cpu().numpy()) < 1e-5
+
+
 @pytest.mark.parametrize("im,pad,dtype", [
     ((24, 24, 24), (48, 48, 48), torch.complex64),
     ((5, 6, 7), (10, 12, 14), torch.complex128),

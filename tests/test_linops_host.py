@@ -330,3 +330,61 @@ def test_polynomial_preconditioner_linop_on_a_diagonal():
         P.l_inf_opt(3, 0.0, 1.0)
     with pytest.raises(ValueError, match="Unknown"):
         P.polynomial_preconditioner(T, 2, "l_1")
+
+
+@pytest.mark.parametrize("inner_type", ["simple_inner", "dense_inner", None])
+def test_toeplitz_normal_with_inner(inner_type):
+    """The reference's Toeplitz cases (tests/test_toeplitz.py:62-131) on host
+    tensors: a Diagonal inner broadcast over the batch and a Dense inner
+    that couples the two batch entries (A -> A1).  The Toeplitz normal must
+    hold no NUFFT, equal A^H inner A up to the Toeplitz approximation, equal
+    the reference-shaped chain (toeplitz_fused=False) and pass the adjoint
+    test."""
+    from tests._util import rel_l2
+
+    torch.manual_seed(11)
+    n, R, K = (16, 16), 4, 100
+    locs = (torch.rand(R, K, 2) - 0.5) * 16
+    kw = dict(output_shape=("R", "K"), batch_shape=("A",), oversamp=2.0)
+    A = tl.NUFFT(locs.clone(), n, **kw)
+    T = tl.NUFFT(locs.clone(), n, toeplitz=True, **kw)
+    Tc = tl.NUFFT(locs.clone(), n, toeplitz=True, toeplitz_fused=False, **kw)
+    inner = None
+    if inner_type == "simple_inner":
+        inner = tl.Diagonal(torch.randn(R, K), ioshape=("A", "R", "K"), broadcast_dims=("A",))
+    elif inner_type == "dense_inner":
+        inner = tl.Dense(torch.randn(2, 2, dtype=torch.complex64), weightshape=("A", "A1"),
+                         ishape=("A", "R", "K"), oshape=("A1", "R", "K"))
+    N, Nc = T.normal(inner), Tc.normal(inner)
+    assert not any(isinstance(m, tl.NUFFT) for m in N.modules())
+    if inner_type == "dense_inner":
+        assert tuple(map(str, N.oshape))[0] == "A1"
+    x = torch.randn(2, *n, dtype=torch.complex64)
+    want = A.H(A(x) if inner is None else inner(A(x)))
+    got = N(x)
+    assert rel_l2(got.numpy(), want.numpy()) < 2e-3
+    assert rel_l2(got.numpy(), Nc(x).numpy()) < 1e-5
+    y = torch.randn(2, *n, dtype=torch.complex64)
+    lhs = torch.vdot(y.flatten(), got.flatten())
+    rhs = torch.vdot(N.H(y).flatten(), x.flatten())
+    assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+def test_toeplitz_psf_shapes_and_errors():
+    """psf_sizing / all_indices (nufft.py:501-541): a coupling kernel is a
+    Dense over (A, A1, k-space); an inner whose in and out shapes differ in
+    length, or whose batch dims are both unspecified, is rejected."""
+    from torchlinops_amd.linops.nufft import all_indices, toeplitz_psf
+
+    assert list(all_indices((2, 3))) == [(0, 0), (0, 1), (0, 2), (1, 0), (1, 1), (1, 2)]
+    locs = (torch.rand(3, 20, 2) - 0.5) * 8
+    A = tl.NUFFT(locs, (8, 8), output_shape=("R", "K"), batch_shape=("A",), oversamp=2.0)
+    dense = tl.Dense(torch.randn(2, 2, dtype=torch.complex64), weightshape=("A", "A1"),
+                     ishape=("A", "R", "K"), oshape=("A1", "R", "K"))
+    k = toeplitz_psf(A, dense)
+    assert isinstance(k, tl.Dense) and tuple(k.weight.shape) == (2, 2, 16, 16)
+    assert tuple(map(str, k.weightshape))[:2] == ("A", "A1")
+    bad = tl.Dense(torch.randn(2, dtype=torch.complex64), weightshape=("A",),
+                   ishape=("A", "R", "K"), oshape=("R", "K"))
+    with pytest.raises(ValueError, match="identical input and output shape lengths"):
+        toeplitz_psf(A, bad)

@@ -20,6 +20,7 @@ The interpolation runs on the HIP kernels through Interpolate.
 from __future__ import annotations
 
 from copy import copy
+from itertools import product
 from math import prod
 from typing import Optional
 
@@ -27,7 +28,8 @@ import torch
 import torch.nn as nn
 
 from ..nameddim import ND, NamedShape as NS, get_nd_shape
-from .basic import FFT, Diagonal, Identity, Pad, Scalar, crop_slice_from_pad, pad_to_size
+from .basic import (FFT, Dense, Diagonal, Identity, Pad, Scalar, crop_slice_from_pad,
+                    pad_to_size)
 from .chain import Chain
 from .interp import Interpolate
 from .sampling import Sampling
@@ -262,8 +264,16 @@ class NUFFT(Chain):
                       in_shape=self.input_shape, batch_shape=self.batch_shape)
             if not self.options.get("toeplitz_fused", True):
                 return pad.normal(self.fft.normal(kernel))
-            return ToeplitzNormal(kernel.weight, self.grid_size, scale_int(self.grid_size, os_),
-                                  pad.ishape)
+            padded = scale_int(self.grid_size, os_)
+            if isinstance(kernel, Dense):
+                # batch-coupling inner: image dims after the kernel's batch dims
+                nd = len(self.grid_size)
+                im_dims = tuple(pad.ishape[len(pad.ishape) - nd:])
+                bi = tuple(kernel.ishape[: len(kernel.ishape) - nd])
+                bo = tuple(kernel.oshape[: len(kernel.oshape) - nd])
+                return ToeplitzNormal(kernel, self.grid_size, padded, bi + im_dims,
+                                      oshape=bo + im_dims)
+            return ToeplitzNormal(kernel.weight, self.grid_size, padded, pad.ishape)
         normal = super().normal(inner)
         if inner is None and self._fused_on and self._fused_dir > 0:
             nufft = self
@@ -334,14 +344,50 @@ def _rescale_locs(locs, c0, w0, c1, w1):
     return torch.stack(cols, dim=-1)
 
 
+def psf_sizing(nufft: NUFFT, inner, toeplitz_oversamp: float = 2.0):
+    """Shapes and sizes of a Toeplitz kernel around ``inner`` (nufft.py:501-536).
+
+    Returns ``(weightshape, ishape, oshape, kernel_size, input_size,
+    batch_sizes)``: the inner's batch dims in and out (its shape minus the
+    NUFFT's output dims) followed by the k-space grid dims; the kernel holds
+    one PSF per (input batch index, output batch index) pair."""
+    nout = len(nufft.output_shape)
+    bi = tuple(inner.ishape[: len(inner.ishape) - nout])
+    bo = tuple(inner.oshape[: len(inner.oshape) - nout])
+    kshape = tuple(nufft.fft._shape.output_grid_shape)
+    if tuple(str(d) for d in bi) == ("...",):
+        weightshape = bi + kshape
+    elif "..." in [str(d) for d in bi] and "..." in [str(d) for d in bo]:
+        raise ValueError(
+            f"Underspecified kernel shape for toeplitz embedding with inner.shape = "
+            f"{inner.shape}. Specify more dimensions of inner to avoid this.")
+    else:
+        weightshape = bi + bo + kshape
+    sizes = tuple(inner.size(d) for d in bi)
+    sizes = tuple(1 if n is None else int(n) for n in sizes)
+    kernel_size = sizes + sizes + scale_int(nufft.pad.im_size, toeplitz_oversamp)
+    input_size = sizes + tuple(nufft.interp.locs.shape[:-1])
+    return weightshape, bi + kshape, bo + kshape, kernel_size, input_size, sizes
+
+
+def all_indices(sizes):
+    """Every index tuple of an array of shape ``sizes``, lexicographic."""
+    return product(*(range(n) for n in sizes))
+
+
 def toeplitz_psf(nufft: NUFFT, inner=None, dtype=None, oversamp: float = 2.0):
     """Fourier-domain point spread kernel K with A^H inner A =
-    Crop F^H (K *) F Pad on a ``oversamp``-times larger grid.
+    Crop F^H (K *) F Pad on an ``oversamp``-times larger grid
+    (nufft.py:395-494).
 
-    The PSF is one adjoint NUFFT of ``inner(ones)`` on the enlarged grid
-    (nufft.py:439-491).  Supported inners: None / Identity and linops that
-    do not couple batch entries (all batch dims broadcast); the kernel is
-    then shared by every batch entry and applied as a Diagonal.
+    Each PSF is one adjoint NUFFT of ``inner`` applied to a unit batch
+    entry, on the enlarged grid with rescaled locations.  An inner that
+    leaves batch entries apart (None, Identity, a Diagonal broadcast over
+    the batch) needs one PSF, shared by every batch entry: the kernel is a
+    Diagonal.  An inner that couples batch entries (a Dense over a batch
+    axis) needs one PSF per input batch index, each holding the response
+    at every output batch index: the kernel is a Dense over (input batch,
+    output batch, k-space) whose einsum does the coupling.
     """
     if isinstance(nufft.interp, Sampling):
         raise NotImplementedError(
@@ -360,20 +406,32 @@ def toeplitz_psf(nufft: NUFFT, inner=None, dtype=None, oversamp: float = 2.0):
                      input_shape=nufft.input_shape, input_kshape=nufft.input_kshape,
                      batch_shape=("...",), oversamp=nufft.oversamp, width=nufft.width,
                      do_prep_locs=False)
-    out_sizes = tuple(nufft.interp.locs.shape[:-1])
-    ones = torch.ones(out_sizes, dtype=dtype, device=device)
-    if inner is not None and not isinstance(inner, Identity):
-        for d in inner.ishape:
-            if d in nufft.output_shape or str(d) == "...":
-                continue
-            if inner.size(d) not in (None, 1):
-                raise NotImplementedError(
-                    "toeplitz_psf: inner linops that couple batch entries are not supported"
-                )
-        ones = inner(ones)
-    otf = nufft_os.H(ones)
     dims = tuple(range(-len(new_grid), 0))
-    kernel = cfftn(otf, dim=dims, norm=None) * (oversamp ** len(grid_size) / prod(new_grid) ** 0.5)
-    kernel = kernel.to(dtype)
+    scale = oversamp ** len(grid_size) / prod(new_grid) ** 0.5
     kshape = tuple(nufft.fft._shape.output_grid_shape)
-    return Diagonal(kernel, ioshape=tuple(nufft.batch_shape) + kshape)
+
+    def psf(v):
+        return cfftn(nufft_os.H(v), dim=dims, norm=None) * scale
+
+    if inner is None or isinstance(inner, Identity):
+        kernel = psf(torch.ones(tuple(nufft.interp.locs.shape[:-1]), dtype=dtype,
+                                device=device)).to(dtype)
+        return Diagonal(kernel, ioshape=tuple(nufft.batch_shape) + kshape)
+    if len(inner.ishape) != len(inner.oshape):
+        raise ValueError(
+            f"Inner linop must have identical input and output shape lengths but got "
+            f"ishape={inner.ishape} and oshape={inner.oshape}")
+    weightshape, ishape, oshape, kernel_size, input_size, sizes = psf_sizing(
+        nufft, inner, oversamp)
+    if prod(sizes) == 1:
+        # no batch coupling: one PSF of inner(ones), shared by every entry
+        ones = torch.ones(tuple(nufft.interp.locs.shape[:-1]), dtype=dtype, device=device)
+        kernel = psf(inner(ones)).to(dtype)
+        return Diagonal(kernel, ioshape=tuple(nufft.batch_shape) + kshape)
+    kernel = torch.zeros(kernel_size, dtype=dtype, device=device)
+    unit = torch.zeros(input_size, dtype=dtype, device=device)
+    for b in all_indices(sizes):
+        unit[b] = 1.0
+        kernel[b] = psf(inner(unit)).to(dtype)
+        unit[b] = 0.0
+    return Dense(kernel, weightshape=weightshape, ishape=ishape, oshape=oshape)

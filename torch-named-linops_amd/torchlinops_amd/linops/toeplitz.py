@@ -80,19 +80,40 @@ def crop_wrapped(y: torch.Tensor, im_size, pad_size, weight=None, out=None) -> t
 class ToeplitzNormal(NamedLinop):
     """``Crop . F^H . (K *) . F . Pad`` applied as pad_wrapped -> fftn ->
     multiply by ifftshift(K) -> ifftn -> crop_wrapped.  ``kernel`` is the
-    centred Fourier-domain PSF ``[..., *pad_size]`` of ``toeplitz_psf``;
-    the adjoint multiplies by its conjugate."""
+    centred Fourier-domain PSF of ``toeplitz_psf``: a tensor
+    ``[..., *pad_size]`` multiplied elementwise, or a ``Dense`` over (input
+    batch, output batch, k-space) whose einsum couples the batch entries
+    (then ``oshape`` names the output batch dims).  The adjoint uses the
+    conjugate kernel."""
 
-    def __init__(self, kernel: torch.Tensor, im_size, pad_size, ishape, name="ToeplitzNormal"):
-        super().__init__(NS(tuple(ishape), tuple(ishape)), name=name)
+    def __init__(self, kernel, im_size, pad_size, ishape, name="ToeplitzNormal", oshape=None):
+        from .basic import Dense
+
+        oshape = tuple(ishape) if oshape is None else tuple(oshape)
+        super().__init__(NS(tuple(ishape), oshape), name=name)
         self.im_size = tuple(int(n) for n in im_size)
         self.pad_size = tuple(int(n) for n in pad_size)
         dims = tuple(range(-len(self.im_size), 0))
         # the two orthonormal FFT scalings (1/sqrt(P) each) are folded into
         # K so that both transforms run unscaled (no extra pass each)
-        self.kernel = nn.Parameter(
-            (torch.fft.ifftshift(kernel, dim=dims) / prod(self.pad_size)).contiguous(),
-            requires_grad=False)
+        if isinstance(kernel, Dense):
+            w = (torch.fft.ifftshift(kernel.weight, dim=dims) / prod(self.pad_size)).contiguous()
+            self.coupling = Dense(w, kernel.weightshape, kernel.ishape, kernel.oshape)
+            self.kernel = self.coupling.weight
+        else:
+            self.coupling = None
+            self.kernel = nn.Parameter(
+                (torch.fft.ifftshift(kernel, dim=dims) / prod(self.pad_size)).contiguous(),
+                requires_grad=False)
+
+    def _multiply(self, y, conj, inplace):
+        if self.coupling is not None:
+            from .basic import Dense
+
+            return Dense.adj_fn(self.coupling, y) if conj else Dense.fn(self.coupling, y)
+        k = self.kernel.to(y.device)
+        k = k.conj() if conj else k
+        return y.mul_(k) if inplace else y * k
 
     def _toeplitz(self, x, conj):
         nd = len(self.im_size)
@@ -101,7 +122,7 @@ class ToeplitzNormal(NamedLinop):
             return self._toeplitz_torch(x, conj)
         y = pad_wrapped(x, self.im_size, self.pad_size)
         y = torch.fft.fftn(y, dim=dims, norm="backward")  # unscaled
-        y.mul_(self.kernel.conj() if conj else self.kernel)
+        y = self._multiply(y, conj, inplace=True)
         y = torch.fft.ifftn(y, dim=dims, norm="forward")  # unscaled
         return crop_wrapped(y, self.im_size, self.pad_size)
 
@@ -117,8 +138,7 @@ class ToeplitzNormal(NamedLinop):
         y = torch.nn.functional.pad(x, pad)
         y = torch.fft.ifftshift(y, dim=dims)
         y = torch.fft.fftn(y, dim=dims, norm="backward")
-        k = self.kernel.to(y.device)
-        y = y * (k.conj() if conj else k)
+        y = self._multiply(y, conj, inplace=False)
         y = torch.fft.ifftn(y, dim=dims, norm="forward")
         y = torch.fft.fftshift(y, dim=dims)
         return y[(Ellipsis, *crop_slice_from_pad(pad))]
