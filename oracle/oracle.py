@@ -407,8 +407,9 @@ def nufft_sampling_adjoint(y, locs, grid_size, oversamp=1.25, prep=True):
     return center_crop(cfftn(g, ndim, inverse=True), padded, grid_size)
 
 
-def nufft(x, locs, oversamp=1.25, width=4.0):
-    """Forward NUFFT in float64: Scale * Interp(FFT_c(Pad(apod * x)))."""
+def nufft(x, locs, oversamp=1.25, width=4.0, nthreads=1):
+    """Forward NUFFT in float64: Scale * Interp(FFT_c(Pad(apod * x)))
+    (``nthreads`` threads for the interpolation)."""
     ndim = locs.shape[-1]
     grid_size = x.shape[x.ndim - ndim:]
     padded = tuple(int(n * oversamp) for n in grid_size)
@@ -417,7 +418,7 @@ def nufft(x, locs, oversamp=1.25, width=4.0):
     y = center_pad(x * apod, grid_size, padded)
     y = cfftn(y, ndim)
     lp = prep_locs(locs, grid_size, padded)
-    y = ungrid(y, lp, width, beta=beta, precision="f64")
+    y = ungrid(y, lp, width, beta=beta, precision="f64", nthreads=nthreads)
     scale = width**ndim * (prod(grid_size) / prod(padded)) ** 0.5
     return y / scale
 
