@@ -24,30 +24,44 @@ from .namedlinop import NamedLinop
 __all__ = ["Sampling"]
 
 
+def _names(shape) -> tuple:
+    """A named-dim tuple, ``("...",)`` when unspecified."""
+    return ("...",) if shape is None else tuple(shape)
+
+
+def _checked_indices(idx, input_size: tuple) -> list:
+    """Integer index tensors (slices expanded), each within ``[0, size-1]``
+    of its axis; ValueError texts as the reference's constructor."""
+    if len(idx) != len(input_size):
+        raise ValueError(
+            f"Input size {input_size} doesn't match index with length {len(idx)}."
+        )
+    out = list(FI.ensure_tensor_indexing(idx, input_size))
+    for axis, (t, size) in enumerate(zip(out, input_size)):
+        lo, hi = (int(v) for v in torch.aminmax(t)) if t.numel() else (0, 0)
+        if lo < 0 or hi >= size:
+            raise ValueError(
+                f"Sampling index must lie within range [0, {size - 1}] but got range "
+                f"[{lo}, {hi}] for dim {axis}"
+            )
+    return out
+
+
 class Sampling(NamedLinop):
+    """``(batch, input) -> (batch, output)``: the input read at one integer
+    index tensor per sampled axis."""
+
     def __init__(self, idx, input_size, output_shape=None, input_shape=None,
                  batch_shape=None):
-        dim = len(input_size)
-        if len(idx) != dim:
-            raise ValueError(
-                f"Input size {input_size} doesn't match index with length {len(idx)}."
-            )
         self.input_size = tuple(int(s) for s in input_size)
-        batch_shape = tuple(batch_shape) if batch_shape is not None else ("...",)
-        input_shape = tuple(input_shape) if input_shape is not None else ("...",)
-        output_shape = tuple(output_shape) if output_shape is not None else ("...",)
-        super().__init__(NS(batch_shape) + NS(input_shape, output_shape))
-        self._shape.batch_shape = batch_shape
-        self._shape.input_shape = input_shape
-        self._shape.output_shape = output_shape
-        idx = FI.ensure_tensor_indexing(idx, self.input_size)
-        for d, (t, s) in enumerate(zip(idx, self.input_size)):
-            if (t < 0).any() or (t >= s).any():
-                raise ValueError(
-                    f"Sampling index must lie within range [0, {s - 1}] but got range "
-                    f"[{t.min().item()}, {t.max().item()}] for dim {d}"
-                )
-        self.idx = nn.ParameterList([nn.Parameter(i, requires_grad=False) for i in idx])
+        checked = _checked_indices(idx, self.input_size)
+        dims = {"batch_shape": _names(batch_shape), "input_shape": _names(input_shape),
+                "output_shape": _names(output_shape)}
+        super().__init__(NS(dims["batch_shape"])
+                         + NS(dims["input_shape"], dims["output_shape"]))
+        for name, names in dims.items():
+            self._shape[name] = names
+        self.idx = nn.ParameterList([nn.Parameter(t, requires_grad=False) for t in checked])
         self._maps = {}
 
     @property
@@ -83,33 +97,33 @@ class Sampling(NamedLinop):
 
     @staticmethod
     def split(sampling, tile):
-        ibatch = tuple(tile.get(d, slice(None)) for d in sampling.ishape)
-        obatch = tuple(tile.get(d, slice(None)) for d in sampling.oshape)
-        if tuple(sampling._shape.output_shape) == (ELLIPSES,):
+        """A tile over the output (point) dims selects those points of every
+        index tensor; with unnamed output dims the linop cannot be split."""
+        out_dims = tuple(sampling._shape.output_shape)
+        if out_dims == (ELLIPSES,):
             return sampling
+        obatch = tuple(tile.get(d, slice(None)) for d in sampling.oshape)
         return type(sampling)(
-            sampling.split_idx(ibatch, obatch, sampling.idx),
+            sampling.split_idx(None, obatch, sampling.idx),
             sampling.input_size,
-            sampling._shape.output_shape,
+            out_dims,
             sampling._shape.input_shape,
             sampling._shape.batch_shape,
         )
 
     def split_idx(self, ibatch, obatch, idx):
-        n = len(idx[0].shape)
-        if n > 0:
-            sl = tuple(obatch[-n:])
-            return [i[sl] for i in idx]
-        return idx
+        """Slice each index tensor by the trailing entries of ``obatch``
+        (one per index-tensor axis)."""
+        point_axes = idx[0].ndim
+        if point_axes == 0:
+            return idx
+        sel = tuple(obatch)[len(obatch) - point_axes:]
+        return [t[sel] for t in idx]
 
     def register_shape(self, name, shape: tuple):
         self._shape[name] = shape
 
     def size(self, dim):
-        out = list(self._shape.output_shape)
-        inp = list(self._shape.input_shape)
-        if dim in out:
-            return self.locs.shape[out.index(dim)]
-        if dim in inp:
-            return self.input_size[inp.index(dim)]
-        return None
+        sizes = dict(zip(self._shape.input_shape, self.input_size))
+        sizes.update(zip(self._shape.output_shape, self.locs.shape[:-1]))
+        return sizes.get(dim)
