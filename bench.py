@@ -53,6 +53,20 @@ def radial_traj(nspokes, nread, n, gen, device):
     return torch.remainder(t, n).to(device).contiguous()
 
 
+def host_threads() -> int:
+    """Host threads for the CPU legs: every core this process may use (its
+    CPU affinity), capped by OMP_NUM_THREADS when the environment sets it
+    (a shared GPU box allots each GPU a share of its cores that way)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    if cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
+
+
 def cpu_baseline(vals_c, locs, y, beta, target_s=6.0):
     """Oracle (C restatement of the reference algorithm) on a bounded sample
     of points of the same workload: all coils, the full grid."""
@@ -60,7 +74,7 @@ def cpu_baseline(vals_c, locs, y, beta, target_s=6.0):
 
     from oracle import oracle as O
 
-    threads = max(1, min(8, os.cpu_count() or 1))
+    threads = host_threads()
     C = vals_c.shape[0]
     grid_size = vals_c.shape[1:]
     m = 1 << 12
@@ -83,7 +97,142 @@ def cpu_baseline(vals_c, locs, y, beta, target_s=6.0):
         "kind": "port",
         "sample": f"{m} of {locs.shape[0]} points x {C} coils, full 256^3 grid; "
                   f"ungrid {tu:.2f} s + grid {tg:.2f} s (oracle C restatement, "
-                  f"{threads} OpenMP threads)",
+                  f"{threads} OpenMP threads of {os.cpu_count()} host CPUs)",
+        "host_cpus": os.cpu_count(),
+    }
+
+
+def blocks_leg(dev, steps: int = 20):
+    """array_to_blocks / blocks_to_array at the 3-D real size: 8 x 128^3
+    complex64 (iid CN(0,1)), 8^3 blocks, stride 4 -> 31^3 blocks per batch
+    entry.  Algorithmic bytes per call (either direction):
+    C * prod(im) * 8 + C * prod(nblocks) * prod(block) * 8 =
+    134 217 728 + 976 191 488 = 1 110 409 216.  Both results are compared
+    with the blocks / image they came from (unfold is a copy; fold of
+    unfold equals the image times the block-coverage counts)."""
+    import torch
+
+    import torchlinops_amd.functional as F
+
+    C, n, blk, st = 8, 128, (8, 8, 8), (4, 4, 4)
+    gen = torch.Generator(device=dev).manual_seed(9)
+    x = torch.randn(C, n, n, n, dtype=torch.complex64, device=dev, generator=gen)
+    b = F.array_to_blocks(x, blk, st)
+    f = F.blocks_to_array(b, (n, n, n), blk, st)
+    for _ in range(2):
+        F.array_to_blocks(x, blk, st)
+        F.blocks_to_array(b, (n, n, n), blk, st)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    torch.cuda.synchronize()
+    for e in ev:
+        e[0].record()
+        F.array_to_blocks(x, blk, st)
+        e[1].record()
+        F.blocks_to_array(b, (n, n, n), blk, st)
+        e[2].record()
+    torch.cuda.synchronize()
+    tu = sum(e[0].elapsed_time(e[1]) for e in ev) / steps * 1e-3
+    tf = sum(e[1].elapsed_time(e[2]) for e in ev) / steps * 1e-3
+    # practical ceilings on this part: torch's copy of the blocks tensor
+    # (976 MB read + written) and a write-only fill of it
+    c = torch.empty_like(b)
+    for _ in range(2):
+        c.copy_(b)
+        c.fill_(1.0)
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        c.copy_(b)
+    e1.record()
+    for _ in range(steps):
+        c.fill_(1.0)
+    e2.record()
+    torch.cuda.synchronize()
+    t_copy = e0.elapsed_time(e1) / steps * 1e-3
+    t_fill = e1.elapsed_time(e2) / steps * 1e-3
+    del c
+    nb = (n - blk[0]) // st[0] + 1
+    bytes_ = C * n**3 * 8 + C * nb**3 * 512 * 8
+    # checks: the first block's first row is the image's corner row; fold of
+    # unfold = image x coverage count (8 in the interior for 8^3 / stride 4)
+    ok_unfold = bool(torch.equal(b[0, 0, 0, 0, 0, 0], x[0, 0, 0, :8]))
+    cnt = F.blocks_to_array(F.array_to_blocks(torch.ones(1, n, n, n, device=dev), blk, st),
+                            (n, n, n), blk, st)
+    ok_fold = bool(torch.allclose(f, x * cnt, rtol=1e-6, atol=1e-6))
+    return {
+        "workload": "array_to_blocks / blocks_to_array, 8 x 128^3 complex64, 8^3 blocks, "
+                    "stride 4 (31^3 blocks)",
+        "algorithmic_bytes": bytes_,
+        "unfold_us": tu * 1e6,
+        "fold_us": tf * 1e6,
+        "unfold_GBps": bytes_ / tu / 1e9,
+        "fold_GBps": bytes_ / tf / 1e9,
+        "unfold_frac": bytes_ / tu / HBM_PEAK,
+        "fold_frac": bytes_ / tf / HBM_PEAK,
+        "checks_pass": ok_unfold and ok_fold,
+        "torch_copy_blocks_us": t_copy * 1e6,
+        "torch_copy_GBps": 2 * b.numel() * 8 / t_copy / 1e9,
+        "torch_fill_blocks_us": t_fill * 1e6,
+        "torch_fill_GBps": b.numel() * 8 / t_fill / 1e9,
+        "reference_rtx3090_us": {"unfold_128^3_c1": 397, "fold_128^3_c1": 340},
+    }
+
+
+def c1_leg(dev, steps: int = 20):
+    """The 2-D CPU-parity workload: 128 x 128 complex64 grid, 16 384 uniform
+    points in [0, 128)^2, KB width 4, beta pi sqrt(8.2); ArrayToBlocks on
+    1 x 128 x 128 complex64, 8 x 8 blocks, stride 4.  Timed on the GPU
+    (HIP kernels, launch-bound at this size) and on the host through the
+    same public functions (the package's torch path for host tensors), with
+    torch using every host core."""
+    import numpy as np
+    import torch
+
+    import torchlinops_amd.functional as F
+
+    gen = torch.Generator().manual_seed(1)
+    x = torch.randn(128, 128, dtype=torch.complex64, generator=gen)
+    locs = torch.rand(16384, 2, generator=gen) * 128
+    y = torch.randn(16384, dtype=torch.complex64, generator=gen)
+    a = torch.randn(1, 128, 128, dtype=torch.complex64, generator=gen)
+    kp = {"beta": float(np.pi * np.sqrt(8.2))}
+
+    def run(xx, ll, yy, aa):
+        u = F.interpolate(xx, ll, 4.0, kernel_params=kp)
+        g = F.interpolate_adjoint(yy, ll, (128, 128), 4.0, kernel_params=kp)
+        b = F.array_to_blocks(aa, (8, 8), (4, 4))
+        f = F.blocks_to_array(b, (128, 128), (8, 8), (4, 4))
+        return u, g, b, f
+
+    xd, ld, yd, ad = (t.to(dev) for t in (x, locs, y, a))
+    for _ in range(3):
+        run(xd, ld, yd, ad)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        outs_d = run(xd, ld, yd, ad)
+    e1.record()
+    torch.cuda.synchronize()
+    gpu_ms = e0.elapsed_time(e1) / steps
+    threads = host_threads()
+    torch.set_num_threads(threads)
+    run(x, locs, y, a)
+    t0 = time.perf_counter()
+    n_cpu = 3
+    for _ in range(n_cpu):
+        outs_h = run(x, locs, y, a)
+    cpu_ms = (time.perf_counter() - t0) / n_cpu * 1e3
+    agree = [float(torch.linalg.vector_norm(d.cpu() - h) / torch.linalg.vector_norm(h))
+             for d, h in zip(outs_d, outs_h)]
+    return {
+        "workload": "2-D 128^2 c64, 16384 uniform points, KB w4: interpolate + "
+                    "interpolate_adjoint + array_to_blocks + blocks_to_array (8^2, stride 4)",
+        "gpu_ms": gpu_ms,
+        "host_ms": cpu_ms,
+        "host_threads": threads,
+        "gpu_vs_host_rel_l2": agree,
     }
 
 
@@ -477,6 +626,9 @@ def main() -> int:
                     help="skip the sampling-mode gather / scatter-add leg")
     ap.add_argument("--no-toeplitz", action="store_true",
                     help="skip the NUFFT-chain and Toeplitz normal legs")
+    ap.add_argument("--no-blocks", action="store_true",
+                    help="skip the array_to_blocks / blocks_to_array leg")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 2-D CPU-parity leg")
     ap.add_argument("--small", action="store_true", help="reduced size (debug only)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one elementwise copy of the 8 x 256^3 complex64 grid "
@@ -616,6 +768,19 @@ def main() -> int:
             vals.cpu().numpy(), locs.cpu().numpy(), ksp.cpu().numpy(), beta
         )
         log(f"cpu baseline: {out['cpu_baseline']['sample']}")
+    if not args.no_blocks and not args.small:
+        log("blocks: array_to_blocks / blocks_to_array ...")
+        out["blocks"] = blocks_leg(dev)
+        log(f"blocks: unfold {out['blocks']['unfold_us']:.1f} us "
+            f"({out['blocks']['unfold_frac'] * 100:.1f} %), fold {out['blocks']['fold_us']:.1f} us "
+            f"({out['blocks']['fold_frac'] * 100:.1f} %), checks {out['blocks']['checks_pass']}; "
+            f"torch copy of the blocks {out['blocks']['torch_copy_GBps']:.0f} GB/s, "
+            f"fill {out['blocks']['torch_fill_GBps']:.0f} GB/s")
+    if not args.no_c1 and not args.small:
+        log("c1: 2-D CPU-parity workload on the GPU and on the host ...")
+        out["c1"] = c1_leg(dev)
+        log(f"c1: GPU {out['c1']['gpu_ms']:.3f} ms, host {out['c1']['host_ms']:.1f} ms "
+            f"({out['c1']['host_threads']} threads)")
     if not args.no_sampling and not args.small:
         log("sampling: gather / scatter-add ...")
         out["sampling"] = sampling_leg(dev, locs, C, n)

@@ -5,9 +5,10 @@
 // when the sizes, strides and pointers allow it) so every lane issues
 // dwordx4 loads/stores.
 //
-// unfold: block-major.  A workgroup walks whole blocks; each lane owns fixed
-//         element slots inside a block, so the in-block offsets are computed
-//         once per lane and reused for every block the workgroup visits.
+// unfold: output-major copy, 32-bit decode of each output vector when the
+//         sizes allow (UN vectors in flight per thread), else block-major:
+//         a workgroup walks whole blocks and each lane owns fixed element
+//         slots inside a block, its in-block offsets computed once.
 // fold:   output-major gather.  Each lane owns V consecutive output elements
 //         and adds the covering blocks' values in lexicographic block order,
 //         which reproduces the reference's sequential `out[slc] += x[blk]`
@@ -170,6 +171,158 @@ __global__ __launch_bounds__(256) void fold_kernel(const S* __restrict__ x,
   }
 }
 
+// ---------------------------------------------------------------------------
+// 32-bit variants, taken when every element index of the input and output
+// is below 2^31 (one 8 x 128^3 complex64 image and its 31^3 blocks of 8^3
+// are 1.3e8 elements).  Indices are decoded with divisions by invariant
+// divisors done as a multiply-high and a shift (magic numbers computed on
+// the host, exact for every numerator below 2^31), where the 64-bit kernels
+// above divide in software.  Each thread moves UN vectors per iteration,
+// spaced a grid apart, so UN loads are in flight before the stores.
+
+struct FDiv {
+  uint32_t d, m, l;  // n / d == (mulhi(n, m) + n) >> l for n < 2^31
+};
+
+static FDiv make_fdiv(uint32_t d) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  const uint64_t m = ((1ull << 32) * ((1ull << l) - d)) / d + 1;
+  return FDiv{d, (uint32_t)m, l};
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FDiv& f) {
+  return (uint32_t)(((uint64_t)__umulhi(n, f.m) + n) >> f.l);
+}
+
+struct BlockGeom32 {
+  FDiv vpb, B12, B2, nbb, nb12, nb2;  // unfold decode
+  FDiv img, S0, S1, s[3];             // fold decode
+  uint32_t B[3], nb[3], im[3], st[3];
+  uint32_t S0v, S1v, imgv, bsz;
+  uint32_t total;  // vectors of the output
+};
+
+constexpr int UN = 4;  // vectors per thread per iteration
+
+template <typename T>
+__global__ __launch_bounds__(256) void unfold32_kernel(const T* __restrict__ x,
+                                                       T* __restrict__ y, BlockGeom32 g) {
+  const uint32_t stride = gridDim.x * 256u;
+  for (uint32_t o0 = blockIdx.x * 256u + threadIdx.x; o0 < g.total; o0 += stride * UN) {
+    T v[UN];
+    uint32_t oo[UN];
+#pragma unroll
+    for (int u = 0; u < UN; ++u) {
+      const uint32_t o = o0 + (uint32_t)u * stride;
+      oo[u] = o;
+      if (o < g.total) {
+        const uint32_t blk = fdiv(o, g.vpb);
+        const uint32_t e = o - blk * g.vpb.d;
+        const uint32_t j0 = fdiv(e, g.B12);
+        const uint32_t r = e - j0 * g.B12.d;
+        const uint32_t j1 = fdiv(r, g.B2);
+        const uint32_t j2 = r - j1 * g.B2.d;
+        const uint32_t n = fdiv(blk, g.nbb);
+        uint32_t rb = blk - n * g.nbb.d;
+        const uint32_t b0 = fdiv(rb, g.nb12);
+        rb -= b0 * g.nb12.d;
+        const uint32_t b1 = fdiv(rb, g.nb2);
+        const uint32_t b2 = rb - b1 * g.nb2.d;
+        const uint32_t in = n * g.imgv + (b0 * g.st[0] + j0) * g.S0v +
+                            (b1 * g.st[1] + j1) * g.S1v + b2 * g.st[2] + j2;
+        v[u] = x[in];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UN; ++u)
+      if (oo[u] < g.total) y[oo[u]] = v[u];
+  }
+}
+
+// Covering blocks of output index i along one axis (32-bit).
+__device__ __forceinline__ void cover32(uint32_t i, uint32_t B, const FDiv& s, uint32_t nb,
+                                        uint32_t& lo, uint32_t& hi) {
+  lo = i + 1 <= B ? 0u : fdiv(i + 1 - B + s.d - 1, s);
+  hi = fdiv(i, s);
+  if (hi > nb - 1) hi = nb - 1;
+}
+
+template <typename S, int V>
+__global__ __launch_bounds__(256) void fold32_kernel(const S* __restrict__ x,
+                                                     S* __restrict__ y, BlockGeom32 g) {
+  using P = Pack<S, V>;
+  const P* __restrict__ xv = reinterpret_cast<const P*>(x);
+  P* __restrict__ yv = reinterpret_cast<P*>(y);
+  const uint32_t stride = gridDim.x * 256u;
+  for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < g.total; q += stride) {
+    const uint32_t n = fdiv(q, g.img);
+    uint32_t r = q - n * g.imgv;
+    const uint32_t i0 = fdiv(r, g.S0);
+    r -= i0 * g.S0v;
+    const uint32_t i1 = fdiv(r, g.S1);
+    const uint32_t i2 = r - i1 * g.S1v;
+    uint32_t lo0, hi0, lo1, hi1, lo2, hi2;
+    cover32(i0, g.B[0], g.s[0], g.nb[0], lo0, hi0);
+    cover32(i1, g.B[1], g.s[1], g.nb[1], lo1, hi1);
+    cover32(i2, g.B[2], g.s[2], g.nb[2], lo2, hi2);
+    P acc;
+#pragma unroll
+    for (int c = 0; c < V; ++c) acc.v[c] = (S)0;
+    for (uint32_t b0 = lo0; b0 <= hi0; ++b0) {
+      for (uint32_t b1 = lo1; b1 <= hi1; ++b1) {
+        const uint32_t row = ((n * g.nb[0] + b0) * g.nb[1] + b1) * g.nb[2];
+        const uint32_t inblk01 =
+            ((i0 - b0 * g.st[0]) * g.B[1] + (i1 - b1 * g.st[1])) * g.B[2];
+        for (uint32_t b2 = lo2; b2 <= hi2; ++b2) {
+          const P v = xv[(row + b2) * g.bsz + inblk01 + (i2 - b2 * g.st[2])];
+#pragma unroll
+          for (int c = 0; c < V; ++c) acc.v[c] += v.v[c];
+        }
+      }
+    }
+    yv[q] = acc;
+  }
+}
+
+// The 32-bit geometry when every index fits, else false.
+static bool make_geom32(const BlockGeom& g, bool fold, BlockGeom32& h) {
+  const int64_t lim = (int64_t)1 << 31;
+  const int64_t nblk = g.nbatch * g.nb[0] * g.nb[1] * g.nb[2];
+  const int64_t vpb = g.B[0] * g.B[1] * g.B[2];
+  const int64_t out_unfold = nblk * vpb;
+  const int64_t img = g.im[0] * g.im[1] * g.im[2];
+  if (out_unfold >= lim || g.nbatch * img >= lim) return false;
+  h.vpb = make_fdiv((uint32_t)vpb);
+  h.B12 = make_fdiv((uint32_t)(g.B[1] * g.B[2]));
+  h.B2 = make_fdiv((uint32_t)g.B[2]);
+  h.nbb = make_fdiv((uint32_t)(g.nb[0] * g.nb[1] * g.nb[2]));
+  h.nb12 = make_fdiv((uint32_t)(g.nb[1] * g.nb[2]));
+  h.nb2 = make_fdiv((uint32_t)g.nb[2]);
+  h.img = make_fdiv((uint32_t)img);
+  h.S0 = make_fdiv((uint32_t)(g.im[1] * g.im[2]));
+  h.S1 = make_fdiv((uint32_t)g.im[2]);
+  for (int d = 0; d < 3; ++d) {
+    h.s[d] = make_fdiv((uint32_t)g.s[d]);
+    h.B[d] = (uint32_t)g.B[d];
+    h.nb[d] = (uint32_t)g.nb[d];
+    h.im[d] = (uint32_t)g.im[d];
+    h.st[d] = (uint32_t)g.s[d];
+  }
+  h.S0v = (uint32_t)(g.im[1] * g.im[2]);
+  h.S1v = (uint32_t)g.im[2];
+  h.imgv = (uint32_t)img;
+  h.bsz = (uint32_t)vpb;
+  h.total = (uint32_t)(fold ? g.nbatch * img : out_unfold);
+  return true;
+}
+
+static unsigned grid_for(uint32_t total, int per_thread) {
+  int64_t wg = ((int64_t)total + 256 * per_thread - 1) / (256 * per_thread);
+  if (wg > 65536) wg = 65536;
+  return (unsigned)(wg < 1 ? 1 : wg);
+}
+
 static void pad3(int ndim, const int64_t* a, int64_t fill, int64_t out[3]) {
   for (int d = 0; d < 3; ++d) out[d] = fill;
   for (int d = 0; d < ndim; ++d) out[3 - ndim + d] = a[d];
@@ -208,6 +361,12 @@ static int pick_vec_bytes(int elem_bytes, const int64_t B[3],
 template <int VB>
 static int launch_unfold(const void* x, void* y, BlockGeom g, hipStream_t st) {
   using T = typename VecT<VB>::T;
+  BlockGeom32 h;
+  if (make_geom32(g, false, h)) {
+    unfold32_kernel<T><<<grid_for(h.total, UN), 256, 0, st>>>((const T*)x, (T*)y, h);
+    TNL_LAUNCH_CHECK("unfold");
+    return TNL_OK;
+  }
   const int64_t vpb = g.B[0] * g.B[1] * g.B[2];
   const int64_t total = g.nbatch * g.nb[0] * g.nb[1] * g.nb[2];
   int64_t grid = total < 16384 ? total : 16384;
@@ -224,6 +383,12 @@ static int launch_unfold(const void* x, void* y, BlockGeom g, hipStream_t st) {
 
 template <typename S, int V>
 static int launch_fold(const void* x, void* y, BlockGeom g, hipStream_t st) {
+  BlockGeom32 h;
+  if (make_geom32(g, true, h)) {
+    fold32_kernel<S, V><<<grid_for(h.total, 1), 256, 0, st>>>((const S*)x, (S*)y, h);
+    TNL_LAUNCH_CHECK("fold");
+    return TNL_OK;
+  }
   const int64_t total = g.nbatch * g.im[0] * g.im[1] * g.im[2];
   int64_t grid = (total + 255) / 256;
   if (grid > 32768) grid = 32768;
