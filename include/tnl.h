@@ -167,6 +167,27 @@ int tnl_grid_ws(const float* vals, const float* locs, float* out, int64_t nbatch
                 const void* plan, void* scratch, int64_t scratch_bytes,
                 void* stream);
 
+/* Largest scratch (bytes) tnl_grid_static can use: 64 MiB, i.e. 1 048 576
+ * point records (points x channel chunks of 8 complex / 16 real). */
+#define TNL_GRID_STATIC_SCRATCH_BYTES (64ll << 20)
+
+/* tnl_grid_ws without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the point records live in a file-scope
+ * device array of TNL_GRID_STATIC_SCRATCH_BYTES bytes that each GPU holds
+ * from the library's load to the end of the process.  No counterpart in the
+ * reference.  Plans needing more scratch than that (tnl_grid_scratch_bytes)
+ * are rejected with TNL_EINVAL before any device work.  The call allocates
+ * nothing; every call on one GPU shares that GPU's array (its contents are
+ * rewritten by each call, nothing needs to start at zero), so calls that
+ * overlap on one GPU (two streams, two host threads) race and must be
+ * serialised by the caller.  Same arguments and results as tnl_grid. */
+int tnl_grid_static(const float* vals, const float* locs, float* out, int64_t nbatch,
+                    int64_t npts, int ndim, const int64_t* grid_size,
+                    const float* width, int kernel, int norm, int pad_mode,
+                    int is_complex, float beta, const tnl_plan_layout* layout,
+                    const void* plan, void* stream);
+
 /* ---- many problems in one persistent launch ----------------------------
  * A batch is a set of independent interpolation problems of one ndim,
  * kernel, norm, pad_mode and dtype, each with its own grid size, width,

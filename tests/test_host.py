@@ -202,3 +202,26 @@ def test_plan_cache_never_builds_under_graph_capture(monkeypatch):
         assert _plan.get_plan(*args) is None and len(built) == 1
     _plan.release_plan_pins()
     assert not _plan._cache._pinned
+
+
+def test_grid_static_rejects_plans_beyond_its_scratch():
+    """tnl_grid_static refuses, before any device work, a plan whose point
+    records exceed TNL_GRID_STATIC_SCRATCH_BYTES (64 MiB = 1 048 576
+    records of 64 bytes); within the limit it needs the records a plan's
+    tnl_grid_scratch_bytes reports."""
+    import ctypes
+
+    from torchlinops_amd import _lib
+    from torchlinops_amd.functional._plan import plan_layout
+
+    h = _lib.lib()
+    text = (ROOT / "include" / "tnl.h").read_text()
+    assert "#define TNL_GRID_STATIC_SCRATCH_BYTES (64ll << 20)" in text
+    L = plan_layout(1_100_000, 3, (64, 64, 64), (4.0,) * 3, 0)  # 8x8x8 tiles: matrix cores
+    need = ctypes.c_int64(0)
+    _lib.check(h.tnl_grid_scratch_bytes(L, 8, 1, ctypes.byref(need)))
+    assert need.value == 1_100_000 * 64 > (64 << 20)
+    args = [None, None, None, 8, 1_100_000, 3, _lib.i64_array((64, 64, 64)),
+            _lib.f32_array((4.0,) * 3), 0, 1, 0, 1, 8.99, L, None, None]
+    with pytest.raises(ValueError, match="more than the static 67108864 bytes"):
+        _lib.check(h.tnl_grid_static(*args))

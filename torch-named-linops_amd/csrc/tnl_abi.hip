@@ -1,5 +1,6 @@
 // extern "C" entry points for interpolation / gridding and error plumbing.
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "tnl_common.hpp"
@@ -173,6 +174,58 @@ extern "C" int tnl_grid_ws(const float* vals, const float* locs, float* out,
   return interp_entry(true, vals, locs, out, nbatch, npts, ndim, grid_size,
                       width, kernel, norm, pad_mode, is_complex, beta, layout,
                       plan, scratch_bytes > 0 ? scratch : nullptr, scratch_bytes, stream);
+}
+
+// ---- allocation-free gridding: the point records in a static device array
+// Each GPU holds its own copy of `scratch` at its own address; host code
+// never passes the array's name to a kernel (that host-side name is not a
+// device address) but the per-device pointer hipGetSymbolAddress returns,
+// taken once per device with that device current.
+static __device__ __align__(256) unsigned char scratch[TNL_GRID_STATIC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+static int static_scratch(void** ptr) {
+  int dev = 0;
+  if (int rc = check_hip(hipGetDevice(&dev), "hipGetDevice")) return rc;
+  if (dev < 0 || dev >= kMaxDevices) return fail(TNL_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (int rc = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                           "hipGetSymbolAddress(scratch)"))
+      return rc;
+    if (!p) return fail(TNL_EHIP, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return TNL_OK;
+}
+
+extern "C" int tnl_grid_static(const float* vals, const float* locs, float* out,
+                               int64_t nbatch, int64_t npts, int ndim,
+                               const int64_t* grid_size, const float* width,
+                               int kernel, int norm, int pad_mode, int is_complex,
+                               float beta, const tnl_plan_layout* layout,
+                               const void* plan, void* stream) {
+  if (nbatch < 0) return fail(TNL_EINVAL, "nbatch must be >= 0");
+  const int64_t need = layout ? grid_scratch_bytes(*layout, nbatch, is_complex) : 0;
+  if (need > TNL_GRID_STATIC_SCRATCH_BYTES)
+    return fail(TNL_EINVAL, "tnl_grid_static: this plan needs " + std::to_string(need) +
+                                " bytes of scratch, more than the static " +
+                                std::to_string((long long)TNL_GRID_STATIC_SCRATCH_BYTES) +
+                                " bytes; use tnl_grid_ws with a caller workspace");
+  void* p = nullptr;
+  if (need > 0 && plan && nbatch > 0 && npts > 0) {
+    if (int rc = static_scratch(&p)) return rc;
+  }
+  return interp_entry(true, vals, locs, out, nbatch, npts, ndim, grid_size, width, kernel,
+                      norm, pad_mode, is_complex, beta, layout, plan, p, p ? need : 0,
+                      stream);
 }
 
 extern "C" int tnl_interp_grad_locs(const float* grid_vals, const float* pt_vals,

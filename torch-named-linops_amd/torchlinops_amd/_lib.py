@@ -196,6 +196,11 @@ SIGNATURES = {
         [_vp, _vp, _vp, _i64, _i64, _i32, _i64p, _f32p, _i32, _i32, _i32,
          _i32, _f32, _layout_p, _vp, _vp, _i64, _vp],
     ),
+    "tnl_grid_static": (
+        _i32,
+        [_vp, _vp, _vp, _i64, _i64, _i32, _i64p, _f32p, _i32, _i32, _i32,
+         _i32, _f32, _layout_p, _vp, _vp],
+    ),
     "tnl_batch_layout_query": (
         _i32, [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _blayout_p]),
     "tnl_batch_build": (_i32, [_vp, _blayout_p, _vp, _vp]),
