@@ -485,7 +485,8 @@ def c3_batched_cg(dev, iters: int = 20, nprob: int = 64, seed: int = 3):
 
 def c5_learned_cg(dev, seed: int = 5, train_n: int = 32, train_problems: int = 64,
                   train_rhs: int = 8, train_cg_iters: int = 100, train_steps: int = 300):
-    """Learned diagonal preconditioner for S^H F^H F S + lam I.  Training:
+    """Learned diagonal preconditioner for S^H F^H F S + lam I (a learned
+    correction of the Jacobi diagonal, judged against plain CG).  Training:
     512 solves generated by plain CG (64 problems x 8 right-hand sides
     A f, f smooth random fields, each problem a random 8-coil geometry and radial
     trajectory at 32^3 / 64^3), network trained on the GPU with the BF16
@@ -550,17 +551,18 @@ def c5_learned_cg(dev, seed: int = 5, train_n: int = 32, train_problems: int = 6
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     with torch.no_grad():
-        d = net(Lm.coil_features(A.maps))[0]
+        d = Lm.learned_diagonal(net, A)
     torch.cuda.synchronize()
     infer_s = time.perf_counter() - t0
-    _, learned = Lm.solve(A, y, d, tol=1e-6, max_iters=3000, check_every=10)
-    # the analytic Jacobi diagonal 1 / (psf(0) sum_c |S_c|^2 + lam), for reference
-    e = (A.maps.abs() ** 2).sum(0)
-    c = 64
-    delta = torch.zeros((128,) * 3, dtype=torch.complex64, device=dev)
-    delta[c, c, c] = 1
-    psf0 = float(A.normal_no_reg(delta)[c, c, c].real / e[c, c, c])
-    _, jacobi = Lm.solve(A, y, 1.0 / (psf0 * e + A.lam), tol=1e-6, max_iters=3000,
+    # the product path: the plain-CG judge first (10 iterations of each at
+    # matched counts), then the winner to 1e-6; and the same without judge
+    _, learned = Lm.solve(A, y, d, tol=1e-6, max_iters=3000, check_every=10, judge_iters=10)
+    _, learned_nj = Lm.solve(A, y, d, tol=1e-6, max_iters=3000, check_every=10)
+    log(f"c5: learned PCG without the judge {learned_nj['iters']} iterations, "
+        f"{learned_nj['seconds']:.2f} s; judge {learned['judge']}")
+    # the analytic Jacobi diagonal 1 / (psf(0) sum_c |S_c|^2 + lam): the
+    # network's starting point
+    _, jacobi = Lm.solve(A, y, Lm.precond_inputs(A)[1], tol=1e-6, max_iters=3000,
                          check_every=10)
     # error after a fixed budget of 20 iterations (the training criterion)
     with torch.no_grad():
@@ -596,6 +598,10 @@ def c5_learned_cg(dev, seed: int = 5, train_n: int = 32, train_problems: int = 6
         "learned_pcg_iters": learned["iters"],
         "learned_true_residual": learned["true_residual"],
         "learned_fallback": learned["fallback"],
+        "learned_judge": learned["judge"],
+        "learned_without_judge_s": learned_nj["seconds"] + infer_s,
+        "learned_without_judge_iters": learned_nj["iters"],
+        "time_ratio_without_judge": (learned_nj["seconds"] + infer_s) / plain["seconds"],
         "inference_ms": infer_s * 1e3,
         "jacobi_pcg_s": jacobi["seconds"],
         "jacobi_pcg_iters": jacobi["iters"],

@@ -9,20 +9,30 @@ once).
 ``PrecondNet`` is a 4-layer, 16-channel 3x3x3 convolutional network whose
 convolutions run in BF16 on the matrix cores with FP32 accumulation
 (``tnl_conv3d_fwd``; data gradient = the same kernel with flipped,
-transposed weights; weight gradient ``tnl_conv3d_wgrad``).  From the
-coil-energy map it predicts a positive diagonal d, the preconditioner
-M = diag(d).  It is trained on solves that plain CG generates: for each
-training problem A and right-hand sides z (A applied to smooth random
-fields), plain CG provides x* = A^-1 z; the loss is the relative error
-||x_k(d) - x*||^2 / ||x*||^2 of k preconditioned-CG iterations from zero,
-differentiated through the unrolled iterations (the interpolation
-kernels' autograd pair supplies the operator's backward).  This targets
-convergence directly; a constant d (plain CG) lies inside the family.
+transposed weights; weight gradient ``tnl_conv3d_wgrad``).  It predicts a
+positive diagonal image-space preconditioner M = diag(d) as a learned
+correction of the analytic Jacobi diagonal: d = J exp(o), J = 1 /
+(psf(0) sum_c |S_c|^2 + lam) (psf(0) = the constant diagonal of F^H F,
+measured once with a unit impulse), o the network output over the features
+of ``precond_inputs`` (how much of each voxel's diagonal is signal rather
+than regularisation, and log J).  The last layer starts at zero, so the
+untrained network is exactly Jacobi.  It is trained on solves that plain CG
+generates: for each training problem A and right-hand sides z (A applied to
+smooth random fields), plain CG provides x* = A^-1 z; the loss is the
+relative error ||x_k(d) - x*||^2 / ||x*||^2 of k preconditioned-CG
+iterations from zero, differentiated through the unrolled iterations (the
+interpolation kernels' autograd pair supplies the operator's backward).
 
 ``solve`` runs preconditioned CG (``DeviceCG``) and keeps plain CG as the
-check: the residual is recomputed with A itself, and if the preconditioned
-run stalls or breaks down, plain CG continues from its iterate.
-No counterpart in the reference, whose CG (pcg.py) is unpreconditioned.
+judge: with ``judge_iters`` it first runs plain CG and the preconditioned
+CG for that many iterations and keeps the preconditioner only if its
+iterate is not behind plain CG's at that matched iteration count, scored by
+CG's own objective (the A-norm error, from one A x each; the recomputed
+residuals are reported beside it) - the loser's work is counted, the
+winner continues from its iterate; and if the preconditioned run stalls,
+breaks down or ends above the tolerance on its recomputed residual, plain
+CG continues from its iterate.  No counterpart in the
+reference, whose CG (pcg.py) is unpreconditioned.
 """
 
 from __future__ import annotations
@@ -42,8 +52,8 @@ from ..linops.toeplitz import crop_wrapped, pad_wrapped
 from .cg import DeviceCG, inner
 
 __all__ = ["SenseNormal", "PrecondNet", "conv3d", "to_channel_last", "pack_conv_weights",
-           "pack_conv_weights_transposed", "coil_features", "pcg_unrolled",
-           "train_preconditioner", "solve"]
+           "pack_conv_weights_transposed", "coil_features", "precond_inputs",
+           "learned_diagonal", "pcg_unrolled", "train_preconditioner", "judge", "solve"]
 
 HIDDEN = 16
 
@@ -157,11 +167,12 @@ class PrecondNet(nn.Module):
             self.weights.append(nn.Parameter(torch.randn(b, a, 3, 3, 3, generator=g) * std))
             self.biases.append(nn.Parameter(torch.zeros(b)))
         with torch.no_grad():
-            self.weights[-1].mul_(0.1)
+            self.weights[-1].zero_()  # output 0: the correction starts at 1
         self.log_scale = nn.Parameter(torch.zeros(()))
 
     def forward(self, feats: torch.Tensor) -> torch.Tensor:
-        """feats fp32 [B, F, Z, Y, X] -> d fp32 [B, Z, Y, X] (> 0)."""
+        """feats fp32 [B, F, Z, Y, X] -> multiplicative correction fp32
+        [B, Z, Y, X] (> 0; exactly 1 before training)."""
         x = to_channel_last(feats)
         n = len(self.weights)
         for i, (w, b) in enumerate(zip(self.weights, self.biases)):
@@ -176,6 +187,37 @@ def coil_features(maps: torch.Tensor) -> torch.Tensor:
     e = (maps.abs() ** 2).sum(0)
     e = e / e.max()
     return torch.stack([e, torch.log(e + 1e-3) / math.log(1e3)], 0)[None].float()
+
+
+def precond_inputs(A: "SenseNormal"):
+    """(features [1, 2, *im] fp32, Jacobi diagonal J [*im] fp32) of a SENSE
+    normal operator: J = 1 / (psf0 e + lam) with e = sum_c |S_c|^2 and psf0
+    the diagonal of F^H F (constant over the image; measured at the centre
+    voxel with a unit impulse); features q = psf0 e / (psf0 e + lam), the
+    fraction of each voxel's diagonal that is signal, and log(J lam) /
+    log(1e-4), both O(1).  Cached on the operator."""
+    cached = getattr(A, "_precond_inputs", None)
+    if cached is not None:
+        return cached
+    e = (A.maps.abs() ** 2).sum(0).float()
+    c = tuple(n // 2 for n in A.im_size)
+    delta = torch.zeros(A.im_size, dtype=torch.complex64, device=A.maps.device)
+    delta[c] = 1.0
+    with torch.no_grad():
+        psf0 = float(A.normal_no_reg(delta)[c].real / e[c].clamp_min(1e-30))
+    lam = A.lam if A.lam > 0 else 1e-6 * psf0 * float(e.max())
+    diag = psf0 * e + lam
+    J = 1.0 / diag
+    q = psf0 * e / diag
+    feats = torch.stack([q, torch.log(J * lam) / math.log(1e-4)], 0)[None].float()
+    A._precond_inputs = (feats, J.float())
+    return A._precond_inputs
+
+
+def learned_diagonal(net: "PrecondNet", A: "SenseNormal") -> torch.Tensor:
+    """d = J x net(features): the network's correction of Jacobi."""
+    feats, J = precond_inputs(A)
+    return J * net(feats)[0]
 
 
 class SenseNormal:
@@ -285,8 +327,9 @@ def pcg_unrolled(A, z: torch.Tensor, d: torch.Tensor, iters: int) -> torch.Tenso
 def train_preconditioner(net: PrecondNet, data, steps: int = 300, lr: float = 3e-3,
                          unroll: int = 12, log=None, max_grad_norm: float = 1.0,
                          keep_best: bool = True):
-    """data: list of (features [1, F, *im], A, z [R, *im], x* [R, *im])
-    with x* = A^-1 z from plain CG.  Minimises the relative error of
+    """data: list of (features, A, z [R, *im], x* [R, *im]) with x* = A^-1 z
+    from plain CG (the features entry is unused: the diagonal is
+    ``learned_diagonal(net, A)``).  Minimises the relative error of
     ``unroll`` preconditioned-CG iterations (Adam, gradients clipped to
     ``max_grad_norm``).  The loss through unrolled CG is rough, so with
     ``keep_best`` the parameters at the start of the pass over ``data``
@@ -305,11 +348,11 @@ def train_preconditioner(net: PrecondNet, data, steps: int = 300, lr: float = 3e
                 best, best_state = pass_loss, snap
 
     for it in range(steps):
-        feats, A, z, x = data[it % nd]
+        _, A, z, x = data[it % nd]
         if keep_best and it % nd == 0:
             close_pass()
             snap = {k: v.detach().clone() for k, v in net.state_dict().items()}
-        d = net(feats)[0]
+        d = learned_diagonal(net, A)
         xk = pcg_unrolled(A, z, d, unroll)
         err = (xk - x).abs() ** 2
         red = tuple(range(1, x.ndim))
@@ -330,27 +373,69 @@ def train_preconditioner(net: PrecondNet, data, steps: int = 300, lr: float = 3e
     return losses
 
 
+def judge(A, y: torch.Tensor, M, iters: int = 10, check_every: int = 5):
+    """Plain CG and CG preconditioned by ``M`` (r -> M r), ``iters``
+    iterations each from zero.  Both are scored by the quantity CG
+    minimises, the energy f(x) = x^H A x / 2 - Re(y^H x) = ||x - x*||_A^2 / 2
+    - const (so a lower f is a smaller A-norm error, without knowing x*),
+    computed from the same A x as the recomputed residual |y - A x| / |y|
+    (reported too; CG's residual norm is not monotone on ill-conditioned
+    systems, its energy is).  Returns (accept, x_plain, x_pre, info):
+    ``accept`` is True when the preconditioned iterate is finite and its
+    energy not above plain CG's."""
+    ynorm = torch.linalg.vector_norm(y)
+
+    def score(x):
+        ax = A(x)
+        f = 0.5 * float(torch.vdot(x.flatten(), ax.flatten()).real) - float(
+            torch.vdot(y.flatten(), x.flatten()).real)
+        return f, float(torch.linalg.vector_norm(y - ax) / ynorm)
+
+    cg_p = DeviceCG(A, lam=0.0, M=None, check_every=check_every)
+    xp = cg_p.solve(y, tol=0.0, max_iters=iters)
+    cg_m = DeviceCG(A, lam=0.0, M=M, check_every=check_every)
+    xm = cg_m.solve(y, tol=0.0, max_iters=iters)
+    fp, rp = score(xp)
+    fm, rm = score(xm)
+    ok = math.isfinite(fm) and math.isfinite(rm) and fm <= fp
+    info = {"accepted": ok, "iters": iters, "plain_energy": fp, "precond_energy": fm,
+            "plain_residual": rp, "precond_residual": rm}
+    return ok, xp, xm, info
+
+
 def solve(A: SenseNormal, y: torch.Tensor, d: Optional[torch.Tensor] = None,
           tol: float = 1e-6, max_iters: int = 1000, check_every: int = 10,
-          check_factor: float = 100.0):
+          check_factor: float = 100.0, judge_iters: int = 0):
     """(Preconditioned) CG until its relative residual |r| / |y| is below
-    ``tol``.  Plain CG checks the result: the residual is recomputed as
-    |y - A x| / |y| with the operator itself, and if the preconditioned run
-    did not converge, broke down (non-finite values) or its recomputed
-    residual exceeds ``check_factor * tol``, plain CG continues from its
-    iterate (or from zero).  Returns (x, info)."""
+    ``tol``.  Plain CG checks the result: with ``judge_iters`` > 0 the
+    preconditioner must first match plain CG after that many iterations of
+    each (``judge``; else plain CG carries on from its own iterate);
+    afterwards the residual is recomputed as |y - A x| / |y| with
+    the operator itself, and if the preconditioned run did not converge,
+    broke down (non-finite values) or its recomputed residual exceeds
+    ``check_factor * tol``, plain CG continues from its iterate (or from
+    zero).  Returns (x, info); info["iters"] counts every iteration run,
+    the judge's included."""
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     # d: None (plain CG), an image-space diagonal, or a callable r -> M r
     M = None if d is None else (d if callable(d) else (lambda r: d * r))
+    x0, spent, verdict = None, 0, None
+    if M is not None and judge_iters > 0:
+        ok, xp, xm, verdict = judge(A, y, M, judge_iters, check_every=judge_iters)
+        spent = 2 * judge_iters
+        if ok:
+            x0 = xm
+        else:
+            M, x0 = None, xp
     cg = DeviceCG(A, lam=0.0, M=M, check_every=check_every)
-    x = cg.solve(y, tol=tol, max_iters=max_iters)
-    iters = cg.iters
+    x = cg.solve(y, x0=x0, tol=tol, max_iters=max_iters)
+    iters = cg.iters + spent
     converged = cg.history[-1] <= tol
     ynorm = torch.linalg.vector_norm(y)
     true_res = float(torch.linalg.vector_norm(y - A(x)) / ynorm)
     fallback = False
-    if d is not None and not (converged and math.isfinite(true_res)
+    if M is not None and not (converged and math.isfinite(true_res)
                               and true_res <= check_factor * tol):
         fallback = True
         x0 = x if math.isfinite(true_res) else None
@@ -361,4 +446,5 @@ def solve(A: SenseNormal, y: torch.Tensor, d: Optional[torch.Tensor] = None,
         true_res = float(torch.linalg.vector_norm(y - A(x)) / ynorm)
     torch.cuda.synchronize()
     return x, dict(seconds=time.perf_counter() - t0, iters=iters, converged=converged,
-                   true_residual=true_res, fallback=fallback, history=list(cg.history))
+                   true_residual=true_res, fallback=fallback, history=list(cg.history),
+                   judge=verdict)
