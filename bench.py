@@ -715,16 +715,20 @@ def main() -> int:
     ms_step = wall / args.steps * 1e3
     value = 2 * C * M / (wall / args.steps)
     dom, t_dom = ("grid", t_g) if t_g >= t_u else ("ungrid", t_u)
-    traffic = None
+    traffic, traffic_kernels = None, []
     tfile = ROOT / "profiles" / "interp_c2_radial_hbm_traffic.json"
     if tfile.exists() and not args.small:
         # HBM bytes per dispatch from the FETCH_SIZE / WRITE_SIZE passes
-        # recorded in profiles/ (rocprofv3 cannot run inside this process)
+        # recorded in profiles/ (rocprofv3 cannot run inside this process),
+        # summed over every kernel of the dominant call: for gridding the
+        # point-record transpose, the split-tile zero pass and the
+        # matrix-core kernel
         tb = json.loads(tfile.read_text())["traffic_bytes"]
-        keys = ("grid_mfma_kernel", "grid_binned_kernel") if dom == "grid" else ("ungrid_binned_kernel",)
-        hits = [v for key in keys for k, v in tb.items()
-                if k.startswith("tnl::" + key) and "true, 1," not in k]
-        traffic = float(hits[0]) if hits else None
+        keys = (("grid_mfma_kernel", "pack_points_kernel", "grid_zero_split_kernel")
+                if dom == "grid" else ("ungrid_binned_kernel<3, true, 2",))
+        hits = {k: float(v) for key in keys for k, v in tb.items() if k.startswith("tnl::" + key)}
+        if hits:
+            traffic, traffic_kernels = sum(hits.values()), sorted(hits)
     out = {
         "metric": METRIC,
         "value": value,
@@ -761,6 +765,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": bytes_call / t_dom / HBM_PEAK,
             "traffic": traffic,
+            "traffic_kernels": traffic_kernels,
             "algorithmic_bytes": bytes_call,
             "ungrid_frac": bytes_call / t_u / HBM_PEAK,
             "grid_frac": bytes_call / t_g / HBM_PEAK,
@@ -777,6 +782,11 @@ def main() -> int:
     if not args.no_blocks and not args.small:
         log("blocks: array_to_blocks / blocks_to_array ...")
         out["blocks"] = blocks_leg(dev)
+        if tfile.exists():
+            tb = json.loads(tfile.read_text())["traffic_bytes"]
+            for key, name in (("unfold", "tnl::unfold32_kernel"), ("fold", "tnl::fold32_kernel")):
+                hit = [float(v) for k, v in tb.items() if k.startswith(name)]
+                out["blocks"][f"{key}_traffic"] = hit[0] if hit else None
         log(f"blocks: unfold {out['blocks']['unfold_us']:.1f} us "
             f"({out['blocks']['unfold_frac'] * 100:.1f} %), fold {out['blocks']['fold_us']:.1f} us "
             f"({out['blocks']['fold_frac'] * 100:.1f} %), checks {out['blocks']['checks_pass']}; "
