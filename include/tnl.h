@@ -143,6 +143,23 @@ int tnl_grid(const float* vals, const float* locs, float* out, int64_t nbatch,
              int is_complex, float beta, const tnl_plan_layout* layout,
              const void* plan, void* stream);
 
+/* ---- float64 / complex128 interpolation and gridding --------------------
+ * tnl_ungrid / tnl_grid for double data (vals, locs and out as doubles,
+ * complex interleaved), computed in double precision with the windows
+ * evaluated in double; no plan (one thread per point; gridding adds with
+ * double atomics into out, which the call zero-fills first).  Stand beside
+ * the same dispatch points (ungrid.py:86-119, grid.py:91-124): the
+ * reference's torch fallback computes in the input's dtype
+ * (grid.py:571-639, ungrid.py:624-684). */
+int tnl_ungrid_f64(const double* vals, const double* locs, double* out, int64_t nbatch,
+                   int64_t npts, int ndim, const int64_t* grid_size, const float* width,
+                   int kernel, int norm, int pad_mode, int is_complex, double beta,
+                   void* stream);
+int tnl_grid_f64(const double* vals, const double* locs, double* out, int64_t nbatch,
+                 int64_t npts, int ndim, const int64_t* grid_size, const float* width,
+                 int kernel, int norm, int pad_mode, int is_complex, double beta,
+                 void* stream);
+
 /* Scratch bytes that tnl_grid_ws uses with this plan for nbatch channels:
  * the channel values re-packed point-major, one 64-byte record (up to 8
  * complex / 16 real channel values) per point and channel chunk, for the

@@ -176,6 +176,47 @@ extern "C" int tnl_grid_ws(const float* vals, const float* locs, float* out,
                       plan, scratch_bytes > 0 ? scratch : nullptr, scratch_bytes, stream);
 }
 
+// ---- float64 / complex128 interpolation (plan-free kernels, double arithmetic)
+static int interp_f64_entry(bool adjoint, const double* vals, const double* locs, double* out,
+                            int64_t nbatch, int64_t npts, int ndim, const int64_t* grid_size,
+                            const float* width, int kernel, int norm, int pad_mode,
+                            int is_complex, double beta, void* stream) {
+  InterpGeom G;
+  if (int rc = make_geom(ndim, grid_size, width, kernel, norm, pad_mode, is_complex,
+                         (float)beta, G))
+    return rc;
+  if (nbatch < 0 || npts < 0) return fail(TNL_EINVAL, "nbatch and npts must be >= 0");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t ncomp = is_complex ? 2 : 1;
+  if (nbatch == 0) return TNL_OK;
+  if (!out) return fail(TNL_EINVAL, "null output pointer");
+  if (adjoint) {
+    // zero bits are 0.0 in double too: fill two floats per double
+    if (int rc = fill_zero(reinterpret_cast<float*>(out), 2 * nbatch * G.ncells * ncomp, s))
+      return rc;
+  }
+  if (npts == 0) return TNL_OK;
+  if (!vals || !locs) return fail(TNL_EINVAL, "null input pointer");
+  return interp_direct_f64(adjoint, vals, locs, out, nbatch, npts, G, is_complex, beta, s);
+}
+
+extern "C" int tnl_ungrid_f64(const double* vals, const double* locs, double* out,
+                              int64_t nbatch, int64_t npts, int ndim,
+                              const int64_t* grid_size, const float* width, int kernel,
+                              int norm, int pad_mode, int is_complex, double beta,
+                              void* stream) {
+  return interp_f64_entry(false, vals, locs, out, nbatch, npts, ndim, grid_size, width,
+                          kernel, norm, pad_mode, is_complex, beta, stream);
+}
+
+extern "C" int tnl_grid_f64(const double* vals, const double* locs, double* out,
+                            int64_t nbatch, int64_t npts, int ndim, const int64_t* grid_size,
+                            const float* width, int kernel, int norm, int pad_mode,
+                            int is_complex, double beta, void* stream) {
+  return interp_f64_entry(true, vals, locs, out, nbatch, npts, ndim, grid_size, width,
+                          kernel, norm, pad_mode, is_complex, beta, stream);
+}
+
 // ---- allocation-free gridding: the point records in a static device array
 // Each GPU holds its own copy of `scratch` at its own address; host code
 // never passes the array's name to a kernel (that host-side name is not a

@@ -6,6 +6,9 @@
 namespace tnl {
 
 int fill_zero(float* p, int64_t n, hipStream_t s);
+int interp_direct_f64(bool adjoint, const double* vals, const double* locs, double* out,
+                      int64_t nbatch, int64_t npts, const InterpGeom& G, int is_complex,
+                      double beta, hipStream_t s);
 
 int interp_direct(bool adjoint, const float* vals, const float* locs,
                   float* out, int64_t nbatch, int64_t npts,

@@ -154,6 +154,176 @@ __global__ __launch_bounds__(256) void interp_direct_kernel(
   }
 }
 
+// ---- float64 / complex128 interpolation (no plan; one thread per point)
+// Windows in double: Kaiser-Bessel I0 from its power series
+// sum_k (z^2 / 4)^k / (k!)^2, summed until a term no longer changes the
+// double sum (the terms fall monotonically once k > z / 2).
+__device__ __forceinline__ double kb_window_f64(double x, double beta) {
+  const double v = fmax(1.0 - x * x, 0.0);
+  const double q = 0.25 * beta * beta * v;  // z^2 / 4
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 400; ++k) {
+    term *= q / ((double)k * (double)k);
+    sum += term;
+    if (term < 1e-17 * sum && k > 2) break;
+  }
+  return sum;
+}
+
+__device__ __forceinline__ double window_f64(int kernel, double x, double beta) {
+  return kernel == TNL_KERNEL_KAISER_BESSEL ? kb_window_f64(x, beta) : fmax(1.0 - fabs(x), 0.0);
+}
+
+// Taps of coordinate t along one padded axis in double: lower =
+// ceil(t - w/2), kept iff |t - g| <= w/2, zero padding drops cells outside
+// [0, n); non-finite coordinates have none.
+struct TapRangeD {
+  int64_t g0;
+  int count;
+};
+
+__device__ __forceinline__ TapRangeD tap_range_f64(double t, double half, int ntap, int64_t n,
+                                                   int pad) {
+  TapRangeD r{0, 0};
+  if (!(fabs(t) < 4.0e18)) return r;
+  const int64_t lower = (int64_t)ceil(t - half);
+  int k0 = 0, k1 = ntap - 1;
+  while (k0 <= k1 && !(fabs((double)(lower + k0) - t) <= half)) ++k0;
+  while (k1 >= k0 && !(fabs((double)(lower + k1) - t) <= half)) --k1;
+  if (k1 < k0) return r;
+  int64_t a = lower + k0, b = lower + k1;
+  if (pad == TNL_PAD_ZERO) {
+    a = a < 0 ? 0 : a;
+    b = b > n - 1 ? n - 1 : b;
+    if (b < a) return r;
+  }
+  r.g0 = a;
+  r.count = (int)(b - a + 1);
+  return r;
+}
+
+template <int NDIM, bool CPLX, bool ADJ>
+__global__ __launch_bounds__(256) void interp_direct_f64_kernel(
+    const double* __restrict__ vals, const double* __restrict__ locs, double* __restrict__ out,
+    int64_t nbatch, int64_t npts, InterpGeom G, double beta) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npts; p += stride) {
+    double t[3] = {0.0, 0.0, 0.0}, half[3], invh[3];
+    TapRangeD r[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      half[d] = (double)G.half[d];
+      invh[d] = 1.0 / half[d];
+      r[d].g0 = 0;
+      r[d].count = 1;
+      if (d >= 3 - NDIM) {
+        t[d] = locs[p * NDIM + d - (3 - NDIM)];
+        r[d] = tap_range_f64(t[d], half[d], G.ntap[d], G.n[d], G.pad);
+      }
+    }
+    for (int64_t b = 0; b < nbatch; ++b) {
+      double ar = 0.0, ai = 0.0;
+      if (ADJ) {
+        ar = CPLX ? vals[2 * (b * npts + p)] : vals[b * npts + p];
+        ai = CPLX ? vals[2 * (b * npts + p) + 1] : 0.0;
+      }
+      double sr = 0.0, si = 0.0;
+      for (int kx = 0; kx < r[0].count; ++kx) {
+        double dx = 0.0, wx = 1.0;
+        int64_t ox = 0;
+        if (NDIM >= 3) {
+          const int64_t g = r[0].g0 + kx;
+          dx = fmin(fabs((double)g - t[0]) * invh[0], 1.0);
+          wx = G.norm == 1 ? window_f64(G.kernel, dx, beta) : 1.0;
+          ox = (G.pad == TNL_PAD_CIRCULAR ? mod_pos(g, G.n[0]) : g) * G.cstride[0];
+        }
+        for (int ky = 0; ky < r[1].count; ++ky) {
+          double dy = 0.0, wxy = wx;
+          int64_t oxy = ox;
+          if (NDIM >= 2) {
+            const int64_t g = r[1].g0 + ky;
+            dy = fmin(fabs((double)g - t[1]) * invh[1], 1.0);
+            if (G.norm == 1) wxy *= window_f64(G.kernel, dy, beta);
+            oxy += (G.pad == TNL_PAD_CIRCULAR ? mod_pos(g, G.n[1]) : g) * G.cstride[1];
+          }
+          for (int kz = 0; kz < r[2].count; ++kz) {
+            const int64_t g = r[2].g0 + kz;
+            const double dz = fmin(fabs((double)g - t[2]) * invh[2], 1.0);
+            double w;
+            if (G.norm == 1) {
+              w = wxy * window_f64(G.kernel, dz, beta);
+            } else {
+              const double rr = sqrt(dx * dx + dy * dy + dz * dz);
+              w = rr <= 1.0 ? window_f64(G.kernel, rr, beta) : 0.0;
+            }
+            if (!isfinite(w)) w = 0.0;
+            const int64_t off =
+                oxy + (G.pad == TNL_PAD_CIRCULAR ? mod_pos(g, G.n[2]) : g) + b * G.ncells;
+            if (ADJ) {
+              if (CPLX) {
+                unsafeAtomicAdd(out + 2 * off, w * ar);
+                unsafeAtomicAdd(out + 2 * off + 1, w * ai);
+              } else {
+                unsafeAtomicAdd(out + off, w * ar);
+              }
+            } else {
+              if (CPLX) {
+                sr = fma(w, vals[2 * off], sr);
+                si = fma(w, vals[2 * off + 1], si);
+              } else {
+                sr = fma(w, vals[off], sr);
+              }
+            }
+          }
+        }
+      }
+      if (!ADJ) {
+        if (CPLX) {
+          out[2 * (b * npts + p)] = sr;
+          out[2 * (b * npts + p) + 1] = si;
+        } else {
+          out[b * npts + p] = sr;
+        }
+      }
+    }
+  }
+}
+
+int interp_direct_f64(bool adjoint, const double* vals, const double* locs, double* out,
+                      int64_t nbatch, int64_t npts, const InterpGeom& G, int is_complex,
+                      double beta, hipStream_t s) {
+  if (npts == 0 || nbatch == 0) return TNL_OK;
+  int64_t nb = (npts + 255) / 256;
+  if (nb > 65536) nb = 65536;
+#define TNL_F64_LAUNCH(ND)                                                                 \
+  {                                                                                        \
+    if (adjoint) {                                                                         \
+      if (is_complex)                                                                      \
+        interp_direct_f64_kernel<ND, true, true><<<(unsigned)nb, 256, 0, s>>>(             \
+            vals, locs, out, nbatch, npts, G, beta);                                       \
+      else                                                                                 \
+        interp_direct_f64_kernel<ND, false, true><<<(unsigned)nb, 256, 0, s>>>(            \
+            vals, locs, out, nbatch, npts, G, beta);                                       \
+    } else {                                                                               \
+      if (is_complex)                                                                      \
+        interp_direct_f64_kernel<ND, true, false><<<(unsigned)nb, 256, 0, s>>>(            \
+            vals, locs, out, nbatch, npts, G, beta);                                       \
+      else                                                                                 \
+        interp_direct_f64_kernel<ND, false, false><<<(unsigned)nb, 256, 0, s>>>(           \
+            vals, locs, out, nbatch, npts, G, beta);                                       \
+    }                                                                                      \
+    TNL_LAUNCH_CHECK("interp_direct_f64");                                                 \
+    return TNL_OK;                                                                         \
+  }
+  switch (G.ndim) {
+    case 1: TNL_F64_LAUNCH(1)
+    case 2: TNL_F64_LAUNCH(2)
+    case 3: TNL_F64_LAUNCH(3)
+  }
+#undef TNL_F64_LAUNCH
+  return fail(TNL_EINVAL, "ndim must be 1, 2 or 3");
+}
+
 __global__ void fill_zero_kernel(float4* __restrict__ p, int64_t n4,
                                  float* __restrict__ tail, int ntail) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;

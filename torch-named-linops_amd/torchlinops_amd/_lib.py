@@ -196,6 +196,16 @@ SIGNATURES = {
         [_vp, _vp, _vp, _i64, _i64, _i32, _i64p, _f32p, _i32, _i32, _i32,
          _i32, _f32, _layout_p, _vp, _vp, _i64, _vp],
     ),
+    "tnl_ungrid_f64": (
+        _i32,
+        [_vp, _vp, _vp, _i64, _i64, _i32, _i64p, _f32p, _i32, _i32, _i32, _i32,
+         ctypes.c_double, _vp],
+    ),
+    "tnl_grid_f64": (
+        _i32,
+        [_vp, _vp, _vp, _i64, _i64, _i32, _i64p, _f32p, _i32, _i32, _i32, _i32,
+         ctypes.c_double, _vp],
+    ),
     "tnl_grid_static": (
         _i32,
         [_vp, _vp, _vp, _i64, _i64, _i32, _i64p, _f32p, _i32, _i32, _i32,

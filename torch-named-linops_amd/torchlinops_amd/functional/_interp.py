@@ -8,10 +8,11 @@ functional API:
   * shape preparation              ungrid.py:577-621, grid.py:522-568
   * kernel defaults                kernels.py:97-105, 245-252
   * autograd pair                  interp.py:13-236
-The arithmetic runs in the HIP kernels of libtnl (tnl_ungrid / tnl_grid)
-for tensors on a HIP device; host tensors take the package's torch
-restatement (``_host.py``), the reference's own dispatch on ``is_cuda``
-(grid.py:91, ungrid.py:86).
+The arithmetic runs in the HIP kernels of libtnl (tnl_ungrid / tnl_grid,
+and tnl_ungrid_f64 / tnl_grid_f64 for float64 / complex128, which compute
+in double like the reference's torch path) for tensors on a HIP device;
+host tensors take the package's torch restatement (``_host.py``), the
+reference's own dispatch on ``is_cuda`` (grid.py:91, ungrid.py:86).
 
 Beyond the reference, the autograd pair also differentiates with respect to
 ``locs`` (the reference returns None there, interp.py:46-70, 157-181): the
@@ -97,15 +98,30 @@ def _check_block_width(width: tuple, is_complex: bool) -> None:
         )
 
 
+_F64 = (torch.float64, torch.complex128)
+
+
 def _check_dtypes(vals: torch.Tensor, locs: torch.Tensor) -> None:
-    ok = (torch.float32, torch.complex64)
-    if not vals.is_cuda:
-        ok = ok + (torch.float64, torch.complex128)
+    ok = (torch.float32, torch.complex64) + _F64
     if vals.dtype not in ok:
-        where = "HIP kernels compute" if vals.is_cuda else "interpolation computes"
-        raise TypeError(f"the {where} in {'/'.join(str(d)[6:] for d in ok)}; got {vals.dtype}")
+        raise TypeError(f"interpolation computes in {'/'.join(str(d)[6:] for d in ok)}; "
+                        f"got {vals.dtype}")
     if not locs.is_floating_point():
         raise TypeError(f"locs must be floating point, got {locs.dtype}")
+
+
+def _launch_f64(adjoint: bool, vals_flat, locs, out, nbatch, npts, ndim, grid_size, width,
+                kernel, norm, pad_mode, is_complex, kernel_params):
+    """Double-precision HIP kernels (tnl_ungrid_f64 / tnl_grid_f64): no plan,
+    windows and sums in double."""
+    _check_options(kernel, norm, pad_mode)
+    locs2d = locs.reshape(npts, ndim).to(torch.float64).contiguous()
+    fn = lib().tnl_grid_f64 if adjoint else lib().tnl_ungrid_f64
+    with torch.cuda.device(vals_flat.device):
+        check(fn(vals_flat.data_ptr(), locs2d.data_ptr(), out.data_ptr(), nbatch, npts, ndim,
+                 i64_array(grid_size), f32_array(width), KERNELS[kernel], int(norm),
+                 PAD_MODES[pad_mode], int(is_complex), _beta(kernel, kernel_params),
+                 stream_of(vals_flat)))
 
 
 def _same_device(op: str, *tensors) -> None:
@@ -200,13 +216,18 @@ def ungrid(
         out = ungrid_host(vals_flat, locs.reshape(npts, ndim), grid_size, width, kernel,
                           norm, pad_mode, _beta(kernel, kernel_params))
         return out.reshape(*batch_shape, *locs_batch_shape)
-    locs2d = locs.reshape(npts, ndim).to(torch.float32).contiguous()
     out = torch.empty(nbatch, npts, dtype=vals.dtype, device=vals.device)
     if is_complex:
         vals_flat = torch.view_as_real(vals_flat)
         out_r = torch.view_as_real(out)
     else:
         out_r = out
+    if vals.dtype in _F64:
+        if nbatch > 0 and npts > 0:
+            _launch_f64(False, vals_flat, locs, out_r, nbatch, npts, ndim, grid_size, width,
+                        kernel, norm, pad_mode, is_complex, kernel_params)
+        return out.reshape(*batch_shape, *locs_batch_shape)
+    locs2d = locs.reshape(npts, ndim).to(torch.float32).contiguous()
     if nbatch > 0 and npts > 0:
         _launch(False, vals_flat, locs, locs2d, out_r, nbatch, npts, ndim,
                 grid_size, width, kernel, norm, pad_mode, is_complex, kernel_params)
@@ -256,13 +277,18 @@ def grid(
         out = grid_host(vals_flat, locs.reshape(npts, ndim), grid_size, width, kernel,
                         norm, pad_mode, _beta(kernel, kernel_params))
         return out.reshape(*batch_shape, *grid_size)
-    locs2d = locs.reshape(npts, ndim).to(torch.float32).contiguous()
     out = torch.empty(nbatch, *grid_size, dtype=vals.dtype, device=vals.device)
     if is_complex:
         vals_flat = torch.view_as_real(vals_flat)
         out_r = torch.view_as_real(out)
     else:
         out_r = out
+    if vals.dtype in _F64:
+        if nbatch > 0:
+            _launch_f64(True, vals_flat, locs, out_r, nbatch, npts, ndim, grid_size, width,
+                        kernel, norm, pad_mode, is_complex, kernel_params)
+        return out.reshape(*batch_shape, *grid_size)
+    locs2d = locs.reshape(npts, ndim).to(torch.float32).contiguous()
     if nbatch > 0:
         _launch(True, vals_flat, locs, locs2d, out_r, nbatch, npts, ndim,
                 grid_size, width, kernel, norm, pad_mode, is_complex, kernel_params)
@@ -307,6 +333,9 @@ def interp_grad_locs(
     _same_device("interp_grad_locs", grid_vals, pt_vals, locs)
     _check_dtypes(grid_vals, locs)
     _check_dtypes(pt_vals, locs)
+    if grid_vals.is_cuda and (grid_vals.dtype in _F64 or pt_vals.dtype in _F64):
+        raise TypeError("the trajectory gradient's HIP kernel computes in float32/complex64; "
+                        "use float32/complex64 device tensors or float64 host tensors")
     g = grid_vals.reshape(nbatch, *grid_size).contiguous()
     h = pt_vals.reshape(nbatch, npts).contiguous()
     if not grid_vals.is_cuda:
