@@ -35,6 +35,11 @@ extern "C" {
 #define TNL_PAD_ZERO 1             /* pad_mode="zero"                    */
 #define TNL_F32 0
 #define TNL_F64 1
+#define TNL_I32 2  /* integer element types: tnl_index_adjoint only */
+#define TNL_I64 3
+#define TNL_I16 4
+#define TNL_I8 5
+#define TNL_U8 6
 
 /* Largest per-dimension tap count ceil(w)+1 the kernels accept.  The
  * reference rejects next_pow2(ceil(w+1)) > 64 (doubled for complex data)
@@ -394,7 +399,9 @@ int tnl_index(const void* x, const int64_t* idx, void* out, int elem_bytes,
  * `y[idx[p]] += x[p]`.  Cells with more points must all be listed in
  * heavy [nheavy]; each is summed by one workgroup in a fixed order
  * (deterministic, within rounding of the sequential sum).  dtype TNL_F32
- * or TNL_F64, is_complex for interleaved (re, im) pairs.  out is fully
+ * or TNL_F64, is_complex for interleaved (re, im) pairs; or an integer type
+ * (TNL_I8, TNL_U8, TNL_I16, TNL_I32, TNL_I64; real only), summed exactly and
+ * wrapping like the reference's index_add_ of that type.  out is fully
  * written (no zeroing needed, no atomics).
  */
 int tnl_index_adjoint(const void* x, const int64_t* order, const int64_t* starts,

@@ -113,6 +113,30 @@ def test_index_adjoint_heavy_cells(dev, dtype):
     assert err < (2e-6 if dtype == torch.complex64 else 1e-14), err
 
 
+@pytest.mark.parametrize("dtype", [torch.int8, torch.uint8, torch.int16, torch.int32,
+                                   torch.int64])
+def test_index_adjoint_integer_types(dev, dtype):
+    """Integer scatter-add is exact and wraps like the sequential sum of the
+    same type, on light and heavy (workgroup-summed) cells alike."""
+    print(f"index_adjoint integer {dtype}", flush=True)
+    g = torch.Generator().manual_seed(17)
+    grid, npts = (6, 5), 6000  # ~200 points per cell: every cell heavy
+    info = torch.iinfo(dtype)
+    # values near the type's range so that the sums wrap many times over
+    lo, hi = max(info.min, -(1 << 62)), min(info.max, 1 << 62)
+    y = torch.randint(lo, hi, (3, npts), generator=g, dtype=torch.int64).to(dtype)
+    idx = tuple(torch.randint(0, n, (npts,), generator=g) for n in grid)
+    light = (torch.randint(0, 40, (3, 50), generator=g, dtype=torch.int64).to(dtype),
+             tuple(torch.randint(0, n, (50,), generator=g) for n in (9, 11)))
+    for vals, ix, gs in [(y, idx, grid), (light[0], light[1], (9, 11))]:
+        got = F.index_adjoint(vals.to(dev), tuple(i.to(dev) for i in ix), gs)
+        assert got.dtype == dtype
+        want = O.index_adjoint(vals.numpy(), tuple(i.numpy() for i in ix), gs)
+        assert np.array_equal(got.cpu().numpy(), want)
+    counts = np.bincount(O.ravel(tuple(i.numpy() for i in idx), grid).reshape(-1))
+    assert counts.min() > FI.HEAVY_MIN
+
+
 def test_index_adjoint_is_deterministic(dev):
     g = torch.Generator().manual_seed(3)
     y = torch.randn(4, 20000, dtype=torch.complex64, generator=g).to(dev)

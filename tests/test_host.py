@@ -107,7 +107,13 @@ def test_index_and_wrap_entry_points_validate_before_launch():
     with pytest.raises(ValueError, match="heavy"):
         _lib.check(h.tnl_index_adjoint(1, 1, 1, 1, 2, 0, 1, 0, 1, 1, 8, 4, None))
     with pytest.raises(ValueError, match="dtype"):
-        _lib.check(h.tnl_index_adjoint(1, 1, 1, None, 0, 0, 1, 5, 1, 1, 8, 4, None))
+        _lib.check(h.tnl_index_adjoint(1, 1, 1, None, 0, 0, 1, 9, 1, 1, 8, 4, None))
+    with pytest.raises(ValueError, match="no complex form"):
+        _lib.check(h.tnl_index_adjoint(1, 1, 1, None, 0, 0, 1, _lib.TNL_I32, 1, 1, 8, 4, None))
+    for code in (_lib.TNL_I8, _lib.TNL_U8, _lib.TNL_I16, _lib.TNL_I32, _lib.TNL_I64):
+        # integer element types are accepted (no batch: no work)
+        assert h.tnl_index_adjoint(None, None, None, None, 0, 0, None, code, 0, 0, 8, 4,
+                                   None) == 0
     im, pad = _lib.i64_array([8, 8]), _lib.i64_array([16, 6])
     with pytest.raises(ValueError, match="im_size <= pad_size"):
         _lib.check(h.tnl_pad_wrapped(1, 1, 8, 1, 2, im, pad, None, None))

@@ -288,12 +288,36 @@ extern "C" int tnl_index_adjoint(const void* x, const int64_t* order,
     return fail(TNL_EINVAL, "nbatch, ngrid, npts and nheavy must be >= 0");
   if (nheavy > 0 && (heavy_min < 1 || !heavy))
     return fail(TNL_EINVAL, "a heavy-cell list needs heavy_min >= 1 and a pointer");
-  if (dtype != TNL_F32 && dtype != TNL_F64)
-    return fail(TNL_EINVAL, "index_adjoint dtype must be float32 or float64");
+  const bool integer = dtype == TNL_I8 || dtype == TNL_U8 || dtype == TNL_I16 ||
+                       dtype == TNL_I32 || dtype == TNL_I64;
+  if (dtype != TNL_F32 && dtype != TNL_F64 && !integer)
+    return fail(TNL_EINVAL, "index_adjoint dtype must be float32, float64 or an integer type");
+  if (integer && is_complex) return fail(TNL_EINVAL, "integer index_adjoint has no complex form");
   if (nbatch == 0 || ngrid == 0) return TNL_OK;
   if (!out || !starts || (npts > 0 && (!x || !order)))
     return fail(TNL_EINVAL, "null pointer");
   hipStream_t s = (hipStream_t)stream;
+  // integer sums are exact and order-free, wrapping like the sequential
+  // index_add_ of the same type; they run in the unsigned type of the same
+  // width, whose addition is the two's-complement sum bit for bit (signed
+  // overflow would be undefined behaviour)
+  switch (dtype) {
+    case TNL_I8:
+    case TNL_U8:
+      return launch_adjoint<uint8_t, 1>(x, order, starts, heavy, nheavy, heavy_min, out, nbatch,
+                                        ngrid, npts, s);
+    case TNL_I16:
+      return launch_adjoint<uint16_t, 1>(x, order, starts, heavy, nheavy, heavy_min, out,
+                                         nbatch, ngrid, npts, s);
+    case TNL_I32:
+      return launch_adjoint<uint32_t, 1>(x, order, starts, heavy, nheavy, heavy_min, out,
+                                         nbatch, ngrid, npts, s);
+    case TNL_I64:
+      return launch_adjoint<uint64_t, 1>(x, order, starts, heavy, nheavy, heavy_min, out,
+                                         nbatch, ngrid, npts, s);
+    default:
+      break;
+  }
   if (dtype == TNL_F32)
     return is_complex ? launch_adjoint<float, 2>(x, order, starts, heavy, nheavy, heavy_min,
                                                  out, nbatch, ngrid, npts, s)

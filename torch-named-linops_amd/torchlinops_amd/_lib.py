@@ -38,6 +38,7 @@ TNL_EWORKSPACE = 4
 KERNELS = {"kaiser_bessel": 0, "spline": 1}
 PAD_MODES = {"circular": 0, "zero": 1}
 TNL_F32, TNL_F64 = 0, 1
+TNL_I32, TNL_I64, TNL_I16, TNL_I8, TNL_U8 = 2, 3, 4, 5, 6
 
 _i64 = ctypes.c_int64
 _i32 = ctypes.c_int

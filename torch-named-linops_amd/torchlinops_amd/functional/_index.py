@@ -30,7 +30,7 @@ import torch
 from torch import Tensor
 from torch.autograd import Function
 
-from .._lib import TNL_F32, TNL_F64, check, lib, stream_of
+from .._lib import TNL_F32, TNL_F64, TNL_I8, TNL_I16, TNL_I32, TNL_I64, TNL_U8, check, lib, stream_of
 from ._host import gather_host, scatter_add_host
 
 __all__ = [
@@ -180,13 +180,20 @@ def index_map(idx: tuple, grid_size: tuple, device=None, flat: Tensor = None) ->
 # ---- HIP launches --------------------------------------------------------
 
 
+_INT_CODES = {torch.int32: TNL_I32, torch.int64: TNL_I64, torch.int16: TNL_I16,
+              torch.int8: TNL_I8, torch.uint8: TNL_U8}
+
+
 def _real_dtype_code(dtype: torch.dtype) -> tuple:
     if dtype in (torch.float32, torch.complex64):
         return TNL_F32, dtype.is_complex
     if dtype in (torch.float64, torch.complex128):
         return TNL_F64, dtype.is_complex
+    if dtype in _INT_CODES:
+        return _INT_CODES[dtype], False
     raise TypeError(
-        f"index_adjoint supports float32/float64/complex64/complex128, got {dtype}"
+        f"index_adjoint supports float32/float64/complex64/complex128 and int8/uint8/int16/"
+        f"int32/int64, got {dtype}"
     )
 
 
