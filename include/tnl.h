@@ -106,7 +106,13 @@ typedef struct tnl_plan_layout {
   int64_t off_itemoff, off_items;
   int64_t off_items_tmp; /* staging for the item ordering during the build
                           * (3-D plans list their work items largest first)  */
-  int64_t bytes;         /* total workspace bytes                             */
+  int64_t off_tpm;       /* 3-D matrix-core plans (8^3 tiles, <= 5 taps, every
+                          * axis 16..1024 cells): per touch (item order) the
+                          * caller index of the point and the union of the
+                          * tile's MFMA blocks that its group of 4 touches
+                          * (counted from the item's first) covers, as int32
+                          * pairs; 0 when absent                            */
+  int64_t bytes;        /* total workspace bytes                             */
 } tnl_plan_layout;
 
 /* Fill *out for a trajectory of npts points; tuning may be NULL. */
@@ -168,9 +174,11 @@ int tnl_grid_f64(const double* vals, const double* locs, double* out, int64_t nb
 /* Scratch bytes that tnl_grid_ws uses with this plan for nbatch channels:
  * the channel values re-packed point-major, one 64-byte record (up to 8
  * complex / 16 real channel values) per point and channel chunk, for the
- * matrix-core gridding kernel (plans whose grid_accum is
- * TNL_GRID_ACCUM_MFMA); 0 for other plans (tnl_grid_ws then equals
- * tnl_grid). */
+ * matrix-core gridding kernels (plans whose grid_accum is
+ * TNL_GRID_ACCUM_MFMA), plus, for plans with touch pairs (off_tpm != 0),
+ * one 64-byte window record per point (its tap weights per axis and first
+ * taps, evaluated once per call instead of once per touched tile); 0 for
+ * other plans (tnl_grid_ws then equals tnl_grid). */
 int tnl_grid_scratch_bytes(const tnl_plan_layout* layout, int64_t nbatch,
                            int is_complex, int64_t* bytes);
 
@@ -181,7 +189,9 @@ int tnl_grid_scratch_bytes(const tnl_plan_layout* layout, int64_t nbatch,
  * coil-major input.  No counterpart in the reference: its Triton kernel
  * reads the channel values of each point where they lie
  * (src/torchlinops/functional/_interp/grid.py:161-513).  Same results as
- * tnl_grid. */
+ * tnl_grid, bit for bit, except for 3-D plans with touch pairs (off_tpm),
+ * where one wave accumulates a whole tile in its own order: equal to
+ * float32 rounding. */
 int tnl_grid_ws(const float* vals, const float* locs, float* out, int64_t nbatch,
                 int64_t npts, int ndim, const int64_t* grid_size,
                 const float* width, int kernel, int norm, int pad_mode,
@@ -190,7 +200,8 @@ int tnl_grid_ws(const float* vals, const float* locs, float* out, int64_t nbatch
                 void* stream);
 
 /* Largest scratch (bytes) tnl_grid_static can use: 64 MiB, i.e. 1 048 576
- * point records (points x channel chunks of 8 complex / 16 real). */
+ * 64-byte records (points x channel chunks of 8 complex / 16 real, plus one
+ * window record per point for plans with touch pairs). */
 #define TNL_GRID_STATIC_SCRATCH_BYTES (64ll << 20)
 
 /* tnl_grid_ws without a workspace argument, for callers that cannot
@@ -203,7 +214,8 @@ int tnl_grid_ws(const float* vals, const float* locs, float* out, int64_t nbatch
  * nothing; every call on one GPU shares that GPU's array (its contents are
  * rewritten by each call, nothing needs to start at zero), so calls that
  * overlap on one GPU (two streams, two host threads) race and must be
- * serialised by the caller.  Same arguments and results as tnl_grid. */
+ * serialised by the caller.  Same arguments as tnl_grid and the same
+ * results as tnl_grid_ws. */
 int tnl_grid_static(const float* vals, const float* locs, float* out, int64_t nbatch,
                     int64_t npts, int ndim, const int64_t* grid_size,
                     const float* width, int kernel, int norm, int pad_mode,

@@ -298,10 +298,13 @@ def test_plan_sort_forms_agree(dev, n):
     ((96, 80), True, 11), ((96, 80), False, 17)])
 def test_grid_ws_records_match_direct_reads(dev, n, cplx, nbatch):
     """tnl_grid_ws (values re-packed point-major into the caller's scratch)
-    gives bit-identical results to tnl_grid (values read from the
-    coil-major input) through the C ABI: same kernel, same summation
-    order (no split tiles here), also with a partial last channel chunk;
-    a scratch below tnl_grid_scratch_bytes is rejected."""
+    against tnl_grid (values read from the coil-major input) through the C
+    ABI, also with a partial last channel chunk.  2-D: the same kernel and
+    summation order (no split tiles here), bit-identical.  3-D plans carry
+    touch pairs, so tnl_grid_ws takes the one-wave kernel with its window
+    records (one accumulator per tile instead of two wave pairs'): equal
+    to float rounding and both within the bar of the float64 oracle.  A
+    scratch below tnl_grid_scratch_bytes is rejected."""
     import ctypes
     from torchlinops_amd import _lib
     from torchlinops_amd.functional import _plan
@@ -319,7 +322,9 @@ def test_grid_ws_records_match_direct_reads(dev, n, cplx, nbatch):
     nbytes = ctypes.c_int64(0)
     _lib.check(h.tnl_grid_scratch_bytes(plan.layout, nbatch, int(cplx), ctypes.byref(nbytes)))
     ch = 8 if cplx else 16
-    assert nbytes.value == -(-nbatch // ch) * 5000 * 64
+    wave = plan.layout.off_tpm != 0
+    assert wave == (nd == 3)
+    assert nbytes.value == (-(-nbatch // ch) + (1 if wave else 0)) * 5000 * 64
     outs = []
     for scratch in (None, torch.empty(nbytes.value, dtype=torch.uint8, device=dev)):
         out = torch.empty(nbatch, *n, dtype=yv.dtype, device=dev)
@@ -333,8 +338,12 @@ def test_grid_ws_records_match_direct_reads(dev, n, cplx, nbatch):
             _lib.check(h.tnl_grid_ws(*args, scratch.data_ptr(), nbytes.value, _lib.stream_of(yv)))
         torch.cuda.synchronize()
         outs.append(out)
-    assert torch.equal(outs[0], outs[1])
+    if wave:
+        assert rel_l2(outs[1].cpu().numpy(), outs[0].cpu().numpy()) < 1e-6
+    else:
+        assert torch.equal(outs[0], outs[1])
     ref = O.grid(y, locs.cpu().numpy(), n, 4.0, "kaiser_bessel", 1, "circular", 1.7, "f64")
+    assert rel_l2(outs[0].cpu().numpy(), ref) < TOL
     assert rel_l2(outs[1].cpu().numpy(), ref) < TOL
     small = torch.empty(nbytes.value - 16, dtype=torch.uint8, device=dev)
     with pytest.raises(ValueError, match="scratch"):

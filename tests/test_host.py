@@ -39,7 +39,7 @@ def test_plan_layout_is_host_only_and_consistent():
     assert L.touch_capacity == 8 * 10_240_000
     offs = [L.off_counts, L.off_starts, L.off_cursor, L.off_perm, L.off_slocs,
             L.off_suboff, L.off_sub, L.off_gcounts, L.off_gstarts, L.off_gcursor,
-            L.off_tlist, L.off_itemoff, L.off_items, L.off_items_tmp]
+            L.off_tlist, L.off_itemoff, L.off_items, L.off_items_tmp, L.off_tpm]
     assert tuple(L.grid_size) == (256, 256, 256) and L.lds_budget == 53248
     assert all(o % 256 == 0 for o in offs)
     assert offs == sorted(offs) and L.bytes > offs[-1]
@@ -54,6 +54,9 @@ def test_plan_layout_is_host_only_and_consistent():
         plan_layout(10, 2, (8, 8), (4.0, 4.0), 0, tuning={"lds_budget": 1 << 20})
     # gridding accumulator: matrix cores for 3-D 8x8x8 tiles, LDS otherwise
     assert L.grid_accum == 2 and L2.grid_accum == 1
+    # touch pairs of the one-wave gridding kernel: 3-D matrix-core plans only
+    assert L.off_tpm > 0 and L2.off_tpm == 0
+    assert L.bytes - L.off_tpm >= 8 * (L.touch_capacity + 4)
     L4 = plan_layout(1000, 3, (64, 64, 64), (4.0,) * 3, 0, tuning={"grid_accum": "lds"})
     assert L4.grid_accum == 1
     L5 = plan_layout(1000, 3, (64, 64, 64), (4.0,) * 3, 0, tuning={"g_tile": (8, 8, 16)})
@@ -214,7 +217,9 @@ def test_grid_static_rejects_plans_beyond_its_scratch():
     """tnl_grid_static refuses, before any device work, a plan whose point
     records exceed TNL_GRID_STATIC_SCRATCH_BYTES (64 MiB = 1 048 576
     records of 64 bytes); within the limit it needs the records a plan's
-    tnl_grid_scratch_bytes reports."""
+    tnl_grid_scratch_bytes reports: one channel record per point and chunk
+    of 8 complex channels, plus one window record per point for plans with
+    touch pairs (3-D, 8x8x8 tiles)."""
     import ctypes
 
     from torchlinops_amd import _lib
@@ -226,7 +231,11 @@ def test_grid_static_rejects_plans_beyond_its_scratch():
     L = plan_layout(1_100_000, 3, (64, 64, 64), (4.0,) * 3, 0)  # 8x8x8 tiles: matrix cores
     need = ctypes.c_int64(0)
     _lib.check(h.tnl_grid_scratch_bytes(L, 8, 1, ctypes.byref(need)))
-    assert need.value == 1_100_000 * 64 > (64 << 20)
+    assert L.off_tpm > 0
+    assert need.value == 1_100_000 * 64 * 2 > (64 << 20)
+    L2 = plan_layout(1_100_000, 2, (256, 256), (4.0,) * 2, 0)  # no touch pairs in 2-D
+    _lib.check(h.tnl_grid_scratch_bytes(L2, 8, 1, ctypes.byref(need)))
+    assert L2.off_tpm == 0 and need.value == 1_100_000 * 64
     args = [None, None, None, 8, 1_100_000, 3, _lib.i64_array((64, 64, 64)),
             _lib.f32_array((4.0,) * 3), 0, 1, 0, 1, 8.99, L, None, None]
     with pytest.raises(ValueError, match="more than the static 67108864 bytes"):

@@ -1495,6 +1495,11 @@ static int launch_grid_mfma(const float* vals, const float* locs, float* out, in
         vals, L.npts, nbatch, rec);
     TNL_LAUNCH_CHECK("pack_points");
   }
+  if (NDIM == 3 && rec && grid_wave_ok(L)) {
+    const int64_t ch = CPLX ? MF_NPL / 2 : MF_NPL;
+    float* wrec = scratch + (nbatch + ch - 1) / ch * L.npts * MF_REC4 * 4;
+    return grid_wave_launch(vals, locs, out, nbatch, G, L, ws, scratch, wrec, CPLX ? 1 : 0, s);
+  }
   const auto kern = rec ? grid_mfma_kernel<NDIM, CPLX, true> : grid_mfma_kernel<NDIM, CPLX, false>;
   constexpr size_t lds = mf_lds_bytes<NDIM>();
   if (int rc = allow_lds((const void*)kern, lds)) return rc;
@@ -1514,7 +1519,8 @@ int64_t grid_scratch_bytes(const tnl_plan_layout& L, int64_t nbatch, int is_comp
   if (L.ndim < 2 || L.grid_accum != TNL_GRID_ACCUM_MFMA || nbatch <= 0 || L.npts <= 0)
     return 0;
   const int64_t ch = is_complex ? MF_NPL / 2 : MF_NPL;
-  return (nbatch + ch - 1) / ch * L.npts * MF_REC4 * (int64_t)sizeof(float4);
+  const int64_t recs = (nbatch + ch - 1) / ch * L.npts * MF_REC4 * (int64_t)sizeof(float4);
+  return recs + (grid_wave_ok(L) ? grid_wave_wrec_bytes(L) : 0);
 }
 
 template <int NDIM, bool CPLX, int CC, int NT>

@@ -1,0 +1,413 @@
+// 3-D gridding on the matrix cores, one wave per work item ("wave
+// gridding").  Used for plans whose gridding tiles are 8x8x8 cells and whose
+// work items carry (point, block mask) touch pairs (plan sections `tpm`),
+// when the caller passes the scratch of tnl_grid_ws.
+//
+// Per call, two pre-passes fill the scratch:
+//   * point-major channel records (pack_points_kernel, shared with the
+//     4-wave kernel): 16 planes (8 complex / 16 real channels) per point and
+//     channel chunk, 64 bytes;
+//   * window records (window_rec_kernel): per point the weights of its kept
+//     taps on each axis (5 slots per axis, zero past the kept taps) and the
+//     first kept tap of each axis wrapped into [0, n), 10 bits each; 64
+//     bytes.
+// A wave owns one (work item, channel chunk) and the whole 8x8x8 x 16-plane
+// accumulator of its tile in registers: 32 blocks of 16 cells (one x plane,
+// a 4x4 (y, z) patch) x 16 planes, each one v_mfma_f32_16x16x4f32 tile.
+// Touches are taken 4 at a time (the MFMA's K): lane (k, n) = (lane / 16,
+// lane % 16) belongs to touch k, plane n.  Separability splits the operands:
+//   A[cell (y, z) of patch P][k] = wy_k(y) * wz_k(z)    (4 values: 4 patches)
+//   B_x[k][n]                    = wx_k(x) * v_k[n]     (8 values: 8 planes)
+// so a group costs 12 multiplies for all of its MFMAs.  Each touch's 1-D
+// windows are placed at their tile-local cells through a small per-wave LDS
+// table: rows of 24 slots per (touch, axis), slots 4..19 cleared, the
+// touch's 5 weights written at slot 8 + (first tap - tile origin) + k, and
+// slots 8..15 (the tile's 8 cells) read back, so no lane selects among
+// registers.  Each touch pair carries the union of the block masks of its
+// group (written by the plan), and the group's MFMAs cover that union.  The
+// records of group g + 1 and the touch pairs of group g + 2 load while group
+// g is processed.  At the end of the item the
+// accumulators go to HBM one x plane at a time through an LDS stage
+// (64-byte z runs), or with float atomics for split tiles (their region is
+// zeroed by the caller's pre-pass).  No barriers: the waves of a CU never
+// share LDS.
+#include "tnl_common.hpp"
+#include "tnl_internal.hpp"
+
+namespace tnl {
+
+namespace {
+
+typedef float wv32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int WG_NPL = 16;          // planes of a channel chunk
+constexpr int WG_REC = 16;          // floats per channel / window record
+constexpr int WG_ROW = 24;          // LDS slots per (touch, axis) window row
+constexpr int WG_WIN = 4 * 3 * WG_ROW;
+constexpr int WG_PS = 68;           // LDS stage stride per plane (floats)
+constexpr int WG_RUN = 8;           // consecutive work units given to one XCD
+constexpr int WG_BG = 8;            // groups of 4 touches per prefetch batch
+constexpr int WG_BT = 4 * WG_BG;    // touches per batch (one lane per record)
+constexpr int WG_XCD = 8;
+
+// Window record of every point (caller order): weights of the kept taps of
+// each axis at slots 5 d + k (zero for k past the kept taps) and, in slot
+// 15, the first kept tap of each axis wrapped into [0, n), bits 10 d.  The
+// weights are the ones the 4-wave kernel evaluates per touch
+// (mf_axis_place): window(min(|(g0 + k) - t| * 2 / w, 1)).
+__global__ __launch_bounds__(256) void window_rec_kernel(const float* __restrict__ locs,
+                                                         int64_t npts, InterpGeom G,
+                                                         float4* __restrict__ wrec) {
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < npts;
+       p += (int64_t)gridDim.x * 256) {
+    float w[WG_REC];
+    unsigned packed = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float t = locs[p * 3 + d];
+      const TapRange r = tap_range(t, G.half[d], G.ntap[d], G.n[d], G.pad);
+      int64_t a = r.g0;
+      if (G.pad == TNL_PAD_CIRCULAR) {
+        if (a < 0) a += G.n[d];
+        else if (a >= G.n[d]) a -= G.n[d];
+        if (a < 0 || a >= G.n[d]) a = mod_pos(r.g0, G.n[d]);
+      }
+      if (r.count == 0) a = 0;
+      const float g0f = (float)r.g0;
+#pragma unroll
+      for (int k = 0; k < 5; ++k)
+        w[5 * d + k] = k < r.count ? window(G.kernel,
+                                            fminf(fabsf((g0f + (float)k) - t) * G.inv_half[d],
+                                                  1.0f),
+                                            G.beta)
+                                   : 0.0f;
+      packed |= (unsigned)a << (10 * d);
+    }
+    w[15] = __uint_as_float(packed);
+    float4* o = wrec + p * 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o[i] = make_float4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+  }
+}
+
+struct WaveGeom {
+  int n[3];
+  int ntiles[3];
+  int64_t ncells;
+  int64_t npts;
+  int64_t nbatch;
+  int nch;   // channel chunks
+  int circ;  // circular padding
+};
+
+// One wave per (work item, channel chunk).
+template <bool CPLX>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 3))) void grid_wave_kernel(
+    float* __restrict__ out, WaveGeom W, const int2* __restrict__ tpm,
+    const float* __restrict__ wrec, const float* __restrict__ rec,
+    const int4* __restrict__ items, const int* __restrict__ nitems_ptr) {
+  constexpr int CH = CPLX ? WG_NPL / 2 : WG_NPL;
+  // records of two batches of WG_BT touches, as the direct-to-LDS loads
+  // leave them: [buffer][16-byte chunk i][lane][4], lane = kind * 32 + touch
+  // (kind 0: window record, 1: channel record); the write-out stage reuses
+  // them once every load has landed
+  // (two separate arrays, so that the compiler's wait analysis can tell
+  // the buffer being read from the one being loaded)
+  __shared__ __align__(16) float rb0[4][64][4];
+  __shared__ __align__(16) float rb1[4][64][4];
+  __shared__ __align__(16) float lw[WG_WIN];
+  __shared__ __align__(16) float stg[WG_NPL * WG_PS];
+
+  const int64_t nunits = (int64_t)(*nitems_ptr) * W.nch;
+  // runs of WG_RUN consecutive units on one XCD (the dispatcher hands
+  // consecutive workgroups to the XCDs in turn)
+  const int64_t bid = blockIdx.x;
+  const int64_t rnd = bid / (WG_XCD * WG_RUN), j = bid - rnd * (WG_XCD * WG_RUN);
+  const int64_t unit = rnd * (WG_XCD * WG_RUN) + (j % WG_XCD) * WG_RUN + j / WG_XCD;
+  if (unit >= nunits) return;
+  const int64_t itx = unit / W.nch;
+  const int ch = (int)(unit - itx * W.nch);
+  const int64_t c0 = (int64_t)ch * CH;
+  const int4 item = items[itx];
+  const int start = item.y, count = item.z;
+
+  int origin[3];
+  {
+    int r = item.x;
+    for (int d = 2; d >= 0; --d) {
+      origin[d] = (r % W.ntiles[d]) * 8;
+      r /= W.ntiles[d];
+    }
+  }
+
+  const int lane = threadIdx.x;
+  const int k4 = lane >> 4, n16 = lane & 15;
+  const int yy = n16 >> 2, zz = n16 & 3;
+  // this lane's weight slot: axis wa, tap wk (lanes n16 < 15)
+  const int wa = n16 >= 10 ? 2 : (n16 >= 5 ? 1 : 0);
+  const int wk = n16 - 5 * wa;
+  const int w_origin = wa == 0 ? origin[0] : (wa == 1 ? origin[1] : origin[2]);
+  const int w_n = wa == 0 ? W.n[0] : (wa == 1 ? W.n[1] : W.n[2]);
+  float* const wrow = lw + (k4 * 3 + wa) * WG_ROW;
+
+  // clear the window table once (slots outside 4..19 are never written)
+  for (int i = lane; i < WG_WIN / 4; i += 64)
+    reinterpret_cast<float4*>(lw)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+
+  wv32x4 acc[32];
+#pragma unroll
+  for (int b = 0; b < 32; ++b) acc[b] = (wv32x4){0.f, 0.f, 0.f, 0.f};
+
+  const int nbatches = (count + WG_BT - 1) / WG_BT;
+  // Touch pair of this lane's touch (lane % 32) of batch b: the caller
+  // index and the union of the block masks of the touch's group of 4 (the
+  // plan writes it to all 4 pairs).  Touches past the item's last read its
+  // last pair.
+  auto pair_of = [&](int b) {
+    int i = WG_BT * b + (lane & (WG_BT - 1));
+    i = i < count ? i : count - 1;
+    return tpm[start + i];
+  };
+  // Issue the direct-to-LDS loads of batch b's records into buffer buf:
+  // lane (kind, touch) copies one 64-byte record in four 16-byte chunks.
+  // Touches past the last copy point 0's records (never read).
+  auto issue = [&](int b, int2 t, float (*dst)[64][4]) {
+    const bool live = WG_BT * b + (lane & (WG_BT - 1)) < count;
+    const int64_t p = live ? t.x : 0;
+    const float* src = lane < WG_BT ? wrec + p * WG_REC
+                                    : rec + ((int64_t)ch * W.npts + p) * WG_REC;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      __builtin_amdgcn_global_load_lds(src + 4 * i, &dst[i][0][0], 16, 0, 0);
+  };
+  // Group gg of the batch in buffer rb; gm = the group's block mask.
+  auto process = [&](const float (*rb)[64][4], int gg, unsigned gm, bool live) {
+    const int t = 4 * gg + k4;
+    const float cw = rb[n16 >> 2][t][n16 & 3];
+    const unsigned cbase = __float_as_uint(rb[3][t][3]);
+    const float cv = live ? rb[n16 >> 2][WG_BT + t][n16 & 3] : 0.f;
+    // stage the group's windows: clear slots 4..19 of the 12 rows, then
+    // place each weight at its tile-local slot
+    if (lane < 48)
+      *reinterpret_cast<float4*>(lw + (lane >> 2) * WG_ROW + 4 + 4 * (lane & 3)) =
+          make_float4(0.f, 0.f, 0.f, 0.f);
+    if (n16 < 15 && live) {
+      int d = (int)((cbase >> (10 * wa)) & 1023u) - w_origin;
+      if (W.circ && d > 7) d -= w_n;
+      int slot = 8 + d + wk;
+      slot = slot < 0 ? 0 : (slot > WG_ROW - 1 ? WG_ROW - 1 : slot);
+      wrow[slot] = cw;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const float* rx = lw + (k4 * 3) * WG_ROW + 8;
+    const float4 x0 = *reinterpret_cast<const float4*>(rx);
+    const float4 x1 = *reinterpret_cast<const float4*>(rx + 4);
+    const float* ry = lw + (k4 * 3 + 1) * WG_ROW + 8 + yy;
+    const float* rz = lw + (k4 * 3 + 2) * WG_ROW + 8 + zz;
+    const float wy0 = ry[0], wy1 = ry[4], wz0 = rz[0], wz1 = rz[4];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const float A[4] = {wy0 * wz0, wy0 * wz1, wy1 * wz0, wy1 * wz1};
+    const float wx[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#ifndef WG_MODE
+#define WG_MODE 1
+#endif
+#if WG_MODE == 0
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const unsigned pm = (gm >> (4 * u)) & 15u;
+      if (pm) {
+        const float b = wx[u] * cv;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if ((pm >> q) & 1u)
+            acc[4 * u + q] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[q], b, acc[4 * u + q], 0, 0, 0);
+      }
+    }
+#elif WG_MODE == 1
+    // (x half, patch) decisions: the 4 planes of a live half, zero weights
+    // where a plane is outside every touch's footprint
+    float B[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) B[u] = wx[u] * cv;
+    const unsigned lo = gm & 0xFFFFu, hi = gm >> 16;
+    const unsigned hq[2] = {(lo | (lo >> 4) | (lo >> 8) | (lo >> 12)) & 15u,
+                            (hi | (hi >> 4) | (hi >> 8) | (hi >> 12)) & 15u};
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if ((hq[h] >> q) & 1u) {
+#pragma unroll
+          for (int u = 4 * h; u < 4 * h + 4; ++u)
+            acc[4 * u + q] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[q], B[u], acc[4 * u + q], 0, 0, 0);
+        }
+#else
+    // patch decisions only: all 8 planes of a live patch
+    float B[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) B[u] = wx[u] * cv;
+    const unsigned all = (gm | (gm >> 4) | (gm >> 8) | (gm >> 12) | (gm >> 16) | (gm >> 20) |
+                          (gm >> 24) | (gm >> 28)) & 15u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if ((all >> q) & 1u) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          acc[4 * u + q] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[q], B[u], acc[4 * u + q], 0, 0, 0);
+      }
+#endif
+  };
+
+  // Batch b + 1's records land in LDS and batch b + 2's touch pairs load
+  // while batch b is processed.  Past the last batch the loads are clamped
+  // repeats (never read).
+  {
+    int2 tcur = pair_of(0);
+    int2 tnext = pair_of(1);
+    issue(0, tcur, rb0);
+    auto half = [&](int b, const float (*rd)[64][4], float (*wr)[64][4]) {
+      // batch b's records and batch b + 1's pairs, issued one batch ago
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+      issue(b + 1 < nbatches ? b + 1 : nbatches - 1, tnext, wr);
+      const int2 tnn = pair_of(b + 2);
+      const int left = count - WG_BT * b;
+#pragma unroll 1
+      for (int gg = 0; gg < WG_BG; ++gg) {
+        if (4 * gg >= left) break;
+        const unsigned gm = (unsigned)__builtin_amdgcn_readlane(tcur.y, 4 * gg);
+        process(rd, gg, gm, 4 * gg + k4 < left);
+      }
+      tcur = tnext;
+      tnext = tnn;
+      // the next issue overwrites this buffer: its reads are done first
+      // (LDS operations of a wave complete in order)
+      __builtin_amdgcn_wave_barrier();
+    };
+    for (int b = 0; b < nbatches; b += 2) {
+      half(b, rb0, rb1);
+      half(b + 1, rb1, rb0);
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): no load left in flight
+  }
+
+  // write-out, one x plane at a time.  Lane (k4, n16) holds, for block
+  // (u, p = 2 py + pz), the cells (y, z) = (4 py + k4, 4 pz + r), r = 0..3,
+  // of plane n16.
+  const bool shared = item.w != 0;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int gx = origin[0] + u;
+    if (gx >= W.n[0]) break;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int py = p >> 1, pz = p & 1;
+      *reinterpret_cast<float4*>(stg + n16 * WG_PS + (4 * py + k4) * 8 + 4 * pz) =
+          make_float4(acc[4 * u + p][0], acc[4 * u + p][1], acc[4 * u + p][2], acc[4 * u + p][3]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // The plane's output, 1024 floats, in runs of one (channel, y row): 8
+    // cells along z, complex (re, im) interleaved (16 floats) or real (8).
+    // Consecutive lanes take consecutive floats of a run, so one store or
+    // atomic instruction covers whole 64-byte runs.
+    constexpr int RL = CPLX ? 16 : 8;  // floats per run
+    const int nz = W.n[2] - origin[2] < 8 ? W.n[2] - origin[2] : 8;
+    auto run_out = [&](int run) -> float* {  // output of float 0 of a run, or null
+      const int c = run >> 3, y = run & 7;
+      const int gy = origin[1] + y;
+      if (gy >= W.n[1] || c0 + c >= W.nbatch) return nullptr;
+      const int64_t cell0 = ((int64_t)gx * W.n[1] + gy) * W.n[2] + origin[2];
+      return out + ((c0 + c) * W.ncells + cell0) * (CPLX ? 2 : 1);
+    };
+    auto stage_at = [&](int run, int w) -> float {  // float w of a run
+      const int c = run >> 3, y = run & 7;
+      return CPLX ? stg[(2 * c + (w & 1)) * WG_PS + 8 * y + (w >> 1)]
+                  : stg[c * WG_PS + 8 * y + w];
+    };
+    if (shared) {
+#pragma unroll 4
+      for (int i = 0; i < 16; ++i) {
+        const int f = 64 * i + lane, run = f / RL, w = f % RL;
+        const float v = stage_at(run, w);
+        float* o = run_out(run);
+        if (o && (CPLX ? (w >> 1) : w) < nz) unsafeAtomicAdd(o + w, v);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int f = 4 * (64 * i + lane), run = f / RL, w = f % RL;
+        const float4 v = make_float4(stage_at(run, w), stage_at(run, w + 1), stage_at(run, w + 2),
+                                     stage_at(run, w + 3));
+        float* o = run_out(run);
+        if (!o) continue;
+        if (nz == 8) {
+          *reinterpret_cast<float4*>(o + w) = v;
+        } else {
+          const float vv[4] = {v.x, v.y, v.z, v.w};
+          for (int e = 0; e < 4; ++e)
+            if ((CPLX ? ((w + e) >> 1) : w + e) < nz) o[w + e] = vv[e];
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+}  // namespace
+
+bool grid_wave_ok(const tnl_plan_layout& L) {
+  if (L.ndim != 3 || L.grid_accum != TNL_GRID_ACCUM_MFMA || L.off_tpm == 0) return false;
+  for (int d = 0; d < 3; ++d) {
+    if (L.g_tile[d] != 8 || L.ntaps[d] > 5) return false;
+    if (L.grid_size[d] < 16 || L.grid_size[d] > 1024) return false;
+  }
+  return true;
+}
+
+int64_t grid_wave_wrec_bytes(const tnl_plan_layout& L) {
+  return L.npts * WG_REC * (int64_t)sizeof(float);
+}
+
+int grid_wave_launch(const float* vals, const float* locs, float* out, int64_t nbatch,
+                     const InterpGeom& G, const tnl_plan_layout& L, const char* ws,
+                     const float* rec, float* wrec, int is_complex, hipStream_t s) {
+  WaveGeom W;
+  for (int d = 0; d < 3; ++d) {
+    W.n[d] = (int)G.n[d];
+    W.ntiles[d] = L.g_ntiles[d];
+  }
+  W.ncells = G.ncells;
+  W.npts = L.npts;
+  W.nbatch = nbatch;
+  const int64_t ch = is_complex ? WG_NPL / 2 : WG_NPL;
+  W.nch = (int)((nbatch + ch - 1) / ch);
+  W.circ = G.pad == TNL_PAD_CIRCULAR;
+  {
+    const int64_t nb = (L.npts + 255) / 256;
+    window_rec_kernel<<<(unsigned)(nb < 16384 ? nb : 16384), 256, 0, s>>>(
+        locs, L.npts, G, reinterpret_cast<float4*>(wrec));
+    TNL_LAUNCH_CHECK("window_rec");
+  }
+  const int* nitems = (const int*)(ws + L.off_itemoff) + L.g_ntiles_total;
+  const int4* items = (const int4*)(ws + L.off_items);
+  const int2* tpm = (const int2*)(ws + L.off_tpm);
+  const int64_t span = WG_XCD * WG_RUN;
+  const int64_t nwg = (L.item_capacity * W.nch + span - 1) / span * span;
+  if (nwg > 0x7fffffff) return fail(TNL_EINVAL, "too many gridding work items");
+  if (!rec) return fail(TNL_EINVAL, "wave gridding needs the point-major records");
+  if (is_complex)
+    grid_wave_kernel<true><<<(unsigned)nwg, 64, 0, s>>>(out, W, tpm, wrec, rec, items, nitems);
+  else
+    grid_wave_kernel<false><<<(unsigned)nwg, 64, 0, s>>>(out, W, tpm, wrec, rec, items, nitems);
+  TNL_LAUNCH_CHECK("grid_wave");
+  return TNL_OK;
+}
+
+}  // namespace tnl

@@ -26,6 +26,15 @@ int interp_binned(bool adjoint, const float* vals, const float* locs, float* out
                   hipStream_t s);
 int64_t grid_scratch_bytes(const tnl_plan_layout& L, int64_t nbatch, int is_complex);
 
+// One-wave matrix-core gridding (tnl_gridwave.hip): plans with touch pairs.
+bool grid_wave_ok(const tnl_plan_layout& L);
+int64_t grid_wave_wrec_bytes(const tnl_plan_layout& L);
+// rec: the point-major channel records (filled by the caller), wrec:
+// grid_wave_wrec_bytes of scratch for the window records (filled here).
+int grid_wave_launch(const float* vals, const float* locs, float* out, int64_t nbatch,
+                     const InterpGeom& G, const tnl_plan_layout& L, const char* ws,
+                     const float* rec, float* wrec, int is_complex, hipStream_t s);
+
 int interp_grad_locs(const float* grid_vals, const float* pt_vals,
                      const float* locs, float* grad, int64_t nbatch,
                      int64_t npts, const InterpGeom& G, int is_complex,

@@ -146,6 +146,12 @@ int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
   L->off_itemoff = take(4 * (L->g_ntiles_total + 1));
   L->off_items = take(16 * L->item_capacity);
   L->off_items_tmp = take(16 * L->item_capacity);
+  // touch pairs of the one-wave matrix-core gridding kernel (4 spare pairs:
+  // a wave reads its last touches 4 at a time)
+  bool wave = ndim == 3 && L->grid_accum == TNL_GRID_ACCUM_MFMA;
+  for (int d = 0; wave && d < 3; ++d)
+    wave = L->g_tile[d] == 8 && L->ntaps[d] <= 5 && n[d] >= 16 && n[d] <= 1024;
+  L->off_tpm = wave ? take(8 * (L->touch_capacity + 4)) : 0;
   L->bytes = off;
   return TNL_OK;
 }
@@ -566,19 +572,29 @@ __global__ __launch_bounds__(256) void sub_bank_order_kernel(const int4* __restr
 // plans with g_cap <= 2048 take the 24 KB form, twice as many resident
 // workgroups per CU as the 48 KB one.  Keys are unique within an item, so
 // both forms give the same order.
+// Union of the block masks (key bits 32..63) of the group of 4 touches
+// holding touch i (groups start at the item's first touch).
+__device__ __forceinline__ unsigned group_mask(const unsigned long long* key, int i, int count) {
+  const int i0 = i & ~3;
+  unsigned m = 0;
+  for (int j = i0; j < i0 + 4 && j < count; ++j) m |= (unsigned)(key[j] >> 32);
+  return m;
+}
+
 template <int CAP>
 __global__ __launch_bounds__(256) void item_sort_kernel(const int4* __restrict__ items,
                                                         const int* __restrict__ nitems_ptr,
                                                         const float* __restrict__ slocs,
                                                         const int* __restrict__ perm,
-                                                        PlanDev P, int* __restrict__ tlist) {
+                                                        PlanDev P, int* __restrict__ tlist,
+                                                        int2* __restrict__ tpm) {
   __shared__ unsigned long long key[CAP];
   __shared__ int val[CAP];
   const int nitems = *nitems_ptr;
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const int4 item = items[it];
     const int start = item.y, count = item.z;
-    if (count < 2 || count > CAP) continue;
+    if (count < 1) continue;
     int origin[3];
     {
       int r = item.x;
@@ -587,6 +603,43 @@ __global__ __launch_bounds__(256) void item_sort_kernel(const int4* __restrict__
         r /= P.gntiles[d];
       }
     }
+    // (block mask << 32 | caller index) of a touch
+    auto key_of = [&](int q) {
+      unsigned mask;
+      if (P.ndim == 3) {
+        unsigned m[3];
+        for (int d = 0; d < 3; ++d)
+          m[d] = tile_cells(slocs[(int64_t)q * 3 + d], P.half[d], P.ntap[d], P.n[d], P.pad,
+                            origin[d], 8);
+        mask = tile_blocks32(m[0], m[1], m[2]);
+      } else {
+        const unsigned my = tile_cells(slocs[(int64_t)q * 2], P.half[1], P.ntap[1], P.n[1],
+                                       P.pad, origin[1], 16);
+        const unsigned mz = tile_cells(slocs[(int64_t)q * 2 + 1], P.half[2], P.ntap[2],
+                                       P.n[2], P.pad, origin[2], 32);
+        mask = tile_blocks32_2d(my, mz);
+      }
+      return ((unsigned long long)mask << 32) | (unsigned)perm[q];
+    };
+    if (count > CAP) {  // scatter order kept; touch pairs still written
+      if (tpm)
+        for (int g0 = 0; g0 < count; g0 += 4 * (int)blockDim.x) {
+          // one thread per group of 4 touches
+          const int i0 = g0 + 4 * (int)threadIdx.x;
+          if (i0 < count) {
+            unsigned long long k[4];
+            unsigned m = 0;
+            const int n = count - i0 < 4 ? count - i0 : 4;
+            for (int j = 0; j < n; ++j) {
+              k[j] = key_of(tlist[start + i0 + j]);
+              m |= (unsigned)(k[j] >> 32);
+            }
+            for (int j = 0; j < n; ++j)
+              tpm[start + i0 + j] = make_int2((int)(unsigned)k[j], (int)m);
+          }
+        }
+      continue;
+    }
     int n2 = 2;
     while (n2 < count) n2 <<= 1;
     for (int i = threadIdx.x; i < n2; i += blockDim.x) {
@@ -594,21 +647,7 @@ __global__ __launch_bounds__(256) void item_sort_kernel(const int4* __restrict__
       int q = 0;
       if (i < count) {
         q = tlist[start + i];
-        unsigned mask;
-        if (P.ndim == 3) {
-          unsigned m[3];
-          for (int d = 0; d < 3; ++d)
-            m[d] = tile_cells(slocs[(int64_t)q * 3 + d], P.half[d], P.ntap[d], P.n[d], P.pad,
-                              origin[d], 8);
-          mask = tile_blocks32(m[0], m[1], m[2]);
-        } else {
-          const unsigned my = tile_cells(slocs[(int64_t)q * 2], P.half[1], P.ntap[1], P.n[1],
-                                         P.pad, origin[1], 16);
-          const unsigned mz = tile_cells(slocs[(int64_t)q * 2 + 1], P.half[2], P.ntap[2],
-                                         P.n[2], P.pad, origin[2], 32);
-          mask = tile_blocks32_2d(my, mz);
-        }
-        k = ((unsigned long long)mask << 32) | (unsigned)perm[q];
+        k = key_of(q);
       }
       key[i] = k;
       val[i] = q;
@@ -630,7 +669,10 @@ __global__ __launch_bounds__(256) void item_sort_kernel(const int4* __restrict__
         __syncthreads();
       }
     }
-    for (int i = threadIdx.x; i < count; i += blockDim.x) tlist[start + i] = val[i];
+    for (int i = threadIdx.x; i < count; i += blockDim.x) {
+      tlist[start + i] = val[i];
+      if (tpm) tpm[start + i] = make_int2((int)(unsigned)key[i], (int)group_mask(key, i, count));
+    }
     __syncthreads();
   }
 }
@@ -709,6 +751,7 @@ extern "C" int tnl_plan_build(const float* locs, const tnl_plan_layout* layout,
   int* tlist = (int*)(ws + L.off_tlist);
   int* itemoff = (int*)(ws + L.off_itemoff);
   int4* items = (int4*)(ws + L.off_items);
+  int2* tpm = L.off_tpm ? (int2*)(ws + L.off_tpm) : nullptr;
   const PlanDev P = make_plan_dev(L);
   const int64_t nb1 = L.u_nbins + 1;
 
@@ -763,10 +806,10 @@ extern "C" int tnl_plan_build(const float* locs, const tnl_plan_layout* layout,
     const int64_t nwg = L.item_capacity < 8192 ? L.item_capacity : 8192;
     if (L.g_cap <= 2048)
       item_sort_kernel<2048><<<(unsigned)nwg, 256, 0, s>>>(items, itemoff + L.g_ntiles_total,
-                                                          slocs, perm, P, tlist);
+                                                          slocs, perm, P, tlist, tpm);
     else
       item_sort_kernel<4096><<<(unsigned)nwg, 256, 0, s>>>(items, itemoff + L.g_ntiles_total,
-                                                          slocs, perm, P, tlist);
+                                                          slocs, perm, P, tlist, tpm);
     TNL_LAUNCH_CHECK("plan item sort");
   }
   if (npts > 0 && L.ndim == 3) {

@@ -102,6 +102,7 @@ class PlanLayout(ctypes.Structure):
         ("off_itemoff", ctypes.c_int64),
         ("off_items", ctypes.c_int64),
         ("off_items_tmp", ctypes.c_int64),
+        ("off_tpm", ctypes.c_int64),
         ("bytes", ctypes.c_int64),
     ]
 
