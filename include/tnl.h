@@ -112,6 +112,11 @@ typedef struct tnl_plan_layout {
                           * tile's MFMA blocks that its group of 4 touches
                           * (counted from the item's first) covers, as int32
                           * pairs; 0 when absent                            */
+  int64_t off_wrec;      /* plans with touch pairs: one 64-byte window record
+                          * per point (tap weights per axis, first taps),
+                          * filled by tnl_plan_windows; 0 when absent        */
+  int32_t wrec_kernel;   /* kernel and beta the window records were last    */
+  float wrec_beta;       /* filled for; wrec_kernel = -1 until then          */
   int64_t bytes;        /* total workspace bytes                             */
 } tnl_plan_layout;
 
@@ -124,6 +129,17 @@ int tnl_plan_layout_query(int64_t npts, int ndim, const int64_t* grid_size,
  * of at least layout->bytes. */
 int tnl_plan_build(const float* locs, const tnl_plan_layout* layout,
                    void* workspace, int64_t workspace_bytes, void* stream);
+
+/* Fill the window records of a plan with touch pairs (layout->off_wrec !=
+ * 0) for one kernel and beta, from the same locs the plan was built from,
+ * and record them in layout->wrec_kernel / wrec_beta.  Gridding calls with
+ * that plan, kernel and beta then read the records from the plan instead
+ * of evaluating them into their scratch on every call.  A plan without the
+ * section is left as it is.  No counterpart in the reference (its kernels
+ * evaluate the window per tap, src/torchlinops/functional/_interp/
+ * grid.py:161-513). */
+int tnl_plan_windows(tnl_plan_layout* layout, void* workspace, const float* locs,
+                     int kernel, float beta, void* stream);
 
 /* ---- interpolation (ungrid) -------------------------------------------
  * out[b, p] = sum_{g in taps(p)} w(t_p - g) * vals[b, g]

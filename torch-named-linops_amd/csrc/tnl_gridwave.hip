@@ -42,8 +42,6 @@ typedef float wv32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int WG_NPL = 16;          // planes of a channel chunk
 constexpr int WG_REC = 16;          // floats per channel / window record
-constexpr int WG_ROW = 24;          // LDS slots per (touch, axis) window row
-constexpr int WG_WIN = 4 * 3 * WG_ROW;
 constexpr int WG_PS = 68;           // LDS stage stride per plane (floats)
 constexpr int WG_RUN = 8;           // consecutive work units given to one XCD
 constexpr int WG_BG = 8;            // groups of 4 touches per prefetch batch
@@ -115,7 +113,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 3))) void
   // the buffer being read from the one being loaded)
   __shared__ __align__(16) float rb0[4][64][4];
   __shared__ __align__(16) float rb1[4][64][4];
-  __shared__ __align__(16) float lw[WG_WIN];
+  // tile-local windows of the batch's touches: [touch][axis][8 cells]
+  __shared__ __align__(16) float tw[WG_BT * 3 * 8];
   __shared__ __align__(16) float stg[WG_NPL * WG_PS];
 
   const int64_t nunits = (int64_t)(*nitems_ptr) * W.nch;
@@ -143,16 +142,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 3))) void
   const int lane = threadIdx.x;
   const int k4 = lane >> 4, n16 = lane & 15;
   const int yy = n16 >> 2, zz = n16 & 3;
-  // this lane's weight slot: axis wa, tap wk (lanes n16 < 15)
-  const int wa = n16 >= 10 ? 2 : (n16 >= 5 ? 1 : 0);
-  const int wk = n16 - 5 * wa;
-  const int w_origin = wa == 0 ? origin[0] : (wa == 1 ? origin[1] : origin[2]);
-  const int w_n = wa == 0 ? W.n[0] : (wa == 1 ? W.n[1] : W.n[2]);
-  float* const wrow = lw + (k4 * 3 + wa) * WG_ROW;
-
-  // clear the window table once (slots outside 4..19 are never written)
-  for (int i = lane; i < WG_WIN / 4; i += 64)
-    reinterpret_cast<float4*>(lw)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 
   wv32x4 acc[32];
 #pragma unroll
@@ -180,86 +169,82 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 3))) void
     for (int i = 0; i < 4; ++i)
       __builtin_amdgcn_global_load_lds(src + 4 * i, &dst[i][0][0], 16, 0, 0);
   };
-  // Group gg of the batch in buffer rb; gm = the group's block mask.
-  auto process = [&](const float (*rb)[64][4], int gg, unsigned gm, bool live) {
-    const int t = 4 * gg + k4;
-    const float cw = rb[n16 >> 2][t][n16 & 3];
-    const unsigned cbase = __float_as_uint(rb[3][t][3]);
-    const float cv = live ? rb[n16 >> 2][WG_BT + t][n16 & 3] : 0.f;
-    // stage the group's windows: clear slots 4..19 of the 12 rows, then
-    // place each weight at its tile-local slot
-    if (lane < 48)
-      *reinterpret_cast<float4*>(lw + (lane >> 2) * WG_ROW + 4 + 4 * (lane & 3)) =
-          make_float4(0.f, 0.f, 0.f, 0.f);
-    if (n16 < 15 && live) {
-      int d = (int)((cbase >> (10 * wa)) & 1023u) - w_origin;
-      if (W.circ && d > 7) d -= w_n;
-      int slot = 8 + d + wk;
-      slot = slot < 0 ? 0 : (slot > WG_ROW - 1 ? WG_ROW - 1 : slot);
-      wrow[slot] = cw;
+  // Place the tile-local windows of the batch in rb (touches 0..left-1)
+  // into tw: clear it, then each (touch, axis) task writes the weights of
+  // its kept taps that fall inside the tile.
+  // per lane and pass j: the task's axis, tile origin and grid size along it
+  // (masks, not selects: selects over an array would become a stack load)
+  int st_a[2], st_org[2], st_n[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int a = (64 * j + lane) % 3;
+    const int m0 = -(a == 0), m1 = -(a == 1), m2 = -(a == 2);
+    st_a[j] = a;
+    st_org[j] = (origin[0] & m0) | (origin[1] & m1) | (origin[2] & m2);
+    st_n[j] = (W.n[0] & m0) | (W.n[1] & m1) | (W.n[2] & m2);
+  }
+  auto stage_windows = [&](const float (*rb)[64][4], int left) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      reinterpret_cast<float4*>(tw)[64 * i + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int task = 64 * j + lane, t = task / 3, a = st_a[j];
+      if (task < 3 * WG_BT && t < left) {
+        const unsigned base = __float_as_uint(rb[3][t][3]);
+        int d = (int)((base >> (10 * a)) & 1023u) - st_org[j];
+        const int n = st_n[j];
+        if (W.circ && d > 7) d -= n;
+        float* row = tw + (t * 3 + a) * 8;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          const int c = d + k, wd = 5 * a + k;
+          if (c >= 0 && c < 8) row[c] = rb[wd >> 2][t][wd & 3];
+        }
+      }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const float* rx = lw + (k4 * 3) * WG_ROW + 8;
-    const float4 x0 = *reinterpret_cast<const float4*>(rx);
-    const float4 x1 = *reinterpret_cast<const float4*>(rx + 4);
-    const float* ry = lw + (k4 * 3 + 1) * WG_ROW + 8 + yy;
-    const float* rz = lw + (k4 * 3 + 2) * WG_ROW + 8 + zz;
-    const float wy0 = ry[0], wy1 = ry[4], wz0 = rz[0], wz1 = rz[4];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const float A[4] = {wy0 * wz0, wy0 * wz1, wy1 * wz0, wy1 * wz1};
-    const float wx[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
-#ifndef WG_MODE
-#define WG_MODE 1
-#endif
-#if WG_MODE == 0
+  };
+  // Operands of group gg of the staged batch: this lane's touch 4 gg + k4.
+  struct Ops {
+    float4 x0, x1;           // x window at the tile's 8 planes
+    float wy0, wy1, wz0, wz1;  // y window at rows yy, yy + 4; z at zz, zz + 4
+    float v;                 // channel value of plane n16
+  };
+  auto ops_of = [&](const float (*rb)[64][4], int gg, bool live) {
+    const int t = 4 * gg + k4;
+    const float* r = tw + t * 24;
+    Ops o;
+    o.x0 = *reinterpret_cast<const float4*>(r);
+    o.x1 = *reinterpret_cast<const float4*>(r + 4);
+    o.wy0 = r[8 + yy];
+    o.wy1 = r[12 + yy];
+    o.wz0 = r[16 + zz];
+    o.wz1 = r[20 + zz];
+    o.v = live ? rb[n16 >> 2][WG_BT + t][n16 & 3] : 0.f;
+    return o;
+  };
+  // The MFMAs of two groups (8 touches) under one decision per block: A (4
+  // patches) from the y/z windows and B (one per live plane) from the x
+  // window and the value, for each group; gm is the union of both groups'
+  // block masks (a group absent from a block adds zeros there).
+  auto mfmas = [&](const Ops& o, const Ops& p, unsigned gm) {
+    const float Ao[4] = {o.wy0 * o.wz0, o.wy0 * o.wz1, o.wy1 * o.wz0, o.wy1 * o.wz1};
+    const float Ap[4] = {p.wy0 * p.wz0, p.wy0 * p.wz1, p.wy1 * p.wz0, p.wy1 * p.wz1};
+    const float xo[8] = {o.x0.x, o.x0.y, o.x0.z, o.x0.w, o.x1.x, o.x1.y, o.x1.z, o.x1.w};
+    const float xp[8] = {p.x0.x, p.x0.y, p.x0.z, p.x0.w, p.x1.x, p.x1.y, p.x1.z, p.x1.w};
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const unsigned pm = (gm >> (4 * u)) & 15u;
       if (pm) {
-        const float b = wx[u] * cv;
+        const float bo = xo[u] * o.v, bp = xp[u] * p.v;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          if ((pm >> q) & 1u)
-            acc[4 * u + q] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[q], b, acc[4 * u + q], 0, 0, 0);
+          if ((pm >> q) & 1u) {
+            acc[4 * u + q] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ao[q], bo, acc[4 * u + q], 0, 0, 0);
+            acc[4 * u + q] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ap[q], bp, acc[4 * u + q], 0, 0, 0);
+          }
       }
     }
-#elif WG_MODE == 1
-    // (x half, patch) decisions: the 4 planes of a live half, zero weights
-    // where a plane is outside every touch's footprint
-    float B[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) B[u] = wx[u] * cv;
-    const unsigned lo = gm & 0xFFFFu, hi = gm >> 16;
-    const unsigned hq[2] = {(lo | (lo >> 4) | (lo >> 8) | (lo >> 12)) & 15u,
-                            (hi | (hi >> 4) | (hi >> 8) | (hi >> 12)) & 15u};
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if ((hq[h] >> q) & 1u) {
-#pragma unroll
-          for (int u = 4 * h; u < 4 * h + 4; ++u)
-            acc[4 * u + q] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[q], B[u], acc[4 * u + q], 0, 0, 0);
-        }
-#else
-    // patch decisions only: all 8 planes of a live patch
-    float B[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) B[u] = wx[u] * cv;
-    const unsigned all = (gm | (gm >> 4) | (gm >> 8) | (gm >> 12) | (gm >> 16) | (gm >> 20) |
-                          (gm >> 24) | (gm >> 28)) & 15u;
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if ((all >> q) & 1u) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-          acc[4 * u + q] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[q], B[u], acc[4 * u + q], 0, 0, 0);
-      }
-#endif
   };
 
   // Batch b + 1's records land in LDS and batch b + 2's touch pairs load
@@ -275,11 +260,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 3))) void
       issue(b + 1 < nbatches ? b + 1 : nbatches - 1, tnext, wr);
       const int2 tnn = pair_of(b + 2);
       const int left = count - WG_BT * b;
+      const int ng = left <= 0 ? 0 : (left >= WG_BT ? WG_BG : (left + 3) >> 2);
+      if (ng > 0) {
+        stage_windows(rd, left);
+        // groups in pairs; a missing second group has no live touch (its
+        // operands are zero) and no mask bits
 #pragma unroll 1
-      for (int gg = 0; gg < WG_BG; ++gg) {
-        if (4 * gg >= left) break;
-        const unsigned gm = (unsigned)__builtin_amdgcn_readlane(tcur.y, 4 * gg);
-        process(rd, gg, gm, 4 * gg + k4 < left);
+        for (int gg = 0; gg < ng; gg += 2) {
+          const bool two = gg + 1 < ng;
+          const Ops o = ops_of(rd, gg, 4 * gg + k4 < left);
+          const Ops p = ops_of(rd, two ? gg + 1 : gg, two && 4 * (gg + 1) + k4 < left);
+          const unsigned m0 = (unsigned)__builtin_amdgcn_readlane(tcur.y, 4 * gg);
+          const unsigned m1 = (unsigned)__builtin_amdgcn_readlane(tcur.y, 4 * gg + 4);
+          mfmas(o, p, m0 | (two ? m1 : 0u));
+        }
       }
       tcur = tnext;
       tnext = tnn;
@@ -378,6 +372,7 @@ int64_t grid_wave_wrec_bytes(const tnl_plan_layout& L) {
 int grid_wave_launch(const float* vals, const float* locs, float* out, int64_t nbatch,
                      const InterpGeom& G, const tnl_plan_layout& L, const char* ws,
                      const float* rec, float* wrec, int is_complex, hipStream_t s) {
+  // wrec: the scratch's window records, unless the plan holds them
   WaveGeom W;
   for (int d = 0; d < 3; ++d) {
     W.n[d] = (int)G.n[d];
@@ -389,7 +384,9 @@ int grid_wave_launch(const float* vals, const float* locs, float* out, int64_t n
   const int64_t ch = is_complex ? WG_NPL / 2 : WG_NPL;
   W.nch = (int)((nbatch + ch - 1) / ch);
   W.circ = G.pad == TNL_PAD_CIRCULAR;
-  {
+  if (L.off_wrec && L.wrec_kernel == G.kernel && L.wrec_beta == G.beta) {
+    wrec = (float*)(ws + L.off_wrec);  // filled by tnl_plan_windows
+  } else {
     const int64_t nb = (L.npts + 255) / 256;
     window_rec_kernel<<<(unsigned)(nb < 16384 ? nb : 16384), 256, 0, s>>>(
         locs, L.npts, G, reinterpret_cast<float4*>(wrec));
@@ -411,3 +408,42 @@ int grid_wave_launch(const float* vals, const float* locs, float* out, int64_t n
 }
 
 }  // namespace tnl
+
+using namespace tnl;
+
+extern "C" int tnl_plan_windows(tnl_plan_layout* layout, void* workspace, const float* locs,
+                                int kernel, float beta, void* stream) {
+  if (!layout) return fail(TNL_EINVAL, "null layout");
+  tnl_plan_layout& L = *layout;
+  if (!L.off_wrec) return TNL_OK;  // no window section in this plan
+  if (!workspace) return fail(TNL_EINVAL, "null workspace");
+  if (L.npts > 0 && !locs) return fail(TNL_EINVAL, "null locs");
+  if (kernel != TNL_KERNEL_KAISER_BESSEL && kernel != TNL_KERNEL_SPLINE)
+    return fail(TNL_EINVAL, "Unrecognized kernel");
+  // the geometry of the plan's axes (its plans are 3-D)
+  InterpGeom G;
+  std::memset(&G, 0, sizeof(G));
+  G.ndim = L.ndim;
+  G.kernel = kernel;
+  G.norm = 1;
+  G.pad = L.pad_mode;
+  G.beta = beta;
+  for (int d = 0; d < 3; ++d) {
+    G.n[d] = L.grid_size[d];
+    G.half[d] = L.width[d] * 0.5f;
+    G.inv_half[d] = 1.0f / (L.width[d] * 0.5f);
+    G.ntap[d] = L.ntaps[d];
+  }
+  G.ncells = G.n[0] * G.n[1] * G.n[2];
+  L.wrec_kernel = -1;  // invalid while being rewritten
+  if (L.npts > 0) {
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t nb = (L.npts + 255) / 256;
+    window_rec_kernel<<<(unsigned)(nb < 16384 ? nb : 16384), 256, 0, s>>>(
+        locs, L.npts, G, reinterpret_cast<float4*>((char*)workspace + L.off_wrec));
+    TNL_LAUNCH_CHECK("plan windows");
+  }
+  L.wrec_kernel = kernel;
+  L.wrec_beta = beta;
+  return TNL_OK;
+}

@@ -152,6 +152,9 @@ int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
   for (int d = 0; wave && d < 3; ++d)
     wave = L->g_tile[d] == 8 && L->ntaps[d] <= 5 && n[d] >= 16 && n[d] <= 1024;
   L->off_tpm = wave ? take(8 * (L->touch_capacity + 4)) : 0;
+  L->off_wrec = wave ? take(64 * std::max<int64_t>(npts, 1)) : 0;
+  L->wrec_kernel = -1;
+  L->wrec_beta = 0.0f;
   L->bytes = off;
   return TNL_OK;
 }

@@ -103,6 +103,9 @@ class PlanLayout(ctypes.Structure):
         ("off_items", ctypes.c_int64),
         ("off_items_tmp", ctypes.c_int64),
         ("off_tpm", ctypes.c_int64),
+        ("off_wrec", ctypes.c_int64),
+        ("wrec_kernel", ctypes.c_int32),
+        ("wrec_beta", ctypes.c_float),
         ("bytes", ctypes.c_int64),
     ]
 
@@ -182,6 +185,7 @@ SIGNATURES = {
         [_i64, _i32, _i64p, _f32p, _i32, _tuning_p, _layout_p],
     ),
     "tnl_plan_build": (_i32, [_vp, _layout_p, _vp, _i64, _vp]),
+    "tnl_plan_windows": (_i32, [_layout_p, _vp, _vp, _i32, _f32, _vp]),
     "tnl_ungrid": (
         _i32,
         [_vp, _vp, _vp, _i64, _i64, _i32, _i64p, _f32p, _i32, _i32, _i32,
