@@ -722,11 +722,13 @@ def main() -> int:
         # recorded in profiles/ (rocprofv3 cannot run inside this process),
         # summed over every kernel of the dominant call: for gridding the
         # point-record transpose, the split-tile zero pass and the
-        # matrix-core kernel
+        # matrix-core kernel (its window records live in the plan)
         tb = json.loads(tfile.read_text())["traffic_bytes"]
-        keys = (("grid_mfma_kernel", "pack_points_kernel", "grid_zero_split_kernel")
-                if dom == "grid" else ("ungrid_binned_kernel<3, true, 2",))
-        hits = {k: float(v) for key in keys for k, v in tb.items() if k.startswith("tnl::" + key)}
+        keys = (("grid_wave_kernel", "grid_mfma_kernel", "pack_points_kernel",
+                 "grid_zero_split_kernel")
+                if dom == "grid" else ("ungrid_wave_kernel", "ungrid_binned_kernel<3, true, 2"))
+        hits = {k: float(v) for key in keys for k, v in tb.items()
+                if k.startswith(("tnl::" + key, "tnl::(anonymous namespace)::" + key))}
         if hits:
             traffic, traffic_kernels = sum(hits.values()), sorted(hits)
     out = {

@@ -167,6 +167,14 @@ def _launch(adjoint: bool, vals_flat, locs, locs2d, out, nbatch, npts, ndim,
         None if plan is None else plan.ptr,
     ]
     with torch.cuda.device(vals_flat.device):
+        if plan is not None:
+            L = plan.layout
+            if L.off_wrec and (L.wrec_kernel != KERNELS[kernel]
+                               or L.wrec_beta != ctypes.c_float(beta).value):
+                # window records kept with the plan: evaluated once per
+                # (plan, kernel, beta), not on every call
+                check(lib().tnl_plan_windows(L, plan.ptr, locs2d.data_ptr(), KERNELS[kernel],
+                                             beta, stream_of(vals_flat)))
         if not adjoint:
             check(lib().tnl_ungrid(*args, stream_of(vals_flat)))
             return
@@ -174,13 +182,6 @@ def _launch(adjoint: bool, vals_flat, locs, locs2d, out, nbatch, npts, ndim,
         if plan is not None:
             check(lib().tnl_grid_scratch_bytes(plan.layout, nbatch, int(is_complex),
                                                ctypes.byref(nbytes)))
-            L = plan.layout
-            if L.off_wrec and (L.wrec_kernel != KERNELS[kernel]
-                               or L.wrec_beta != ctypes.c_float(beta).value):
-                # window records kept with the plan: evaluated once per
-                # (plan, kernel, beta), not on every gridding call
-                check(lib().tnl_plan_windows(L, plan.ptr, locs2d.data_ptr(), KERNELS[kernel],
-                                             beta, stream_of(vals_flat)))
         if nbytes.value > 0:
             # point-major records for the matrix-core gridding kernel; from
             # torch's caching allocator, so repeated calls do not hipMalloc
