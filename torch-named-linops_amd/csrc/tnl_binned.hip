@@ -10,9 +10,11 @@
 //   for the later chunks.
 // grid: "owner computes".  One workgroup per (work item, channel chunk); a
 //   work item is a gridding tile with the list of points touching it.  Taps
-//   inside the tile are accumulated on the matrix cores (grid_mfma_kernel)
-//   or by plain LDS read-modify-writes in which no two lanes or waves share
-//   a word (grid_binned_kernel), then the tile is written to HBM once with
+//   inside the tile are accumulated on the matrix cores (grid_mfma_kernel;
+//   3-D plans with touch pairs hand over to the one-wave kernel of
+//   tnl_gridwave.hip when the caller passes a scratch) or by plain LDS
+//   read-modify-writes in which no two lanes or waves share a word
+//   (grid_binned_kernel), then the tile is written to HBM once with
 //   coalesced stores.  No zero-fill pass and no global atomics, except for
 //   tiles with more than g_cap touches, which are split across workgroups
 //   and combined with global float atomics into a region zeroed by a small
