@@ -77,9 +77,8 @@ constexpr int UPTS = 2;  // cached points per thread (max_sub <= 512)
 // taps per axis (a 12^3 window): the window is staged cell-major with the
 // two channels of a cell side by side (16 bytes), so each tap of a point
 // is one 16-byte LDS read for both channels instead of two 8-byte reads of
-// channel planes (gfx950 serves 16-byte LDS reads at about twice the bytes
-// per clock: the gather of the 3-D radial workload, 8 coils, 256^3, ran
-// 1.74 -> 1.31 ms without its staging).
+// channel planes (measured on the 3-D radial workload, 8 coils, 256^3,
+// 20000 x 512 points: the gather without its staging ran 1.74 -> 1.31 ms).
 constexpr int ILV_W = 12;
 
 template <int NDIM, bool CPLX, int CC, int NT, bool ILV = false>
@@ -125,14 +124,16 @@ __device__ __forceinline__ void ungrid_sub(
     if (c0 > 0) __syncthreads();  // previous chunk's gathers are done
     if constexpr (ILV) {
       // ---- stage window, interleaved: window cell e = (wx * 12 + wy) * 12
-      // + wz holds channels c0 and c0 + 1 as one float4.  One cell per
-      // thread and pass (the loads of consecutive threads run along z); a
-      // thread issues all its loads before its stores.  The caller
-      // guarantees n >= 12 per axis, so one subtraction wraps a window
-      // cell under circular padding; under zero padding cells past the
+      // + wz holds channels c0 and c0 + 1 as one float4.  One pair of
+      // z-neighbour cells per thread and pass: a 16-byte load per channel
+      // (the loads of consecutive threads run along z) and two 16-byte
+      // stores; a thread issues all its loads before its stores.  The
+      // caller guarantees n >= 12 per axis and an even n[2]: one
+      // subtraction wraps a window cell under circular padding and both
+      // cells of a pair wrap together; under zero padding cells past the
       // grid are zero (tile origins are never negative).
-      constexpr int WC = ILV_W * ILV_W * ILV_W;
-      constexpr int NPASS = (WC + BIN_THREADS - 1) / BIN_THREADS;
+      constexpr int NPAIR = ILV_W * ILV_W * ILV_W / 2;
+      constexpr int NPASS = (NPAIR + BIN_THREADS - 1) / BIN_THREADS;
       const int n0 = (int)G.n[0], n1 = (int)G.n[1], n2 = (int)G.n[2];
       const bool circ = G.pad == TNL_PAD_CIRCULAR;
       const T* __restrict__ v0 = v + c0 * G.ncells;
@@ -141,20 +142,15 @@ __device__ __forceinline__ void ungrid_sub(
       // coordinates of all passes would stay live through the gather
       int tid;
       asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"((int)threadIdx.x));
-      // passes in two halves (fewer registers held across the loads)
+      float4 a[NPASS], b[NPASS];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-      constexpr int HP = (NPASS + 1) / 2;
-      T a[HP], b[HP];
-#pragma unroll
-      for (int j = 0; j < HP; ++j) {
-        const int i = h * HP + j;
+      for (int i = 0; i < NPASS; ++i) {
         const int e = tid + i * BIN_THREADS;
-        a[j] = C::zero();
-        b[j] = C::zero();
-        if (i < NPASS && e < WC) {
-          const int wx = e / (ILV_W * ILV_W), e2 = e - wx * (ILV_W * ILV_W);
-          const int wy = e2 / ILV_W, wz = e2 - wy * ILV_W;
+        a[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        b[i] = a[i];
+        if (e < NPAIR) {
+          const int row = e / (ILV_W / 2), wz = 2 * (e - row * (ILV_W / 2));
+          const int wx = row / ILV_W, wy = row - wx * ILV_W;
           int gx = origin[0] + wx, gy = origin[1] + wy, gz = origin[2] + wz;
           bool in = true;
           if (circ) {
@@ -166,18 +162,19 @@ __device__ __forceinline__ void ungrid_sub(
           }
           if (in) {
             const int64_t cell = ((int64_t)gx * n1 + gy) * n2 + gz;
-            a[j] = v0[cell];
-            if (nc > 1) b[j] = v1[cell];
+            a[i] = *reinterpret_cast<const float4*>(v0 + cell);
+            if (nc > 1) b[i] = *reinterpret_cast<const float4*>(v1 + cell);
           }
         }
       }
 #pragma unroll
-      for (int j = 0; j < HP; ++j) {
-        const int i = h * HP + j;
-        const int e = threadIdx.x + i * BIN_THREADS;
-        if (i < NPASS && e < WC)
-          reinterpret_cast<float4*>(lds)[e] = make_float4(a[j].x, a[j].y, b[j].x, b[j].y);
-      }
+      for (int i = 0; i < NPASS; ++i) {
+        const int e = tid + i * BIN_THREADS;
+        if (e < NPAIR) {
+          float4* d = reinterpret_cast<float4*>(lds) + 2 * e;
+          d[0] = make_float4(a[i].x, a[i].y, b[i].x, b[i].y);
+          d[1] = make_float4(a[i].z, a[i].w, b[i].z, b[i].w);
+        }
       }
     } else {
     // ---- stage window: channel planes [c][cell] in LDS (lanes gathering
@@ -325,9 +322,6 @@ __device__ __forceinline__ void ungrid_sub(
               for (int c = 0; c < CC; ++c) C::fma(acc[c], wt, tv[ky][kz][c]);
             }
           }
-          // one x row of reads in flight at a time (the interleaved form's
-          // constant offsets would otherwise let all 64 reads hoist)
-          if constexpr (ILV) __builtin_amdgcn_sched_barrier(0);
         }
       } else {
 #pragma unroll 1
@@ -1626,6 +1620,8 @@ static int launch_binned(bool adjoint, const float* vals, float* out,
     if constexpr (NDIM == 3 && CPLX && CC == 2 && NT == 5) {
       bool ilv = true;
       for (int d = 0; d < 3; ++d) ilv = ilv && B.win[d] == ILV_W && G.n[d] >= ILV_W;
+      // z pairs of cells: even n[2], 16-byte aligned channel planes
+      ilv = ilv && G.n[2] % 2 == 0 && (reinterpret_cast<uintptr_t>(vals) & 15) == 0;
       if (ilv) kern = ungrid_binned_kernel<3, true, 2, 5, true>;
     }
     if (int rc = allow_lds((const void*)kern, lds)) return rc;
