@@ -141,17 +141,28 @@ class DeviceCG:
 
     def solve(self, y: torch.Tensor, x0: Optional[torch.Tensor] = None, tol: float = 1e-6,
               max_iters: int = 200) -> torch.Tensor:
+        self.start(y, x0)
+        return self.resume(tol, max_iters)
+
+    def start(self, y: torch.Tensor, x0: Optional[torch.Tensor] = None) -> None:
+        """Set up the iteration for A x = y from x0 (zero when None)."""
         self._init(y, x0)
         self.history = [self.rel_residual()]
         self.iters = 0
-        graph = None
+        self._graph, self._graph_n = None, 0
+
+    def resume(self, tol: float = 1e-6, max_iters: int = 200) -> torch.Tensor:
+        """Continue the iteration from its current state (the same Krylov
+        sequence as one uninterrupted run) until the relative residual is
+        below ``tol`` or ``max_iters`` iterations in total have run."""
         while self.iters < max_iters and self.history[-1] > tol:
             n = min(self.check_every, max_iters - self.iters)
-            if self.use_graph and y.is_cuda and n == self.check_every:
-                if graph is None:
-                    graph = self._capture(n)  # runs this block eagerly, then records
+            if self.use_graph and self.y.is_cuda and n == self.check_every:
+                if self._graph is None or self._graph_n != n:
+                    # runs this block eagerly, then records it
+                    self._graph, self._graph_n = self._capture(n), n
                 else:
-                    graph.replay()
+                    self._graph.replay()
             else:
                 for _ in range(n):
                     self._iteration()

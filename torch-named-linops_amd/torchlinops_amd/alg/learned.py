@@ -24,12 +24,14 @@ iterations from zero, differentiated through the unrolled iterations (the
 interpolation kernels' autograd pair supplies the operator's backward).
 
 ``solve`` runs preconditioned CG (``DeviceCG``) and keeps plain CG as the
-judge: with ``judge_iters`` it first runs plain CG and the preconditioned
-CG for that many iterations and keeps the preconditioner only if its
-iterate is not behind plain CG's at that matched iteration count, scored by
-CG's own objective (the A-norm error, from one A x each; the recomputed
-residuals are reported beside it) - the loser's work is counted, the
-winner continues from its iterate; and if the preconditioned run stalls,
+judge: with ``judge_iters`` it first runs the preconditioned CG for that
+many iterations and plain CG for the same time (the same count unless
+applying the preconditioner costs time of its own) and keeps the
+preconditioner only if its iterate is not behind plain CG's at that
+matched cost, scored by CG's own objective (the A-norm error, from one
+A x each; the recomputed residuals are reported beside it) - the loser's
+work is counted, the winner continues its own Krylov sequence without a
+restart; and if the preconditioned run stalls,
 breaks down or ends above the tolerance on its recomputed residual, plain
 CG continues from its iterate.  No counterpart in the
 reference, whose CG (pcg.py) is unpreconditioned.
@@ -374,16 +376,22 @@ def train_preconditioner(net: PrecondNet, data, steps: int = 300, lr: float = 3e
 
 
 def judge(A, y: torch.Tensor, M, iters: int = 10, check_every: int = 5):
-    """Plain CG and CG preconditioned by ``M`` (r -> M r), ``iters``
-    iterations each from zero.  Both are scored by the quantity CG
+    """Plain CG and CG preconditioned by ``M`` (r -> M r) from zero,
+    compared at matched cost: ``iters`` preconditioned iterations against
+    as many plain iterations as run in the same time (``iters``, or more
+    when applying ``M`` makes a preconditioned iteration dearer; wall time
+    of each run, synchronised).  Both are scored by the quantity CG
     minimises, the energy f(x) = x^H A x / 2 - Re(y^H x) = ||x - x*||_A^2 / 2
     - const (so a lower f is a smaller A-norm error, without knowing x*),
     computed from the same A x as the recomputed residual |y - A x| / |y|
     (reported too; CG's residual norm is not monotone on ill-conditioned
-    systems, its energy is).  Returns (accept, x_plain, x_pre, info):
-    ``accept`` is True when the preconditioned iterate is finite and its
-    energy not above plain CG's."""
+    systems, its energy is).  Returns (accept, cg_plain, cg_pre, info):
+    the two ``DeviceCG`` runs, stopped where they were compared, so the
+    winner continues its own Krylov sequence (no restart); ``accept`` is
+    True when the preconditioned iterate is finite and its energy not above
+    plain CG's."""
     ynorm = torch.linalg.vector_norm(y)
+    sync = y.is_cuda
 
     def score(x):
         ax = A(x)
@@ -391,16 +399,34 @@ def judge(A, y: torch.Tensor, M, iters: int = 10, check_every: int = 5):
             torch.vdot(y.flatten(), x.flatten()).real)
         return f, float(torch.linalg.vector_norm(y - ax) / ynorm)
 
-    cg_p = DeviceCG(A, lam=0.0, M=None, check_every=check_every)
-    xp = cg_p.solve(y, tol=0.0, max_iters=iters)
+    def timed(cg, n):
+        if sync:
+            torch.cuda.synchronize()
+        t = time.perf_counter()
+        cg.resume(tol=0.0, max_iters=cg.iters + n)
+        if sync:
+            torch.cuda.synchronize()
+        return time.perf_counter() - t
+
     cg_m = DeviceCG(A, lam=0.0, M=M, check_every=check_every)
-    xm = cg_m.solve(y, tol=0.0, max_iters=iters)
-    fp, rp = score(xp)
-    fm, rm = score(xm)
+    cg_m.start(y)
+    t_m = timed(cg_m, iters)
+    cg_p = DeviceCG(A, lam=0.0, M=None, check_every=check_every)
+    cg_p.start(y)
+    t_p = timed(cg_p, iters)
+    if t_p > 0 and t_m > 1.05 * t_p:
+        # the preconditioner costs time of its own: plain CG runs on for
+        # the time it leaves (at most 10x the budget)
+        extra = min(int(round(iters * (t_m / t_p - 1.0))), 10 * iters)
+        if extra > 0:
+            t_p += timed(cg_p, extra)
+    fp, rp = score(cg_p.x)
+    fm, rm = score(cg_m.x)
     ok = math.isfinite(fm) and math.isfinite(rm) and fm <= fp
-    info = {"accepted": ok, "iters": iters, "plain_energy": fp, "precond_energy": fm,
-            "plain_residual": rp, "precond_residual": rm}
-    return ok, xp, xm, info
+    info = {"accepted": ok, "iters": iters, "plain_iters": cg_p.iters,
+            "precond_seconds": t_m, "plain_seconds": t_p, "plain_energy": fp,
+            "precond_energy": fm, "plain_residual": rp, "precond_residual": rm}
+    return ok, cg_p, cg_m, info
 
 
 def solve(A: SenseNormal, y: torch.Tensor, d: Optional[torch.Tensor] = None,
@@ -408,28 +434,34 @@ def solve(A: SenseNormal, y: torch.Tensor, d: Optional[torch.Tensor] = None,
           check_factor: float = 100.0, judge_iters: int = 0):
     """(Preconditioned) CG until its relative residual |r| / |y| is below
     ``tol``.  Plain CG checks the result: with ``judge_iters`` > 0 the
-    preconditioner must first match plain CG after that many iterations of
-    each (``judge``; else plain CG carries on from its own iterate);
+    preconditioner must first match plain CG at matched cost (``judge``;
+    else plain CG carries on from its own state);
     afterwards the residual is recomputed as |y - A x| / |y| with
     the operator itself, and if the preconditioned run did not converge,
     broke down (non-finite values) or its recomputed residual exceeds
     ``check_factor * tol``, plain CG continues from its iterate (or from
     zero).  Returns (x, info); info["iters"] counts every iteration run,
-    the judge's included."""
-    torch.cuda.synchronize()
+    the judge's included (the winner's judge iterations are the start of
+    its own run)."""
+    if y.is_cuda:
+        torch.cuda.synchronize()
     t0 = time.perf_counter()
     # d: None (plain CG), an image-space diagonal, or a callable r -> M r
     M = None if d is None else (d if callable(d) else (lambda r: d * r))
-    x0, spent, verdict = None, 0, None
+    spent, verdict = 0, None
     if M is not None and judge_iters > 0:
-        ok, xp, xm, verdict = judge(A, y, M, judge_iters, check_every=judge_iters)
-        spent = 2 * judge_iters
-        if ok:
-            x0 = xm
-        else:
-            M, x0 = None, xp
-    cg = DeviceCG(A, lam=0.0, M=M, check_every=check_every)
-    x = cg.solve(y, x0=x0, tol=tol, max_iters=max_iters)
+        ok, cg_p, cg_m, verdict = judge(A, y, M, judge_iters, check_every=judge_iters)
+        # the winner carries on from where the judge stopped it; the
+        # loser's iterations count as spent
+        cg, lost = (cg_m, cg_p) if ok else (cg_p, cg_m)
+        if not ok:
+            M = None
+        spent = lost.iters
+        cg.check_every = check_every
+    else:
+        cg = DeviceCG(A, lam=0.0, M=M, check_every=check_every)
+        cg.start(y)
+    x = cg.resume(tol=tol, max_iters=max_iters)
     iters = cg.iters + spent
     converged = cg.history[-1] <= tol
     ynorm = torch.linalg.vector_norm(y)
@@ -444,7 +476,8 @@ def solve(A: SenseNormal, y: torch.Tensor, d: Optional[torch.Tensor] = None,
         iters += cg2.iters
         converged = cg2.history[-1] <= tol
         true_res = float(torch.linalg.vector_norm(y - A(x)) / ynorm)
-    torch.cuda.synchronize()
+    if y.is_cuda:
+        torch.cuda.synchronize()
     return x, dict(seconds=time.perf_counter() - t0, iters=iters, converged=converged,
                    true_residual=true_res, fallback=fallback, history=list(cg.history),
                    judge=verdict)
