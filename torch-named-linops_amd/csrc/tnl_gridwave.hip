@@ -18,15 +18,17 @@
 // lane % 16) belongs to touch k, plane n.  Separability splits the operands:
 //   A[cell (y, z) of patch P][k] = wy_k(y) * wz_k(z)    (4 values: 4 patches)
 //   B_x[k][n]                    = wx_k(x) * v_k[n]     (8 values: 8 planes)
-// so a group costs 12 multiplies for all of its MFMAs.  Each touch's 1-D
-// windows are placed at their tile-local cells through a small per-wave LDS
-// table: rows of 24 slots per (touch, axis), slots 4..19 cleared, the
-// touch's 5 weights written at slot 8 + (first tap - tile origin) + k, and
-// slots 8..15 (the tile's 8 cells) read back, so no lane selects among
-// registers.  Each touch pair carries the union of the block masks of its
-// group (written by the plan), and the group's MFMAs cover that union.  The
-// records of group g + 1 and the touch pairs of group g + 2 load while group
-// g is processed.  At the end of the item the
+// so a group costs 12 multiplies for all of its MFMAs.  Touches go in
+// batches of 32; both records of a batch's touches come into LDS by direct
+// loads.  Each touch's 1-D windows are placed at their tile-local cells
+// through a small per-wave LDS table: rows of 16 slots per (touch, axis),
+// cleared, the touch's 5 weights written at slot 4 + (first tap - tile
+// origin) + k, and slots 4..11 (the tile's 8 cells) read back, so no lane
+// selects among registers.  Groups of 4 touches go in pairs: each touch pair
+// carries the union of the block masks of its group (written by the plan),
+// and the pair's MFMAs cover the union of its two groups.  The records of
+// batch b + 1 and the touch pairs of batch b + 2 load while batch b is
+// processed.  At the end of the item the
 // accumulators go to HBM one x plane at a time through an LDS stage
 // (64-byte z runs), or with float atomics for split tiles (their region is
 // zeroed by the caller's pre-pass).  No barriers: the waves of a CU never
@@ -113,8 +115,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 3))) void
   // the buffer being read from the one being loaded)
   __shared__ __align__(16) float rb0[4][64][4];
   __shared__ __align__(16) float rb1[4][64][4];
-  // tile-local windows of the batch's touches: [touch][axis][8 cells]
-  __shared__ __align__(16) float tw[WG_BT * 3 * 8];
+  // tile-local windows of the batch's touches: [touch][axis][16 slots],
+  // the tile's 8 cells at slots 4..11; slots 0..3 and 12..15 catch the taps
+  // that fall outside the tile, so placing a weight needs no test
+  __shared__ __align__(16) float tw[WG_BT * 3 * 16];
   __shared__ __align__(16) float stg[WG_NPL * WG_PS];
 
   const int64_t nunits = (int64_t)(*nitems_ptr) * W.nch;
@@ -170,8 +174,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 3))) void
       __builtin_amdgcn_global_load_lds(src + 4 * i, &dst[i][0][0], 16, 0, 0);
   };
   // Place the tile-local windows of the batch in rb (touches 0..left-1)
-  // into tw: clear it, then each (touch, axis) task writes the weights of
-  // its kept taps that fall inside the tile.
+  // into tw: each (touch, axis) task clears its row, then writes the
+  // weights of its kept taps at their cells.  All reads of a lane's tasks
+  // are issued before its stores (one LDS round trip, no per-tap waits).
   // per lane and pass j: the task's axis, tile origin and grid size along it
   // (masks, not selects: selects over an array would become a stack load)
   int st_a[2], st_org[2], st_n[2];
@@ -184,22 +189,37 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 3))) void
     st_n[j] = (W.n[0] & m0) | (W.n[1] & m1) | (W.n[2] & m2);
   }
   auto stage_windows = [&](const float (*rb)[64][4], int left) {
+    unsigned base[2];
+    float wk[2][5];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-      reinterpret_cast<float4*>(tw)[64 * i + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int j = 0; j < 2; ++j) {
+      const int task = 64 * j + lane, a = st_a[j];
+      const int t = task < 3 * WG_BT ? task / 3 : 0;
+      base[j] = __float_as_uint(rb[3][t][3]);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const int wd = 5 * a + k;
+        wk[j][k] = rb[wd >> 2][t][wd & 3];
+      }
+    }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int task = 64 * j + lane, t = task / 3, a = st_a[j];
-      if (task < 3 * WG_BT && t < left) {
-        const unsigned base = __float_as_uint(rb[3][t][3]);
-        int d = (int)((base >> (10 * a)) & 1023u) - st_org[j];
-        const int n = st_n[j];
-        if (W.circ && d > 7) d -= n;
-        float* row = tw + (t * 3 + a) * 8;
+      if (task < 3 * WG_BT) {
+        // every row is cleared, live or not: a row of a dead touch still
+        // multiplies a zero value, which must not meet a stale weight
+        float* row = tw + (t * 3 + a) * 16;
 #pragma unroll
-        for (int k = 0; k < 5; ++k) {
-          const int c = d + k, wd = 5 * a + k;
-          if (c >= 0 && c < 8) row[c] = rb[wd >> 2][t][wd & 3];
+        for (int i = 0; i < 4; ++i)
+          reinterpret_cast<float4*>(row)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (t < left) {
+          int d = (int)((base[j] >> (10 * a)) & 1023u) - st_org[j];
+          if (W.circ && d > 7) d -= st_n[j];
+          // a touch's first tap lies 4 cells before the tile at most and
+          // inside it at the latest; the clamp keeps the stores in the row
+          d = d < -4 ? -4 : (d > 7 ? 7 : d);
+#pragma unroll
+          for (int k = 0; k < 5; ++k) row[4 + d + k] = wk[j][k];
         }
       }
     }
@@ -212,14 +232,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 3))) void
   };
   auto ops_of = [&](const float (*rb)[64][4], int gg, bool live) {
     const int t = 4 * gg + k4;
-    const float* r = tw + t * 24;
+    const float* r = tw + t * 48 + 4;  // x, y, z rows at r, r + 16, r + 32
     Ops o;
     o.x0 = *reinterpret_cast<const float4*>(r);
     o.x1 = *reinterpret_cast<const float4*>(r + 4);
-    o.wy0 = r[8 + yy];
-    o.wy1 = r[12 + yy];
-    o.wz0 = r[16 + zz];
-    o.wz1 = r[20 + zz];
+    o.wy0 = r[16 + yy];
+    o.wy1 = r[20 + yy];
+    o.wz0 = r[32 + zz];
+    o.wz1 = r[36 + zz];
     o.v = live ? rb[n16 >> 2][WG_BT + t][n16 & 3] : 0.f;
     return o;
   };
