@@ -354,8 +354,9 @@ def test_c2_full_size_properties(dev):
     """The headline configuration at full size: 8 coils, 256^3 complex64,
     20000 x 512 radial points, KB w=4.  The planned kernels (LDS-staged
     interpolation, matrix-core gridding with point records) against the
-    plan-free kernels, adjointness in float64, linearity, and the oracle
-    on 4096 sampled points of the interpolation."""
+    plan-free kernels, adjointness in float64, linearity, the oracle on
+    4096 sampled points of the interpolation and on 124 sampled cells of
+    the gridding."""
     import torchlinops_amd.functional as F
     from torchlinops_amd import config
 
@@ -398,6 +399,26 @@ def test_c2_full_size_properties(dev):
     e_o = rel_l2(got, want)
     print(f"oracle on 4096 sampled points: {e_o:.2e}", flush=True)
     assert e_o <= TOL
+    # the gridding output against the float64 oracle on sampled cells: the
+    # densest 4^3 block (the centre every spoke crosses, ~10^5 touches per
+    # cell) and 60 random cells.  Gridding sums over points, so the points
+    # whose footprints reach those cells (circular distance <= w/2 on every
+    # axis) give the whole value there; coils 0 and 7.
+    rs = np.random.default_rng(23)
+    centre = np.stack(np.meshgrid(*[np.arange(126, 130)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    cells = np.concatenate([centre, rs.integers(0, n, (60, 3))])
+    near = torch.zeros(locs.shape[0], dtype=torch.bool, device=dev)
+    for c in torch.from_numpy(cells).to(dev, torch.float32):
+        d = (locs - c).abs()
+        near |= (torch.minimum(d, n - d) <= 2.0).all(dim=1)
+    sel = near.nonzero().squeeze(1)
+    print(f"oracle on {len(cells)} grid cells from {sel.numel()} points", flush=True)
+    want = O.grid(y[[0, 7]][:, sel].cpu().numpy(), locs[sel].cpu().numpy(), (n, n, n), 4.0,
+                  "kaiser_bessel", 1, "circular", BETA, "f64")
+    at = (slice(None),) + tuple(cells.T)
+    e_c = rel_l2(ahy[[0, 7]].cpu().numpy()[at], want[at])
+    print(f"oracle on sampled grid cells: {e_c:.2e}", flush=True)
+    assert e_c <= TOL
 
 
 F64_CASES = [
