@@ -202,16 +202,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 3))) void
         wk[j][k] = rb[wd >> 2][t][wd & 3];
       }
     }
+    // every row is cleared, live or not (a row of a dead touch still
+    // multiplies a zero value, which must not meet a stale weight); lane l
+    // clears 16-byte units l, l + 64, ..., so each store instruction covers
+    // 1 KB without bank conflicts (clearing row by row put 16 lanes on the
+    // same banks)
+#pragma unroll
+    for (int i = 0; i < WG_BT * 3 * 16 / 4 / 64; ++i)
+      reinterpret_cast<float4*>(tw)[64 * i + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int task = 64 * j + lane, t = task / 3, a = st_a[j];
       if (task < 3 * WG_BT) {
-        // every row is cleared, live or not: a row of a dead touch still
-        // multiplies a zero value, which must not meet a stale weight
         float* row = tw + (t * 3 + a) * 16;
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-          reinterpret_cast<float4*>(row)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (t < left) {
           int d = (int)((base[j] >> (10 * a)) & 1023u) - st_org[j];
           if (W.circ && d > 7) d -= st_n[j];
