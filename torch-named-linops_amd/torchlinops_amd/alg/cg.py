@@ -154,9 +154,13 @@ class DeviceCG:
     def resume(self, tol: float = 1e-6, max_iters: int = 200) -> torch.Tensor:
         """Continue the iteration from its current state (the same Krylov
         sequence as one uninterrupted run) until the relative residual is
-        below ``tol`` or ``max_iters`` iterations in total have run."""
+        below ``tol`` or ``max_iters`` iterations in total have run.  The
+        residual is checked at multiples of ``check_every`` iterations of
+        the whole run (a run stopped between two checks catches up to the
+        next multiple first)."""
         while self.iters < max_iters and self.history[-1] > tol:
-            n = min(self.check_every, max_iters - self.iters)
+            n = self.check_every - self.iters % self.check_every
+            n = min(n, max_iters - self.iters)
             if self.use_graph and self.y.is_cuda and n == self.check_every:
                 if self._graph is None or self._graph_n != n:
                     # runs this block eagerly, then records it
