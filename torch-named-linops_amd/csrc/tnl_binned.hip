@@ -984,17 +984,14 @@ __device__ __forceinline__ int mf_slot(int l) {
 }
 
 // TL-cell tile-local window of axis d for the point at coordinate t,
-// written to row[0..TL): each tap's weight lands in the slot of the cell it
-// falls on (taps wrap around the grid under circular padding; no two taps
-// share a cell, as the footprint never exceeds the grid), zero elsewhere.
-// Returns the mask of cells some tap falls on.
+// written to row[0..TL), which the caller has cleared: each tap's weight
+// lands in the slot of the cell it falls on (taps wrap around the grid
+// under circular padding; no two taps share a cell, as the footprint never
+// exceeds the grid).  Returns the mask of cells some tap falls on.
 template <int TL, int ORD>
 __device__ __forceinline__ unsigned mf_axis_place(float t, int d, int origin,
                                                   const InterpGeom& G,
                                                   float* __restrict__ row) {
-  float4* r4 = reinterpret_cast<float4*>(row);
-#pragma unroll
-  for (int i = 0; i < TL / 4; ++i) r4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
   unsigned pm = 0;
   const TapRange r = tap_range(t, G.half[d], G.ntap[d], G.n[d], G.pad);
   // The first kept tap wrapped into [0, n) under circular padding: one add
@@ -1126,6 +1123,22 @@ __device__ __forceinline__ void mf_phase1(const MfLds<NDIM>& S, const MfLane& ln
   unsigned m = 0;
   float* ra = S.wa + tid * Sh::RA;
   float* rb = S.wb + tid * Sh::RB;
+  // clear the window rows of this wave's 64 points (live or not: a dead
+  // point's rows still meet zero values in the MFMAs), lane l taking the
+  // 16-byte units l, l + 64, ... of the wave's contiguous rows, so each
+  // store covers 1 KB without bank conflicts (a lane clearing its own row
+  // put lanes 128 or 64 bytes apart on the same banks); the rows are this
+  // wave's own, and its LDS stores complete in order, so no barrier
+  {
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int w0 = tid & ~63, lane = tid & 63;
+    float4* a4 = reinterpret_cast<float4*>(S.wa + w0 * Sh::RA);
+    float4* b4 = reinterpret_cast<float4*>(S.wb + w0 * Sh::RB);
+#pragma unroll
+    for (int i = 0; i < Sh::RA / 4; ++i) a4[64 * i + lane] = z4;
+#pragma unroll
+    for (int i = 0; i < Sh::RB / 4; ++i) b4[64 * i + lane] = z4;
+  }
   if (live) {
     if constexpr (NDIM == 3) {
       const unsigned px = mf_axis_place<8, MF_X8>(pt[0], 0, origin[0], G, ra);
@@ -1137,12 +1150,6 @@ __device__ __forceinline__ void mf_phase1(const MfLds<NDIM>& S, const MfLane& ln
       const unsigned pz = mf_axis_place<32, MF_Z32>(pt[1], 2, origin[2], G, ra);
       m = tile_blocks32_2d(py, pz);
     }
-  } else {
-    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-    for (int i = 0; i < Sh::RA / 4; ++i) reinterpret_cast<float4*>(ra)[i] = z4;
-#pragma unroll
-    for (int i = 0; i < Sh::RB / 4; ++i) reinterpret_cast<float4*>(rb)[i] = z4;
   }
   float tv[MF_NPL];
 #pragma unroll
