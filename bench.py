@@ -133,6 +133,22 @@ def blocks_leg(dev, steps: int = 20):
     torch.cuda.synchronize()
     tu = sum(e[0].elapsed_time(e[1]) for e in ev) / steps * 1e-3
     tf = sum(e[1].elapsed_time(e[2]) for e in ev) / steps * 1e-3
+    # each op repeated back to back: the 134 MB image then stays in the
+    # 256 MB last-level cache between unfold calls (the blocks are written
+    # with streaming stores, which do not displace it); alternating with
+    # fold above, fold's 976 MB of block reads evict it
+    rep = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    torch.cuda.synchronize()
+    rep[0].record()
+    for _ in range(steps):
+        F.array_to_blocks(x, blk, st)
+    rep[1].record()
+    for _ in range(steps):
+        F.blocks_to_array(b, (n, n, n), blk, st)
+    rep[2].record()
+    torch.cuda.synchronize()
+    tu_rep = rep[0].elapsed_time(rep[1]) / steps * 1e-3
+    tf_rep = rep[1].elapsed_time(rep[2]) / steps * 1e-3
     # practical ceilings on this part: torch's copy of the blocks tensor
     # (976 MB read + written) and a write-only fill of it
     c = torch.empty_like(b)
@@ -170,6 +186,10 @@ def blocks_leg(dev, steps: int = 20):
         "fold_GBps": bytes_ / tf / 1e9,
         "unfold_frac": bytes_ / tu / HBM_PEAK,
         "fold_frac": bytes_ / tf / HBM_PEAK,
+        "unfold_repeated_us": tu_rep * 1e6,
+        "fold_repeated_us": tf_rep * 1e6,
+        "unfold_repeated_frac": bytes_ / tu_rep / HBM_PEAK,
+        "fold_repeated_frac": bytes_ / tf_rep / HBM_PEAK,
         "checks_pass": ok_unfold and ok_fold,
         "torch_copy_blocks_us": t_copy * 1e6,
         "torch_copy_GBps": 2 * b.numel() * 8 / t_copy / 1e9,
@@ -800,7 +820,9 @@ def main() -> int:
                 out["blocks"][f"{key}_traffic"] = hit[0] if hit else None
         log(f"blocks: unfold {out['blocks']['unfold_us']:.1f} us "
             f"({out['blocks']['unfold_frac'] * 100:.1f} %), fold {out['blocks']['fold_us']:.1f} us "
-            f"({out['blocks']['fold_frac'] * 100:.1f} %), checks {out['blocks']['checks_pass']}; "
+            f"({out['blocks']['fold_frac'] * 100:.1f} %) alternating; repeated: unfold "
+            f"{out['blocks']['unfold_repeated_us']:.1f} us, fold "
+            f"{out['blocks']['fold_repeated_us']:.1f} us; checks {out['blocks']['checks_pass']}; "
             f"torch copy of the blocks {out['blocks']['torch_copy_GBps']:.0f} GB/s, "
             f"fill {out['blocks']['torch_fill_GBps']:.0f} GB/s")
     if not args.no_c1 and not args.small:

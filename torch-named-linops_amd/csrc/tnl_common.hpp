@@ -24,6 +24,46 @@ int check_hip(hipError_t e, const char* what);
     if (_e != hipSuccess) return ::tnl::check_hip(_e, what);       \
   } while (0)
 
+// ------------------------------------------------- streaming accesses
+// Stores and loads marked non-temporal (the `nt` bit of the vector memory
+// instruction): for data that streams through once, so it does not displace
+// data that is reused from the caches.  Plain vector memory instructions;
+// any type of 1, 2, 4, 8 or 16 bytes.  Measured on array_to_blocks (8 x
+// 128^3 complex64, 8^3 blocks, stride 4: 976 MB of blocks written, the
+// image re-read 8 times from L2): 303 -> 180 us with streaming stores.
+template <typename T>
+__device__ __forceinline__ void stream_store(T* p, const T& v) {
+  typedef unsigned u4 __attribute__((ext_vector_type(4)));
+  typedef unsigned u2 __attribute__((ext_vector_type(2)));
+  if constexpr (sizeof(T) == 16) {
+    __builtin_nontemporal_store(*reinterpret_cast<const u4*>(&v), reinterpret_cast<u4*>(p));
+  } else if constexpr (sizeof(T) == 8) {
+    __builtin_nontemporal_store(*reinterpret_cast<const u2*>(&v), reinterpret_cast<u2*>(p));
+  } else if constexpr (sizeof(T) == 4) {
+    __builtin_nontemporal_store(*reinterpret_cast<const unsigned*>(&v),
+                                reinterpret_cast<unsigned*>(p));
+  } else {
+    *p = v;
+  }
+}
+template <typename T>
+__device__ __forceinline__ T stream_load(const T* p) {
+  typedef unsigned u4 __attribute__((ext_vector_type(4)));
+  typedef unsigned u2 __attribute__((ext_vector_type(2)));
+  T v;
+  if constexpr (sizeof(T) == 16) {
+    *reinterpret_cast<u4*>(&v) = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p));
+  } else if constexpr (sizeof(T) == 8) {
+    *reinterpret_cast<u2*>(&v) = __builtin_nontemporal_load(reinterpret_cast<const u2*>(p));
+  } else if constexpr (sizeof(T) == 4) {
+    *reinterpret_cast<unsigned*>(&v) =
+        __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(p));
+  } else {
+    v = *p;
+  }
+  return v;
+}
+
 // ------------------------------------------------------- kernel weights
 // 16^k / (k!)^2: coefficients of I0(z) = sum_k u^k 16^k / (k!)^2 with
 // u = z^2 / 64.  The factor 16^k keeps every coefficient up to k = 31 a

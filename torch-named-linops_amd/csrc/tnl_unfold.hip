@@ -74,7 +74,7 @@ __global__ __launch_bounds__(UNFOLD_THREADS) void unfold_kernel(
 #pragma unroll
     for (int k = 0; k < UNFOLD_SLOTS; ++k) {
       const int64_t e = threadIdx.x + (int64_t)k * UNFOLD_THREADS;
-      if (e < vpb) y[obase + e] = x[ibase + ioff[k]];
+      if (e < vpb) stream_store(&y[obase + e], x[ibase + ioff[k]]);
     }
   }
 }
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(UNFOLD_THREADS) void unfold_large_kernel(
       const int64_t q = e - j0 * B12;
       const int64_t j1 = q / g.B[2];
       const int64_t j2 = q - j1 * g.B[2];
-      y[obase + e] = x[ibase + j0 * S0 + j1 * S1 + j2];
+      stream_store(&y[obase + e], x[ibase + j0 * S0 + j1 * S1 + j2]);
     }
   }
 }
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) void unfold32_kernel(const T* __restrict__ x,
 #pragma unroll
     for (int u = 0; u < UN; ++u) {
       const uint32_t o = o0 + 256u * u;
-      if (o < hi) y[o] = v[u];
+      if (o < hi) stream_store(&y[o], v[u]);
     }
   }
 }
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(256) void fold32_kernel(const S* __restrict__ x,
         }
       }
     }
-    yv[q] = acc;
+    stream_store(&yv[q], acc);
   }
 }
 
