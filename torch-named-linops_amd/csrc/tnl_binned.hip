@@ -375,7 +375,7 @@ __device__ __forceinline__ void ungrid_sub(
 }
 
 template <int NDIM, bool CPLX, int CC, int NT, bool ILV>
-__global__ __launch_bounds__(BIN_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void ungrid_binned_kernel(
+__global__ __launch_bounds__(BIN_THREADS) void ungrid_binned_kernel(
     const float* __restrict__ vals, float* __restrict__ out, int64_t nbatch,
     InterpGeom G, BinGeom B, const int* __restrict__ perm,
     const float* __restrict__ slocs, const int4* __restrict__ sub,
