@@ -117,8 +117,10 @@ __device__ __forceinline__ void ungrid_sub(
   const int w2 = B.win[2];
   const int nrows = CC * B.win[0] * B.win[1];
 
-  // per-point data cached across channel chunks
-  float cw[UPTS][3 * NT];
+  // per-point data cached across channel chunks (the weights of the
+  // grid's axes only: a 2-D point's leading axis is the constant 1, 0, ...)
+  constexpr int WOFF = (3 - NDIM) * NT;
+  float cw[UPTS][NDIM * NT];
   int crow[UPTS], ccnt[UPTS], cpnt[UPTS];
 
   static_assert(!ILV || (NDIM == 3 && CPLX && CC == 2 && NT == 5), "interleaved form");
@@ -236,7 +238,9 @@ __device__ __forceinline__ void ungrid_sub(
       for (int k = 0; k < UPTS; ++k)
         if (c0 > 0 && slot == k) {
 #pragma unroll
-          for (int e = 0; e < 3 * NT; ++e) w[e] = cw[k][e];
+          for (int e = 0; e < WOFF; ++e) w[e] = e == 0 ? 1.0f : 0.0f;
+#pragma unroll
+          for (int e = 0; e < NDIM * NT; ++e) w[WOFF + e] = cw[k][e];
           rowoff = crow[k];
           cnt0 = ccnt[k] & 255; cnt1 = (ccnt[k] >> 8) & 255; cnt2 = (ccnt[k] >> 16) & 255;
           p = cpnt[k];
@@ -278,7 +282,7 @@ __device__ __forceinline__ void ungrid_sub(
         for (int k = 0; k < UPTS; ++k)
           if (slot == k) {
 #pragma unroll
-            for (int e = 0; e < 3 * NT; ++e) cw[k][e] = w[e];
+            for (int e = 0; e < NDIM * NT; ++e) cw[k][e] = w[WOFF + e];
             crow[k] = rowoff;
             ccnt[k] = cnt0 | (cnt1 << 8) | (cnt2 << 16);
             cpnt[k] = p;
@@ -318,6 +322,27 @@ __device__ __forceinline__ void ungrid_sub(
                   C::fma(acc[0], wt, make_float2(qv[ky][kz].x, qv[ky][kz].y));
                   C::fma(acc[CC - 1], wt, make_float2(qv[ky][kz].z, qv[ky][kz].w));
                 }
+              }
+            }
+            continue;
+          }
+          if constexpr (NDIM == 2 && CC >= 4) {
+            // 2-D with 4+ channels: one y row of 4 x CC reads at a time
+            // (the whole 4 x 4 x CC window would take 16 CC registers)
+            const T* rp = lds + rowoff;
+#pragma unroll
+            for (int ky = 0; ky < 4; ++ky) {
+              T tr[4][CC];
+#pragma unroll
+              for (int kz = 0; kz < 4; ++kz)
+#pragma unroll
+                for (int c = 0; c < CC; ++c) tr[kz][c] = rp[ky * B.win[2] + c * B.wcells + kz];
+              const float wxy = wx * w[NT + ky];
+#pragma unroll
+              for (int kz = 0; kz < 4; ++kz) {
+                const float wt = wxy * w[2 * NT + kz];
+#pragma unroll
+                for (int c = 0; c < CC; ++c) C::fma(acc[c], wt, tr[kz][c]);
               }
             }
             continue;
