@@ -78,6 +78,44 @@ def test_batched_rejects_mixed_tiling_and_wide_3d(dev):
         F.BatchedInterp(l * 2, [(16, 16), (16, 16, 16)], [4.0, 4.0], [1, 1])
 
 
+def test_batched_wide_3d_on_matrix_cores(dev):
+    """3-D batches wider than 5 taps per axis (widths 5 and 6: 6 and 7
+    taps) whose plans grid on the matrix cores (grids above 16 per axis):
+    batched interpolation and gridding equal the per-problem calls and the
+    float64 oracle.  (With LDS-accumulator plans they are refused, above.)"""
+    rng = np.random.default_rng(710)
+    P = _problems(rng, 3, n=3, grids=[24, 32], mrange=(2000, 8000), cmax=3,
+                  widths=[5.0, 6.0])
+    locs = [to_dev(p["locs"], dev) for p in P]
+    B = F.BatchedInterp(locs, [p["grid"] for p in P], [p["w"] for p in P], [p["C"] for p in P],
+                        kernel_params=[{"beta": p["beta"]} for p in P])
+    assert B.layout.grid_mfma == 1 and B.layout.nt == 8
+    xs = [crandn(rng, (p["C"], *p["grid"])) for p in P]
+    ys = [crandn(rng, (p["C"], p["M"])) for p in P]
+    gflat = B.new_grid()
+    for v, x in zip(B.grid_views(gflat), xs):
+        v.copy_(to_dev(x, dev))
+    pflat = B.new_points()
+    for v, y in zip(B.pts_views(pflat), ys):
+        v.copy_(to_dev(y, dev))
+    u = B.ungrid(gflat)
+    g = B.grid(pflat)
+    with config.using(interp_plan="always"):
+        for i, p in enumerate(P):
+            kp = {"beta": p["beta"]}
+            su = F.interpolate(to_dev(xs[i], dev), locs[i], p["w"], kernel_params=kp)
+            sg = F.interpolate_adjoint(to_dev(ys[i], dev), locs[i], p["grid"], p["w"],
+                                       kernel_params=kp)
+            bu = B.pts_views(u)[i]
+            bg = B.grid_views(g)[i]
+            assert rel_l2(bu.cpu().numpy(), su.cpu().numpy()) < 1e-6, i
+            assert rel_l2(bg.cpu().numpy(), sg.cpu().numpy()) < 1e-6, i
+            ou = O.ungrid(xs[i], p["locs"], p["w"], beta=p["beta"], precision="f64")
+            og = O.grid(ys[i], p["locs"], p["grid"], p["w"], beta=p["beta"], precision="f64")
+            assert rel_l2(bu.cpu().numpy(), ou) < 1e-5, i
+            assert rel_l2(bg.cpu().numpy(), og) < 1e-5, i
+
+
 def _cg_problems(rng, n=3):
     P = []
     for i in range(n):

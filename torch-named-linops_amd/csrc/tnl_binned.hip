@@ -1854,11 +1854,14 @@ int batch_prepare(const InterpGeom* G, const tnl_batch_problem* P, int nprob,
     L->total_items += pl.item_capacity;
   }
   L->nt = nt <= 5 ? 5 : 8;
-  if (L->ndim == 3 && L->nt > 5)
-    return fail(TNL_EINVAL, "3-D gridding covers at most 5 taps per axis");
   L->grid_mfma = 1;
   for (int i = 0; i < nprob; ++i)
     if (P[i].layout->grid_accum != TNL_GRID_ACCUM_MFMA) L->grid_mfma = 0;
+  // the matrix-core gridding takes any tap count up to 8; the LDS
+  // accumulator's lanes cover 5 per axis in 3-D
+  if (L->ndim == 3 && L->nt > 5 && !L->grid_mfma)
+    return fail(TNL_EINVAL,
+                "3-D batches wider than 5 taps per axis need matrix-core gridding plans");
   L->reserved = 0;
   // channels per workgroup: the largest power of two (<= 8) whose LDS fits
   // every problem under the smallest budget
@@ -2079,9 +2082,13 @@ int batch_run(bool adjoint, const tnl_batch_layout& L, const void* ws,
               : launch_grid_mfma_batched<2, false>(L, D, in, out, s);
   }
   if (L.ndim == 3) {
-    if (L.nt != 5) return fail(TNL_EINVAL, "3-D batches cover at most 5 taps per axis");
-    return cx ? batch_dispatch_cc<3, true, 5>(adjoint, L, D, in, out, s)
-              : batch_dispatch_cc<3, false, 5>(adjoint, L, D, in, out, s);
+    if (L.nt == 5)
+      return cx ? batch_dispatch_cc<3, true, 5>(adjoint, L, D, in, out, s)
+                : batch_dispatch_cc<3, false, 5>(adjoint, L, D, in, out, s);
+    // wider 3-D batches: interpolation here, gridding on the matrix cores
+    // above (the layout refuses them otherwise)
+    return cx ? batch_dispatch_cc<3, true, 8>(adjoint, L, D, in, out, s)
+              : batch_dispatch_cc<3, false, 8>(adjoint, L, D, in, out, s);
   }
   if (L.nt == 5)
     return cx ? batch_dispatch_cc<2, true, 5>(adjoint, L, D, in, out, s)
