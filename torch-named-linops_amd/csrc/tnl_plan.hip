@@ -104,7 +104,12 @@ int plan_layout(int64_t npts, int ndim, const int64_t* grid_size,
   L->g_ntiles_total = (int64_t)L->g_ntiles[0] * L->g_ntiles[1] * L->g_ntiles[2];
   L->max_sub = (tune && tune->max_sub > 0) ? tune->max_sub : 512;
   L->sub_capacity = L->u_nbins + (npts + L->max_sub - 1) / L->max_sub + 1;
-  L->g_cap = (tune && tune->g_cap > 0) ? tune->g_cap : 2048;
+  // touches per gridding work item: 4096 in 3-D (the 3-D radial workload,
+  // 8 x 256^3, 20000 x 512 points, grids in 1.82 ms against 1.91 ms with
+  // 2048: fewer split tiles and atomic adds; 6144 and up are slower, as
+  // items past 4096 touches keep their tile order instead of the block
+  // mask order), 2048 in 2-D
+  L->g_cap = (tune && tune->g_cap > 0) ? tune->g_cap : (ndim == 3 ? 4096 : 2048);
   L->lds_budget = (tune && tune->lds_budget > 0) ? tune->lds_budget : 53248;
   if (L->lds_budget > 160 * 1024)
     return fail(TNL_EINVAL, "lds_budget above the 160 KiB LDS of a CU");
