@@ -257,9 +257,10 @@ def test_plan_sort_forms_agree(dev, n):
     subproblem points by LDS bank) size their LDS for the plan's caps:
     g_cap <= 2048 and max_sub <= 512 take the small forms.  With 2000
     uniform points no item or subproblem reaches those caps (about 30
-    points per 8^3 bin and 250 per 32x32 bin), so raising the caps to
-    g_cap 4096 / max_sub 2048 (the large forms) changes nothing else and
-    must give bit-identical interpolation and gridding."""
+    points per 8^3 bin and 250 per 32x32 bin), so g_cap 2048 / max_sub 512
+    (the small forms, set explicitly: 3-D plans default to g_cap 4096) and
+    g_cap 4096 / max_sub 2048 (the large forms) must give bit-identical
+    interpolation and gridding."""
     import torchlinops_amd.functional as F
     from torchlinops_amd import config
     from torchlinops_amd.functional import _plan
@@ -272,7 +273,7 @@ def test_plan_sort_forms_agree(dev, n):
     y = to_dev(crandn(rng, (3, 2000)), dev)
     outs = []
     try:
-        for knobs in ({}, {"g_cap": 4096, "max_sub": 2048}):
+        for knobs in ({"g_cap": 2048, "max_sub": 512}, {"g_cap": 4096, "max_sub": 2048}):
             with _plan._tuning_lock:
                 _plan._tuning.pop(nd, None)
             _plan.set_tuning(nd, **knobs)
@@ -348,6 +349,48 @@ def test_grid_ws_records_match_direct_reads(dev, n, cplx, nbatch):
     small = torch.empty(nbytes.value - 16, dtype=torch.uint8, device=dev)
     with pytest.raises(ValueError, match="scratch"):
         _lib.check(h.tnl_grid_ws(*args, small.data_ptr(), small.numel(), _lib.stream_of(yv)))
+
+
+UNGRID_FORM_CASES = [
+    # grid_size, nbatch, pad, misaligned: 8^3 bins with 5 taps per axis stage
+    # their 12^3 windows two channels interleaved when every axis has >= 12
+    # cells, the last axis is even and the values are 16-byte aligned
+    ((24, 32, 40), 5, "circular", False),  # interleaved, chunks of 2, 2, 1
+    ((24, 32, 40), 3, "zero", False),  # interleaved, zero padding past the grid
+    ((24, 32, 41), 4, "circular", False),  # odd last axis: channel planes
+    ((24, 32, 40), 4, "circular", True),  # 8-byte aligned values: channel planes
+]
+
+
+@pytest.mark.parametrize("case", UNGRID_FORM_CASES, ids=lambda c: f"{c[0]}-nb{c[1]}-{c[2]}-mis{int(c[3])}")
+def test_ungrid_staging_forms(dev, case):
+    """Each staging form of the 3-D complex interpolation against the
+    oracle, with points on integer coordinates, at the grid edges and
+    outside it (both pads)."""
+    import torchlinops_amd.functional as F
+    from torchlinops_amd import config
+
+    grid_size, nbatch, pad, misaligned = case
+    print(f"ungrid staging form {case}", flush=True)
+    rng = np.random.default_rng(91)
+    locs = rng.uniform(-2, np.array(grid_size) + 2, (4000, 3)).astype(np.float32)
+    locs[:9] = np.round(locs[:9])
+    locs[9:12] = np.array(grid_size, dtype=np.float32) - 0.25
+    x = crandn(rng, (nbatch, *grid_size))
+    xd = to_dev(x, dev)
+    if misaligned:
+        buf = torch.zeros(xd.numel() + 1, dtype=xd.dtype, device=dev)
+        buf[1:].copy_(xd.reshape(-1))
+        xd = buf[1:].view(xd.shape)
+        assert xd.data_ptr() % 16 == 8
+    with config.using(interp_plan="always"):
+        got = F.interpolate(xd, to_dev(locs, dev), 4.0, "kaiser_bessel", 1, pad, {"beta": BETA})
+    got = got.cpu().numpy()
+    ref32 = O.ungrid(x, locs, 4.0, "kaiser_bessel", 1, pad, BETA, "f32")
+    ref64 = O.ungrid(x, locs, 4.0, "kaiser_bessel", 1, pad, BETA, "f64")
+    e32, e64 = rel_l2(got, ref32), rel_l2(got, ref64)
+    print(f"  rel err f32 {e32:.2e} f64 {e64:.2e}", flush=True)
+    assert e32 <= TOL and e64 <= TOL
 
 
 def test_c2_full_size_properties(dev):
