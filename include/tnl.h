@@ -157,6 +157,28 @@ int tnl_ungrid(const float* vals, const float* locs, float* out,
                int norm, int pad_mode, int is_complex, float beta,
                const tnl_plan_layout* layout, const void* plan, void* stream);
 
+/* Scratch bytes that tnl_ungrid_ws uses: npts x nbatch channel values
+ * (x2 floats if complex). */
+int tnl_ungrid_scratch_bytes(int64_t npts, int64_t nbatch, int is_complex, int64_t* bytes);
+
+/* tnl_ungrid with a caller-owned, 16-byte aligned device scratch of at
+ * least tnl_ungrid_scratch_bytes bytes (contents on entry ignored and
+ * overwritten).  With a plan, the interpolation writes each point's values
+ * of all channels side by side into the scratch (whole 64-byte lines for
+ * 8 complex channels) and one tiled transpose writes them to `out` in
+ * its [nbatch, npts] layout.  For trajectories listed in scattered order
+ * (neighbouring points in different bins) this replaces the channel-major
+ * stores, which leave each 64-byte line of `out` partly written by many
+ * workgroups; for trajectories in acquisition order (spokes, spirals) the
+ * plain call is faster.  Without a plan the scratch is unused.  No
+ * counterpart in the reference (it stands beside ungrid.py:103-117).  Same
+ * results as tnl_ungrid, bit for bit. */
+int tnl_ungrid_ws(const float* vals, const float* locs, float* out, int64_t nbatch,
+                  int64_t npts, int ndim, const int64_t* grid_size, const float* width,
+                  int kernel, int norm, int pad_mode, int is_complex, float beta,
+                  const tnl_plan_layout* layout, const void* plan, void* scratch,
+                  int64_t scratch_bytes, void* stream);
+
 /* ---- gridding (interpolate adjoint) ------------------------------------
  * out[b, g] = sum_{p : g in taps(p)} w(t_p - g) * vals[b, p]
  * Replaces GRID[ndim][grid](...) at

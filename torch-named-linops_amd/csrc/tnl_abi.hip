@@ -110,9 +110,15 @@ static int interp_entry(bool adjoint, const float* vals, const float* locs,
       return fail(TNL_EINVAL,
                   "plan was built for another trajectory size, grid, width or pad_mode");
     if (binned_supported(G, L, adjoint)) {
-      if (scratch) {
+      if (scratch && adjoint) {
         if (scratch_bytes < grid_scratch_bytes(L, nbatch, is_complex))
           return fail(TNL_EINVAL, "scratch smaller than tnl_grid_scratch_bytes");
+      }
+      if (scratch && !adjoint) {
+        if (scratch_bytes < npts * nbatch * ncomp * (int64_t)sizeof(float))
+          return fail(TNL_EINVAL, "scratch smaller than tnl_ungrid_scratch_bytes");
+        if (reinterpret_cast<uintptr_t>(scratch) % 16 != 0)
+          return fail(TNL_EINVAL, "scratch must be 16-byte aligned");
       }
       return interp_binned(adjoint, vals, locs, out, nbatch, G, is_complex, L, plan,
                            (float*)scratch, s);
@@ -153,6 +159,27 @@ extern "C" int tnl_grid(const float* vals, const float* locs, float* out,
   return interp_entry(true, vals, locs, out, nbatch, npts, ndim, grid_size,
                       width, kernel, norm, pad_mode, is_complex, beta, layout,
                       plan, nullptr, 0, stream);
+}
+
+extern "C" int tnl_ungrid_scratch_bytes(int64_t npts, int64_t nbatch, int is_complex,
+                                        int64_t* bytes) {
+  if (!bytes) return fail(TNL_EINVAL, "null pointer");
+  if (npts < 0 || nbatch < 0) return fail(TNL_EINVAL, "npts and nbatch must be >= 0");
+  *bytes = npts * nbatch * (is_complex ? 2 : 1) * (int64_t)sizeof(float);
+  return TNL_OK;
+}
+
+extern "C" int tnl_ungrid_ws(const float* vals, const float* locs, float* out,
+                             int64_t nbatch, int64_t npts, int ndim,
+                             const int64_t* grid_size, const float* width,
+                             int kernel, int norm, int pad_mode, int is_complex,
+                             float beta, const tnl_plan_layout* layout,
+                             const void* plan, void* scratch, int64_t scratch_bytes,
+                             void* stream) {
+  if (scratch_bytes < 0) return fail(TNL_EINVAL, "scratch_bytes must be >= 0");
+  return interp_entry(false, vals, locs, out, nbatch, npts, ndim, grid_size,
+                      width, kernel, norm, pad_mode, is_complex, beta, layout,
+                      plan, scratch, scratch_bytes, stream);
 }
 
 extern "C" int tnl_grid_scratch_bytes(const tnl_plan_layout* layout, int64_t nbatch,

@@ -84,11 +84,13 @@ constexpr int ILV_W = 12;
 #define ILV_RY 2
 #endif
 
+// pm: write point-major, out[p * nbatch + c] (all channels of a point side
+// by side) instead of channel-major out[c * npts + p].
 template <int NDIM, bool CPLX, int CC, int NT, bool ILV = false>
 __device__ __forceinline__ void ungrid_sub(
     const float* __restrict__ vals, float* __restrict__ out, int64_t nbatch,
     const InterpGeom& G, const BinGeom& B, const int* __restrict__ perm,
-    const float* __restrict__ slocs, int4 sp) {
+    const float* __restrict__ slocs, int4 sp, bool pm = false) {
   using C = Ch<CPLX>;
   using T = typename C::T;
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -98,7 +100,7 @@ __device__ __forceinline__ void ungrid_sub(
   if (bin == B.nbins) {  // points without taps: zero output
     for (int i = threadIdx.x; i < count; i += BIN_THREADS) {
       const int p = perm[start + i];
-      for (int64_t c = 0; c < nbatch; ++c) o[c * B.npts + p] = C::zero();
+      for (int64_t c = 0; c < nbatch; ++c) o[pm ? p * nbatch + c : c * B.npts + p] = C::zero();
     }
     return;
   }
@@ -394,7 +396,7 @@ __device__ __forceinline__ void ungrid_sub(
       }
 #pragma unroll
       for (int c = 0; c < CC; ++c)
-        if (c < nc) o[(c0 + c) * B.npts + p] = acc[c];
+        if (c < nc) o[pm ? p * nbatch + c0 + c : (c0 + c) * B.npts + p] = acc[c];
     }
   }
 }
@@ -404,7 +406,7 @@ __global__ __launch_bounds__(BIN_THREADS) void ungrid_binned_kernel(
     const float* __restrict__ vals, float* __restrict__ out, int64_t nbatch,
     InterpGeom G, BinGeom B, const int* __restrict__ perm,
     const float* __restrict__ slocs, const int4* __restrict__ sub,
-    const int* __restrict__ nsub_ptr) {
+    const int* __restrict__ nsub_ptr, bool pm) {
   // The grid is sized for the plan's subproblem capacity; remap over the
   // live count only, so that every XCD gets its share of the real work
   // (hardware hands workgroup b to XCD b % NUM_XCD).
@@ -413,7 +415,41 @@ __global__ __launch_bounds__(BIN_THREADS) void ungrid_binned_kernel(
   if ((int64_t)blockIdx.x >= n8) return;
   const int64_t s = xcd_remap(blockIdx.x, n8);
   if (s >= nsub) return;
-  ungrid_sub<NDIM, CPLX, CC, NT, ILV>(vals, out, nbatch, G, B, perm, slocs, sub[s]);
+  ungrid_sub<NDIM, CPLX, CC, NT, ILV>(vals, out, nbatch, G, B, perm, slocs, sub[s], pm);
+}
+
+// Point-major interpolation output (npts x nbatch, from ungrid_sub with pm)
+// to the caller's channel-major layout (nbatch x npts): tiles of UNPK_P
+// points through LDS, 8 channels at a time; the reads run along the
+// point-major rows and the writes along each channel's points.
+constexpr int UNPK_P = 128;
+
+template <bool CPLX>
+__global__ __launch_bounds__(256) void unpack_points_kernel(const float* __restrict__ src,
+                                                            float* __restrict__ dst,
+                                                            int64_t nbatch, int64_t npts) {
+  using T = typename Ch<CPLX>::T;
+  constexpr int TS = UNPK_P + 1;  // padded row: the 8 channels fall in different banks
+  __shared__ T tile[8 * TS];
+  const T* __restrict__ sv = reinterpret_cast<const T*>(src);
+  T* __restrict__ dv = reinterpret_cast<T*>(dst);
+  for (int64_t p0 = (int64_t)blockIdx.x * UNPK_P; p0 < npts;
+       p0 += (int64_t)gridDim.x * UNPK_P) {
+    const int np = (int)(npts - p0 < UNPK_P ? npts - p0 : UNPK_P);
+    for (int64_t c0 = 0; c0 < nbatch; c0 += 8) {
+      const int cc = (int)(nbatch - c0 < 8 ? nbatch - c0 : 8);
+      __syncthreads();  // the previous tile's writes are done
+      for (int e = threadIdx.x; e < np * cc; e += 256) {
+        const int p = e / cc, c = e - p * cc;
+        tile[c * TS + p] = sv[(p0 + p) * nbatch + c0 + c];
+      }
+      __syncthreads();
+      for (int e = threadIdx.x; e < cc * UNPK_P; e += 256) {
+        const int c = e / UNPK_P, p = e - c * UNPK_P;
+        if (p < np) dv[(c0 + c) * npts + p0 + p] = tile[c * TS + p];
+      }
+    }
+  }
 }
 
 // Pre-pass for batched gridding: zero the output region of every split
@@ -1659,7 +1695,7 @@ int64_t grid_scratch_bytes(const tnl_plan_layout& L, int64_t nbatch, int is_comp
 template <int NDIM, bool CPLX, int CC, int NT>
 static int launch_binned(bool adjoint, const float* vals, float* out,
                          int64_t nbatch, const InterpGeom& G,
-                         const tnl_plan_layout& L, const char* ws,
+                         const tnl_plan_layout& L, const char* ws, float* pmb,
                          hipStream_t s) {
   const BinGeom B = make_bin_geom(L, adjoint);
   const int comp_bytes = CPLX ? 8 : 4;
@@ -1678,9 +1714,15 @@ static int launch_binned(bool adjoint, const float* vals, float* out,
     }
     if (int rc = allow_lds((const void*)kern, lds)) return rc;
     kern<<<dim3((unsigned)nwg, 1), BIN_THREADS, lds, s>>>(
-        vals, out, nbatch, G, B, perm, slocs, (const int4*)(ws + L.off_sub),
-        (const int*)(ws + L.off_suboff) + L.u_nbins + 1);
+        vals, pmb ? pmb : out, nbatch, G, B, perm, slocs, (const int4*)(ws + L.off_sub),
+        (const int*)(ws + L.off_suboff) + L.u_nbins + 1, pmb != nullptr);
     TNL_LAUNCH_CHECK("ungrid_binned");
+    if (pmb) {
+      int64_t nwu = (B.npts + UNPK_P - 1) / UNPK_P;
+      if (nwu > 16384) nwu = 16384;
+      unpack_points_kernel<CPLX><<<(unsigned)nwu, 256, 0, s>>>(pmb, out, nbatch, B.npts);
+      TNL_LAUNCH_CHECK("unpack_points");
+    }
   } else if constexpr (NDIM == 3 && NT > 5) {
     return fail(TNL_EINVAL, "3-D gridding covers at most 5 taps per axis");
   } else {
@@ -1707,7 +1749,7 @@ static int launch_binned(bool adjoint, const float* vals, float* out,
 template <int NDIM, bool CPLX, int NT>
 static int dispatch_cc(bool adjoint, const float* vals, float* out,
                        int64_t nbatch, const InterpGeom& G,
-                       const tnl_plan_layout& L, const char* ws,
+                       const tnl_plan_layout& L, const char* ws, float* pmb,
                        hipStream_t s) {
   const BinGeom B = make_bin_geom(L, adjoint);
   int cc;
@@ -1723,10 +1765,10 @@ static int dispatch_cc(bool adjoint, const float* vals, float* out,
       cc *= 2;
   }
   switch (cc) {
-    case 8: return launch_binned<NDIM, CPLX, 8, NT>(adjoint, vals, out, nbatch, G, L, ws, s);
-    case 4: return launch_binned<NDIM, CPLX, 4, NT>(adjoint, vals, out, nbatch, G, L, ws, s);
-    case 2: return launch_binned<NDIM, CPLX, 2, NT>(adjoint, vals, out, nbatch, G, L, ws, s);
-    default: return launch_binned<NDIM, CPLX, 1, NT>(adjoint, vals, out, nbatch, G, L, ws, s);
+    case 8: return launch_binned<NDIM, CPLX, 8, NT>(adjoint, vals, out, nbatch, G, L, ws, pmb, s);
+    case 4: return launch_binned<NDIM, CPLX, 4, NT>(adjoint, vals, out, nbatch, G, L, ws, pmb, s);
+    case 2: return launch_binned<NDIM, CPLX, 2, NT>(adjoint, vals, out, nbatch, G, L, ws, pmb, s);
+    default: return launch_binned<NDIM, CPLX, 1, NT>(adjoint, vals, out, nbatch, G, L, ws, pmb, s);
   }
 }
 
@@ -1739,8 +1781,10 @@ static int dispatch_nt(bool adjoint, const float* vals, const float* locs, float
   for (int d = 1; d < 3; ++d) nt = G.ntap[d] > nt ? G.ntap[d] : nt;
   if (adjoint && use_grid_mfma(G, L))
     return launch_grid_mfma<NDIM, CPLX>(vals, locs, out, nbatch, G, L, ws, scratch, s);
-  if (nt <= 5) return dispatch_cc<NDIM, CPLX, 5>(adjoint, vals, out, nbatch, G, L, ws, s);
-  return dispatch_cc<NDIM, CPLX, 8>(adjoint, vals, out, nbatch, G, L, ws, s);
+  // interpolation with a scratch: point-major into it, then one transpose
+  float* pmb = adjoint ? nullptr : scratch;
+  if (nt <= 5) return dispatch_cc<NDIM, CPLX, 5>(adjoint, vals, out, nbatch, G, L, ws, pmb, s);
+  return dispatch_cc<NDIM, CPLX, 8>(adjoint, vals, out, nbatch, G, L, ws, pmb, s);
 }
 
 int interp_binned(bool adjoint, const float* vals, const float* locs, float* out,

@@ -196,6 +196,12 @@ SIGNATURES = {
         [_vp, _vp, _vp, _i64, _i64, _i32, _i64p, _f32p, _i32, _i32, _i32,
          _i32, _f32, _layout_p, _vp, _vp],
     ),
+    "tnl_ungrid_scratch_bytes": (_i32, [_i64, _i64, _i32, _i64p]),
+    "tnl_ungrid_ws": (
+        _i32,
+        [_vp, _vp, _vp, _i64, _i64, _i32, _i64p, _f32p, _i32, _i32, _i32,
+         _i32, _f32, _layout_p, _vp, _vp, _i64, _vp],
+    ),
     "tnl_grid_scratch_bytes": (_i32, [_layout_p, _i64, _i32, _i64p]),
     "tnl_grid_ws": (
         _i32,

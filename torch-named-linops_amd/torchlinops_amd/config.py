@@ -2,7 +2,8 @@
 
 The first four names and their defaults follow the reference's module-level
 flags (src/torchlinops/config.py:32-42) and its `using` context manager
-(config.py:51-120).  `interp_plan` is specific to this library.
+(config.py:51-120).  `interp_plan` and `interp_point_major` are specific
+to this library.
 
 reduce_identity_in_normal : bool
     Drop an inner Identity inside ``normal(inner)``.  Default True.
@@ -16,6 +17,13 @@ interp_plan : str
     ``"auto"`` (default) builds and caches a point-binning plan for 2-D/3-D
     interpolation when the problem is large enough to profit; ``"always"``
     / ``"never"`` force the binned or the direct kernels.
+interp_point_major : str
+    How planned interpolation writes its output.  ``"auto"`` (default)
+    writes each point's channels side by side into a scratch and transposes
+    them into the output when the plan found the trajectory listed in
+    scattered order and there are at least 8 floats per point (4 complex
+    channels); ``"always"`` / ``"never"`` force that form or the direct
+    channel-major stores.  Results are the same bits either way.
 """
 
 from __future__ import annotations
@@ -27,6 +35,7 @@ cache_adjoint_normal: bool = True
 log_device_transfers: bool = True
 shape_inference: bool = False
 interp_plan: str = "auto"
+interp_point_major: str = "auto"
 
 
 def inner_not_relevant(inner) -> bool:
@@ -44,6 +53,7 @@ class ConfigContext:
         "log_device_transfers",
         "shape_inference",
         "interp_plan",
+        "interp_point_major",
     }
 
     def __init__(self, **kwargs) -> None:

@@ -40,6 +40,7 @@ from .._lib import (
     stream_of,
 )
 from ._host import grid_host, ungrid_host
+from .. import config
 from ._plan import get_plan
 
 __all__ = [
@@ -176,7 +177,23 @@ def _launch(adjoint: bool, vals_flat, locs, locs2d, out, nbatch, npts, ndim,
                 check(lib().tnl_plan_windows(L, plan.ptr, locs2d.data_ptr(), KERNELS[kernel],
                                              beta, stream_of(vals_flat)))
         if not adjoint:
-            check(lib().tnl_ungrid(*args, stream_of(vals_flat)))
+            pm = config.interp_point_major
+            if pm not in ("auto", "always", "never"):
+                raise ValueError(
+                    f"config.interp_point_major must be auto/always/never, got {pm!r}")
+            floats = nbatch * (2 if is_complex else 1)
+            if plan is not None and (pm == "always" or (pm == "auto" and plan.scattered
+                                                        and floats >= 8)):
+                # point-major values into a scratch, then one transpose into
+                # out (whole lines instead of scattered 8-byte stores)
+                nbytes = ctypes.c_int64(0)
+                check(lib().tnl_ungrid_scratch_bytes(npts, nbatch, int(is_complex),
+                                                     ctypes.byref(nbytes)))
+                scratch = torch.empty(nbytes.value, dtype=torch.uint8, device=vals_flat.device)
+                check(lib().tnl_ungrid_ws(*args, scratch.data_ptr(), nbytes.value,
+                                          stream_of(vals_flat)))
+            else:
+                check(lib().tnl_ungrid(*args, stream_of(vals_flat)))
             return
         nbytes = ctypes.c_int64(0)
         if plan is not None:

@@ -102,10 +102,30 @@ def plan_layout(npts, ndim, grid_size, width, pad_mode_code, tuning=None) -> Pla
 class Plan:
     layout: PlanLayout
     workspace: torch.Tensor
+    # trajectory listed in scattered order (order_coherence below
+    # SCATTERED_BELOW): interpolation then writes through a point-major scratch
+    scattered: bool = False
 
     @property
     def ptr(self) -> int:
         return self.workspace.data_ptr()
+
+
+# Below this fraction of caller-order neighbours sharing an interpolation bin
+# a trajectory counts as listed in scattered order: spokes and spirals in
+# acquisition order give 0.9 and more, points in random order about 0.
+SCATTERED_BELOW = 0.25
+
+
+def order_coherence(locs2d: torch.Tensor, tile) -> float:
+    """Fraction of consecutive points (p, p + 1) of the caller's order whose
+    coordinates fall in the same interpolation bin (tile of `tile` cells per
+    axis).  One pass over the coordinates and one host read."""
+    if locs2d.shape[0] < 2:
+        return 1.0
+    t = torch.tensor([float(v) for v in tile], dtype=locs2d.dtype, device=locs2d.device)
+    key = torch.floor(locs2d / t)
+    return float((key[1:] == key[:-1]).all(dim=1).float().mean())
 
 
 def build_plan(locs2d: torch.Tensor, ndim, grid_size, width, pad_code, tuning=None) -> Plan:
@@ -115,7 +135,9 @@ def build_plan(locs2d: torch.Tensor, ndim, grid_size, width, pad_code, tuning=No
     with torch.cuda.device(locs2d.device):
         check(lib().tnl_plan_build(locs2d.data_ptr(), L, ws.data_ptr(), int(L.bytes),
                                    stream_of(locs2d)))
-    return Plan(L, ws)
+        tile = [L.u_tile[d] for d in range(3 - ndim, 3)]
+        scattered = order_coherence(locs2d, tile) < SCATTERED_BELOW
+    return Plan(L, ws, scattered)
 
 
 class _PlanCache:
