@@ -18,6 +18,11 @@ def test_spokes_in_order_are_coherent_and_shuffled_are_scattered():
     shuffled = order_coherence(locs[torch.randperm(locs.shape[0])], (8, 8, 8))
     print(f"coherence ordered {ordered:.3f} shuffled {shuffled:.3f}")
     assert ordered > 0.8 > SCATTERED_BELOW > 0.05 > shuffled
+    # long lists are sampled in runs: the estimate stays on the same side
+    sampled = order_coherence(locs, (8, 8, 8), runs=8, run_len=256)
+    sampled_shuffled = order_coherence(locs[torch.randperm(locs.shape[0])], (8, 8, 8),
+                                       runs=8, run_len=256)
+    assert abs(sampled - ordered) < 0.1 and sampled_shuffled < 0.05
 
 
 def test_spiral_2d_and_tiny_inputs():

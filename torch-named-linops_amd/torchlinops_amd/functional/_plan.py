@@ -117,13 +117,20 @@ class Plan:
 SCATTERED_BELOW = 0.25
 
 
-def order_coherence(locs2d: torch.Tensor, tile) -> float:
+def order_coherence(locs2d: torch.Tensor, tile, runs: int = 256, run_len: int = 4096) -> float:
     """Fraction of consecutive points (p, p + 1) of the caller's order whose
     coordinates fall in the same interpolation bin (tile of `tile` cells per
-    axis).  One pass over the coordinates and one host read."""
-    if locs2d.shape[0] < 2:
+    axis).  Lists longer than runs * run_len points are sampled as `runs`
+    evenly spaced runs of `run_len` consecutive points.  One host read."""
+    n = locs2d.shape[0]
+    if n < 2:
         return 1.0
     t = torch.tensor([float(v) for v in tile], dtype=locs2d.dtype, device=locs2d.device)
+    if n > runs * run_len:
+        starts = torch.linspace(0, n - run_len, runs, device=locs2d.device).long()
+        idx = starts[:, None] + torch.arange(run_len, device=locs2d.device)[None, :]
+        key = torch.floor(locs2d[idx] / t)  # [runs, run_len, ndim]
+        return float((key[:, 1:] == key[:, :-1]).all(dim=-1).float().mean())
     key = torch.floor(locs2d / t)
     return float((key[1:] == key[:-1]).all(dim=1).float().mean())
 
