@@ -102,6 +102,51 @@ def cpu_baseline(vals_c, locs, y, beta, target_s=6.0):
     }
 
 
+def uniform_leg(vals, ksp, n: int, width: float, kp: dict, steps: int = 5) -> dict:
+    """The 3-D radial workload's sizes with the points uniform in the cube and
+    listed in random order (a scattered caller order): interpolation through
+    the point-major scratch (what the plan picks for such orders) and with
+    channel-major stores, gridding, and the same points listed in bin order."""
+    import torch
+
+    import torchlinops_amd.functional as F
+    from torchlinops_amd import config
+    from torchlinops_amd.functional import _plan
+
+    C, M = ksp.shape
+    gen = torch.Generator().manual_seed(5)
+    locs = (torch.rand(M, 3, generator=gen) * n).to(vals.device)
+    cells = torch.floor(locs / 8).long()
+    binned = locs[torch.argsort((cells[:, 0] * 64 + cells[:, 1]) * 64 + cells[:, 2])].contiguous()
+
+    def t_us(fn):
+        fn()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(steps):
+            fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / steps * 1e3
+
+    res = {"workload": f"{C} coils, {n}^3, {M} points uniform in the cube, random order",
+           "order_coherence": _plan.order_coherence(locs, (8, 8, 8)),
+           "bin_ordered_coherence": _plan.order_coherence(binned, (8, 8, 8))}
+    with config.using(interp_plan="always"):
+        res["ungrid_us"] = t_us(lambda: F.interpolate(vals, locs, width, kernel_params=kp))
+        with config.using(interp_point_major="never"):
+            res["ungrid_channel_major_us"] = t_us(
+                lambda: F.interpolate(vals, locs, width, kernel_params=kp))
+        res["grid_us"] = t_us(
+            lambda: F.interpolate_adjoint(ksp, locs, (n, n, n), width, kernel_params=kp))
+        res["bin_ordered_ungrid_us"] = t_us(
+            lambda: F.interpolate(vals, binned, width, kernel_params=kp))
+        res["bin_ordered_grid_us"] = t_us(
+            lambda: F.interpolate_adjoint(ksp, binned, (n, n, n), width, kernel_params=kp))
+    return res
+
+
 def blocks_leg(dev, steps: int = 20):
     """array_to_blocks / blocks_to_array at the 3-D real size: 8 x 128^3
     complex64 (iid CN(0,1)), 8^3 blocks, stride 4 -> 31^3 blocks per batch
@@ -664,6 +709,8 @@ def main() -> int:
     ap.add_argument("--no-blocks", action="store_true",
                     help="skip the array_to_blocks / blocks_to_array leg")
     ap.add_argument("--no-c1", action="store_true", help="skip the 2-D CPU-parity leg")
+    ap.add_argument("--no-uniform", action="store_true",
+                    help="skip the uniform random-order variant of the 3-D workload")
     ap.add_argument("--small", action="store_true", help="reduced size (debug only)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one elementwise copy of the 8 x 256^3 complex64 grid "
@@ -810,6 +857,14 @@ def main() -> int:
             vals.cpu().numpy(), locs.cpu().numpy(), ksp.cpu().numpy(), beta
         )
         log(f"cpu baseline: {out['cpu_baseline']['sample']}")
+    if not args.no_uniform and not args.small:
+        log("uniform: the same sizes, points uniform in the cube in random order ...")
+        out["uniform_random"] = uniform_leg(vals, ksp, n, width, kp)
+        ur = out["uniform_random"]
+        log(f"uniform: ungrid {ur['ungrid_us']:.1f} us (channel-major stores "
+            f"{ur['ungrid_channel_major_us']:.1f} us), grid {ur['grid_us']:.1f} us; "
+            f"bin order: ungrid {ur['bin_ordered_ungrid_us']:.1f} us, "
+            f"grid {ur['bin_ordered_grid_us']:.1f} us")
     if not args.no_blocks and not args.small:
         log("blocks: array_to_blocks / blocks_to_array ...")
         out["blocks"] = blocks_leg(dev)
