@@ -197,5 +197,9 @@ def test_c5_sense_operator_full_configuration(dev):
     print(f"C5 plain CG {plain['iters']} iterations, recomputed residual "
           f"{plain['true_residual']:.2e}; diagonal PCG {pre['iters']} iterations, "
           f"{pre['true_residual']:.2e} (fallback {pre['fallback']})")
-    assert plain["converged"] and plain["true_residual"] < 3e-6
-    assert pre["converged"] and pre["true_residual"] < 3e-6
+    # the recomputed residual differs from CG's recursive 1e-6 by the
+    # float32 operator's own error (3.5e-6 relative against float64, above),
+    # and gridding's split tiles add in atomic order, so it varies from run
+    # to run (2.1e-6 to 3.3e-6 measured): bounded by tol + that error
+    assert plain["converged"] and plain["true_residual"] < 5e-6
+    assert pre["converged"] and pre["true_residual"] < 5e-6

@@ -330,6 +330,9 @@ def test_polynomial_preconditioned_cg(dev):
         lmax = float(torch.linalg.vector_norm(v))
         v = v / lmax
     lam = 1e-2 * lmax
+    from torchlinops_amd.functional import _plan
+
+    print(f"lmax {lmax:.4g}; 2-D tuning {_plan.get_tuning(2)}", flush=True)
     op = lambda x: (N(x) + lam * x) / (lmax + lam)  # spectrum in (0, 1]
     y = torch.randn(grid, dtype=torch.complex64, device=dev, generator=g)
     tol = 1e-4

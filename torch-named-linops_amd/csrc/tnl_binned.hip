@@ -401,14 +401,12 @@ __device__ __forceinline__ void ungrid_sub(
   }
 }
 
-// PM: point-major output (see ungrid_sub), a template argument so the
-// channel-major form carries no per-store select.
-template <int NDIM, bool CPLX, int CC, int NT, bool ILV, bool PM>
+template <int NDIM, bool CPLX, int CC, int NT, bool ILV>
 __global__ __launch_bounds__(BIN_THREADS) void ungrid_binned_kernel(
     const float* __restrict__ vals, float* __restrict__ out, int64_t nbatch,
     InterpGeom G, BinGeom B, const int* __restrict__ perm,
     const float* __restrict__ slocs, const int4* __restrict__ sub,
-    const int* __restrict__ nsub_ptr) {
+    const int* __restrict__ nsub_ptr, bool pm) {
   // The grid is sized for the plan's subproblem capacity; remap over the
   // live count only, so that every XCD gets its share of the real work
   // (hardware hands workgroup b to XCD b % NUM_XCD).
@@ -417,7 +415,7 @@ __global__ __launch_bounds__(BIN_THREADS) void ungrid_binned_kernel(
   if ((int64_t)blockIdx.x >= n8) return;
   const int64_t s = xcd_remap(blockIdx.x, n8);
   if (s >= nsub) return;
-  ungrid_sub<NDIM, CPLX, CC, NT, ILV>(vals, out, nbatch, G, B, perm, slocs, sub[s], PM);
+  ungrid_sub<NDIM, CPLX, CC, NT, ILV>(vals, out, nbatch, G, B, perm, slocs, sub[s], pm);
 }
 
 // Point-major interpolation output (npts x nbatch, from ungrid_sub with pm)
@@ -1706,21 +1704,18 @@ static int launch_binned(bool adjoint, const float* vals, float* out,
   if (!adjoint) {
     const int64_t nwg = (L.sub_capacity + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
     const size_t lds = ungrid_lds_bytes(B, CC, comp_bytes);
-    auto kern = pmb ? ungrid_binned_kernel<NDIM, CPLX, CC, NT, false, true>
-                    : ungrid_binned_kernel<NDIM, CPLX, CC, NT, false, false>;
+    auto kern = ungrid_binned_kernel<NDIM, CPLX, CC, NT, false>;
     if constexpr (NDIM == 3 && CPLX && CC == 2 && NT == 5) {
       bool ilv = true;
       for (int d = 0; d < 3; ++d) ilv = ilv && B.win[d] == ILV_W && G.n[d] >= ILV_W;
       // z pairs of cells: even n[2], 16-byte aligned channel planes
       ilv = ilv && G.n[2] % 2 == 0 && (reinterpret_cast<uintptr_t>(vals) & 15) == 0;
-      if (ilv)
-        kern = pmb ? ungrid_binned_kernel<3, true, 2, 5, true, true>
-                   : ungrid_binned_kernel<3, true, 2, 5, true, false>;
+      if (ilv) kern = ungrid_binned_kernel<3, true, 2, 5, true>;
     }
     if (int rc = allow_lds((const void*)kern, lds)) return rc;
     kern<<<dim3((unsigned)nwg, 1), BIN_THREADS, lds, s>>>(
         vals, pmb ? pmb : out, nbatch, G, B, perm, slocs, (const int4*)(ws + L.off_sub),
-        (const int*)(ws + L.off_suboff) + L.u_nbins + 1);
+        (const int*)(ws + L.off_suboff) + L.u_nbins + 1, pmb != nullptr);
     TNL_LAUNCH_CHECK("ungrid_binned");
     if (pmb) {
       int64_t nwu = (B.npts + UNPK_P - 1) / UNPK_P;
